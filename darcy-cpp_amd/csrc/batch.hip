@@ -1,0 +1,254 @@
+// Persistent solve kernel: each workgroup takes problems from a device work
+// queue and runs the whole solve of one problem -- validation, hierarchy
+// construction, and the preconditioned CG loop with its multigrid cycles --
+// without returning to the host.  Problems of different sizes share one
+// launch (uneven batches); a single problem is a batch of one.
+//
+// Mirrors include/darcy/solver.hpp:83-174 (the PCG driver) per problem, with
+// the batch semantics of src/bindings.cpp:140-197 (independent samples,
+// per-sample diagnostics, invalid samples reported rather than solved).
+// Two additions have no reference counterpart: a warm start u0, and the FP32
+// cycle inside an FP64 flexible CG (beta = -alpha <z_new, A p> / <z, r>,
+// Notay's FCG(1), since r_new - r_old = -alpha A p).
+#include "abi_types.hpp"
+#include "setup.hpp"
+
+namespace darcy_hip {
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / kWave;
+
+struct Shared {
+    Hier h;
+    double red[4];
+    unsigned long long bad;
+    int prob;
+};
+
+// Barrier, then one wave combines the chunk sums of `nd` reductions.
+__device__ inline void reduce_parts(Shared& s, int nd, int C) {
+    __syncthreads();
+    if (threadIdx.x < kWave) {
+        for (int d = 0; d < nd; ++d) {
+            const double v = final_sum(s.h.part[d], C);
+            if (threadIdx.x == 0) s.red[d] = v;
+        }
+    }
+    __syncthreads();
+}
+
+template <class T>
+__device__ inline void precondition(const Hier& h, const SolveParams& P, int N, Exec e) {
+    if (P.precond == kPcMultigrid || P.mode == kModeApplyPreconditioner) {
+        block_cycle<T>(h, 0, true, CycleOpts{P.pre, P.post, P.gamma}, e);
+    } else if (P.precond == kPcJacobi) {  // solver.hpp:127-132
+        for (int k = e.tid; k < N; k += e.nthr) h.z[k] = h.r[k] * h.id[0][k];
+        __syncthreads();
+    } else {  // solver.hpp:124-126
+        for (int k = e.tid; k < N; k += e.nthr) h.z[k] = h.r[k];
+        __syncthreads();
+    }
+}
+
+template <class T>
+__device__ void run_problem(Shared& s, const Problem& pb, ProblemStats& st, const SolveParams& P, char* slot) {
+    const Exec e{static_cast<int>(threadIdx.x), kBlock};
+    const int wave = threadIdx.x / kWave;
+    const int n = pb.n, N = n * n, C = chunks_of(N);
+    constexpr bool mixed = sizeof(T) == 4;
+
+    // Validation (solver.hpp:68-77): the first offending flat index.
+    if (threadIdx.x == 0) s.bad = ~0ull;
+    __syncthreads();
+    const long long bad = first_invalid(pb.a, N, e);
+    if (bad != INT64_MAX) atomicMin(&s.bad, static_cast<unsigned long long>(bad));
+    __syncthreads();
+    if (s.bad != ~0ull) {
+        if (threadIdx.x == 0) st = ProblemStats{0, kInvalid, 1, 0, 0.0, static_cast<long long>(s.bad)};
+        return;
+    }
+
+    const bool mg = P.mode == kModeApplyPreconditioner || (P.mode == kModeSolve && P.precond == kPcMultigrid);
+    if (threadIdx.x == 0) plan_slot(slot, n, mixed, &s.h);
+    __syncthreads();
+    const Hier& h = s.h;
+
+    // Hierarchy (level.hpp:79-97, multigrid.hpp:55-71, level.hpp:171-211).
+    build_fine_faces(pb.a, n, h.ex[0], h.ny[0], e);
+    build_fine_invdiag(pb.a, n, h.id[0], e);
+    __syncthreads();
+    if (mg) {
+        for (int l = 0; l + 1 < h.L; ++l) {
+            galerkin_faces(h.n[l], h.ex[l], h.ny[l], h.ex[l + 1], h.ny[l + 1], e);
+            galerkin_invdiag(h.n[l], h.ex[l], h.ny[l], h.id[l + 1], e);
+            if (mixed) to_fp32(h.n[l], h.ex[l], h.ny[l], h.id[l], h.ex32[l], h.ny32[l], h.id32[l], e);
+            __syncthreads();
+        }
+        if (mixed) {
+            const int l = h.L - 1;
+            to_fp32(h.n[l], h.ex[l], h.ny[l], h.id[l], h.ex32[l], h.ny32[l], h.id32[l], e);
+            __syncthreads();
+        }
+    }
+    const Lv<double> A0 = level_view<double>(h, 0);
+    float* const b0f = mixed ? static_cast<float*>(h.b[0]) : nullptr;
+    const float* const x0f = mixed ? static_cast<const float*>(h.x[0]) : nullptr;
+    auto zval = [&](int k) -> double { return mixed ? static_cast<double>(x0f[k]) : h.z[k]; };
+
+    if (P.mode == kModeApplyOperator) {  // bindings.cpp:199-216
+        for (int k = e.tid; k < N; k += e.nthr) {
+            const int i = k / n, j = k - i * n;
+            pb.out[k] = apply_at(A0, i, j, [&](int t) { return pb.in[t]; });
+        }
+        if (threadIdx.x == 0) st = ProblemStats{0, kConverged, 1, 0, 0.0, -1};
+        return;
+    }
+    if (P.mode == kModeApplyPreconditioner) {  // bindings.cpp:218-245
+        for (int k = e.tid; k < N; k += e.nthr) {
+            h.r[k] = pb.in[k];
+            if (mixed) b0f[k] = static_cast<float>(pb.in[k]);
+        }
+        __syncthreads();
+        precondition<T>(h, P, N, e);
+        for (int k = e.tid; k < N; k += e.nthr) pb.out[k] = zval(k);
+        if (threadIdx.x == 0) st = ProblemStats{0, kConverged, h.L, 0, 0.0, -1};
+        return;
+    }
+
+    // b = f h^2 (solver.hpp:92-102); r = b - A u0 (warm start) or b.
+    const double hh = 1.0 / static_cast<double>(n + 1);
+    const double h2 = hh * hh;
+    double* const u = pb.out;
+    chunk_sums<2>(N, wave, kWaves, h.part, [&](int k, double* acc) {
+        const double bk = (pb.rhs ? pb.rhs[k] : pb.f) * h2;
+        acc[0] += bk * bk;
+        double rk = bk;
+        if (pb.in) {
+            const int i = k / n, j = k - i * n;
+            rk = bk - apply_at(A0, i, j, [&](int t) { return pb.in[t]; });
+            u[k] = pb.in[k];
+        } else {
+            u[k] = 0.0;
+        }
+        h.r[k] = rk;
+        acc[1] += rk * rk;
+        if (mixed) b0f[k] = static_cast<float>(rk);
+    });
+    reduce_parts(s, 2, C);
+    const double bnorm = sqrt(s.red[0]);
+    const int levels = mg ? h.L : 1;
+    if (bnorm == 0.0) {  // solver.hpp:105-108: f == 0 gives u == 0 exactly
+        for (int k = e.tid; k < N; k += e.nthr) u[k] = 0.0;
+        if (threadIdx.x == 0) st = ProblemStats{0, kConverged, 1, 0, 0.0, -1};
+        return;
+    }
+    double res = pb.in ? sqrt(s.red[1]) / bnorm : 1.0;
+    const bool hist = P.record_history && pb.history;
+    if (hist && threadIdx.x == 0) pb.history[0] = res;
+    if (pb.in && res < P.tol) {
+        if (threadIdx.x == 0) st = ProblemStats{0, kConverged, levels, hist ? 1 : 0, res, -1};
+        return;
+    }
+
+    precondition<T>(h, P, N, e);
+    chunk_sums<1>(N, wave, kWaves, h.part, [&](int k, double* acc) { acc[0] += h.r[k] * zval(k); });
+    reduce_parts(s, 1, C);
+    double rz = s.red[0];
+
+    double* pold = h.p;
+    double* pnew = h.u;  // the slot's u is free: the solution lives in pb.out
+    double beta = 0.0;
+    bool first = true;
+    int status = kMaxIter, iters = 0;
+    for (int it = 0; it < P.max_iter; ++it) {
+        // p = z + beta p (solver.hpp:171, blas.hpp:50-56), fused with
+        // ap = A p (solver.hpp:147) and <p, ap> (solver.hpp:148).
+        chunk_sums<1>(N, wave, kWaves, h.part, [&](int k, double* acc) {
+            const int i = k / n, j = k - i * n;
+            auto pv = [&](int t) -> double { return first ? zval(t) : zval(t) + beta * pold[t]; };
+            const double v = apply_at(A0, i, j, pv);
+            const double pk = pv(k);
+            pnew[k] = pk;
+            h.ap[k] = v;
+            acc[0] += pk * v;
+        });
+        reduce_parts(s, 1, C);
+        const double pap = s.red[0];
+        if (!(fabs(pap) > 0.0) || !isfinite(pap)) {  // solver.hpp:149-155
+            status = kBreakdown;
+            iters = it;
+            break;
+        }
+        const double alpha = rz / pap;
+        // u += alpha p; r -= alpha ap; <r, r>  (blas.hpp:62-77)
+        chunk_sums<1>(N, wave, kWaves, h.part, [&](int k, double* acc) {
+            u[k] += alpha * pnew[k];
+            const double ri = h.r[k] - alpha * h.ap[k];
+            h.r[k] = ri;
+            acc[0] += ri * ri;
+            if (mixed) b0f[k] = static_cast<float>(ri);
+        });
+        reduce_parts(s, 1, C);
+        res = sqrt(s.red[0]) / bnorm;
+        iters = it + 1;
+        if (hist && threadIdx.x == 0) pb.history[iters] = res;
+        if (res < P.tol) {
+            status = kConverged;
+            break;
+        }
+        if (iters == P.max_iter) break;
+        precondition<T>(h, P, N, e);
+        chunk_sums<2>(N, wave, kWaves, h.part, [&](int k, double* acc) {
+            const double zk = zval(k);
+            acc[0] += h.r[k] * zk;
+            if (mixed) acc[1] += zk * h.ap[k];
+        });
+        reduce_parts(s, mixed ? 2 : 1, C);
+        beta = mixed ? (-alpha * s.red[1]) / rz : s.red[0] / rz;
+        rz = s.red[0];
+        double* t = pold;
+        pold = pnew;
+        pnew = t;
+        first = false;
+    }
+    if (threadIdx.x == 0)
+        st = ProblemStats{iters, status, levels, hist ? iters + 1 : 0, res, -1};
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_batch(BatchArgs A) {
+    __shared__ Shared s;
+    char* slot = A.slots + static_cast<size_t>(blockIdx.x) * A.slot_bytes;
+    for (;;) {
+        if (threadIdx.x == 0) s.prob = static_cast<int>(atomicAdd(A.next, 1u));
+        __syncthreads();
+        const int t = s.prob;
+        if (t >= A.count) return;
+        const int p = A.order ? A.order[t] : t;
+        run_problem<T>(s, A.probs[p], A.stats[p], A.prm, slot);
+        __syncthreads();
+    }
+}
+
+template __global__ void k_batch<double>(BatchArgs);
+template __global__ void k_batch<float>(BatchArgs);
+
+}  // namespace darcy_hip
+
+// Launch entry points for the host code of this library.
+extern "C" hipError_t darcy_hip_launch_batch(const darcy_hip::BatchArgs* args, int grid, int mixed,
+                                             hipStream_t stream) {
+    if (mixed)
+        hipLaunchKernelGGL(darcy_hip::k_batch<float>, dim3(grid), dim3(darcy_hip::kBlock), 0, stream, *args);
+    else
+        hipLaunchKernelGGL(darcy_hip::k_batch<double>, dim3(grid), dim3(darcy_hip::kBlock), 0, stream, *args);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t darcy_hip_batch_occupancy(int mixed, int* blocks_per_cu) {
+    if (mixed)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, darcy_hip::k_batch<float>,
+                                                            darcy_hip::kBlock, 0);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, darcy_hip::k_batch<double>,
+                                                        darcy_hip::kBlock, 0);
+}

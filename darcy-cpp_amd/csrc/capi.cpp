@@ -1,0 +1,450 @@
+// Host side of the C ABI (include/darcy_hip.h): argument checks, device
+// workspace, staging of host buffers, kernel launches and result readback.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/darcy_hip.h"
+#include "abi_types.hpp"
+
+using namespace darcy_hip;
+
+extern "C" hipError_t darcy_hip_launch_batch(const BatchArgs* args, int grid, int mixed, hipStream_t stream);
+extern "C" hipError_t darcy_hip_batch_occupancy(int mixed, int* blocks_per_cu);
+
+namespace {
+
+// A growable device allocation owned by the context.
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    hipError_t reserve(size_t want) {
+        if (want <= bytes) return hipSuccess;
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+        const hipError_t e = hipMalloc(&ptr, want);
+        if (e == hipSuccess) bytes = want;
+        return e;
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    template <class T>
+    T* as() const {
+        return static_cast<T*>(ptr);
+    }
+};
+
+struct Failure {
+    int code;
+    std::string msg;
+};
+
+}  // namespace
+
+struct darcy_ctx {
+    int device = 0;
+    int cus = 0;
+    std::string arch;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+    std::string error;
+    std::mutex mu;
+    DevBuf slots, probs, stats, order, counter, staging;
+};
+
+namespace {
+
+int fail(darcy_ctx* c, int code, const std::string& msg) {
+    if (c) c->error = msg;
+    return code;
+}
+
+int hip_fail(darcy_ctx* c, hipError_t e, const char* what) {
+    const int code = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? DARCY_OUT_OF_MEMORY : DARCY_HIP_ERROR;
+    return fail(c, code, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr, what)                           \
+    do {                                              \
+        const hipError_t e_ = (expr);                 \
+        if (e_ != hipSuccess) return hip_fail(ctx, e_, what); \
+    } while (0)
+
+// Option checks in the order and wording of the reference
+// (solver.hpp:85-87, bindings.cpp:43-46).
+int check_opts(darcy_ctx* ctx, const darcy_opts& o) {
+    if (!(o.tol > 0.0)) return fail(ctx, DARCY_INVALID_INPUT, "tol must be positive");
+    if (o.max_iter < 1) return fail(ctx, DARCY_INVALID_INPUT, "max_iter must be at least 1");
+    if (o.preconditioner < DARCY_PC_NONE || o.preconditioner > DARCY_PC_MULTIGRID)
+        return fail(ctx, DARCY_INVALID_INPUT, "unknown preconditioner; expected one of: multigrid, jacobi, none");
+    if (o.pre_sweeps < 1 || o.post_sweeps < 1)
+        return fail(ctx, DARCY_INVALID_INPUT, "pre_sweeps and post_sweeps must be at least 1");
+    if (o.cycle_index < 1 || o.cycle_index > 2)
+        return fail(ctx, DARCY_INVALID_INPUT, "cycle_index must be 1 (V-cycle) or 2 (W-cycle)");
+    if (o.precision != DARCY_FP64 && o.precision != DARCY_MIXED)
+        return fail(ctx, DARCY_INVALID_INPUT, "precision must be 0 (fp64) or 1 (mixed)");
+    return DARCY_OK;
+}
+
+// Largest grid side one call may use: flat indices and face counts are
+// 32-bit inside the kernels.
+constexpr int64_t kMaxSide = 32768;
+
+struct Job {
+    int mode;
+    const darcy_opts* opts;
+    int64_t count;
+    const int64_t* n;
+    const double* const* a;
+    const double* const* rhs;
+    const double* f;
+    const double* const* in;
+    double* const* out;
+    darcy_stats* stats;
+    double* history;  // single-problem history (host or device per opts)
+    int32_t history_cap;
+    bool device_ptrs;
+};
+
+int run(darcy_ctx* ctx, const Job& job) {
+    const darcy_opts& o = *job.opts;
+    const bool mg = job.mode == kModeApplyPreconditioner || o.preconditioner == DARCY_PC_MULTIGRID;
+    const bool mixed = mg && o.precision == DARCY_MIXED;
+    const int64_t count = job.count;
+    if (count == 0) return DARCY_OK;
+    int nmax = 0;
+    for (int64_t i = 0; i < count; ++i) {
+        if (job.n[i] < 1) return fail(ctx, DARCY_INVALID_INPUT, "grid size must be at least 1");
+        if (job.n[i] > kMaxSide) return fail(ctx, DARCY_INVALID_INPUT, "grid size exceeds 32768");
+        if (!job.a[i] || !job.out[i]) return fail(ctx, DARCY_INVALID_INPUT, "null array argument");
+        nmax = std::max<int>(nmax, static_cast<int>(job.n[i]));
+    }
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+
+    // Device copies of host inputs; identical host pointers (a shared rhs)
+    // are uploaded once.
+    std::vector<Problem> probs(static_cast<size_t>(count));
+    const bool hist = job.mode == kModeSolve && o.record_history && job.history && count == 1;
+    size_t stage_bytes = 0;
+    std::unordered_map<const void*, size_t> staged;
+    std::vector<std::pair<const void*, size_t>> uploads;  // host ptr, bytes
+    auto plan_upload = [&](const double* p, size_t cells) -> size_t {
+        auto it = staged.find(p);
+        if (it != staged.end()) return it->second;
+        const size_t off = stage_bytes;
+        stage_bytes += align_up(cells * 8);
+        staged.emplace(p, off);
+        uploads.emplace_back(p, cells * 8);
+        return off;
+    };
+    std::vector<size_t> off_a(count), off_rhs(count, SIZE_MAX), off_in(count, SIZE_MAX), off_out(count);
+    if (!job.device_ptrs) {
+        for (int64_t i = 0; i < count; ++i) {
+            const size_t cells = static_cast<size_t>(job.n[i] * job.n[i]);
+            off_a[i] = plan_upload(job.a[i], cells);
+            if (job.rhs && job.rhs[i]) off_rhs[i] = plan_upload(job.rhs[i], cells);
+            if (job.in && job.in[i]) off_in[i] = plan_upload(job.in[i], cells);
+        }
+        for (int64_t i = 0; i < count; ++i) {
+            off_out[i] = stage_bytes;
+            stage_bytes += align_up(static_cast<size_t>(job.n[i] * job.n[i]) * 8);
+        }
+    }
+    size_t hist_off = stage_bytes;
+    if (hist && !job.device_ptrs) stage_bytes += align_up(static_cast<size_t>(o.max_iter + 1) * 8);
+    HIP_TRY(ctx->staging.reserve(std::max<size_t>(stage_bytes, 256)), "staging allocation");
+    char* stage = ctx->staging.as<char>();
+    for (const auto& up : uploads)
+        HIP_TRY(hipMemcpyAsync(stage + staged[up.first], up.first, up.second, hipMemcpyHostToDevice, ctx->stream),
+                "upload");
+
+    for (int64_t i = 0; i < count; ++i) {
+        Problem& p = probs[i];
+        p.n = static_cast<int32_t>(job.n[i]);
+        p.pad = 0;
+        p.f = job.f ? job.f[i] : 1.0;
+        if (job.device_ptrs) {
+            p.a = job.a[i];
+            p.rhs = job.rhs ? job.rhs[i] : nullptr;
+            p.in = job.in ? job.in[i] : nullptr;
+            p.out = job.out[i];
+            p.history = hist ? job.history : nullptr;
+        } else {
+            p.a = reinterpret_cast<const double*>(stage + off_a[i]);
+            p.rhs = off_rhs[i] != SIZE_MAX ? reinterpret_cast<const double*>(stage + off_rhs[i]) : nullptr;
+            p.in = off_in[i] != SIZE_MAX ? reinterpret_cast<const double*>(stage + off_in[i]) : nullptr;
+            p.out = reinterpret_cast<double*>(stage + off_out[i]);
+            p.history = hist ? reinterpret_cast<double*>(stage + hist_off) : nullptr;
+        }
+    }
+
+    // Largest problems first, so the tail of the queue holds the short jobs.
+    std::vector<int32_t> order(static_cast<size_t>(count));
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return job.n[x] > job.n[y]; });
+
+    int per_cu = 1;
+    HIP_TRY(darcy_hip_batch_occupancy(mixed, &per_cu), "occupancy query");
+    per_cu = std::max(per_cu, 1);
+    Hier scratch_h;
+    const size_t slot_bytes = plan_slot(nullptr, nmax, mixed, &scratch_h);
+    int64_t grid = std::min<int64_t>(count, static_cast<int64_t>(ctx->cus) * per_cu);
+    // Keep the workspace within a quarter of free device memory.
+    size_t free_b = 0, total_b = 0;
+    HIP_TRY(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+    const size_t budget = std::max<size_t>(free_b / 4 + ctx->slots.bytes, slot_bytes);
+    grid = std::max<int64_t>(1, std::min<int64_t>(grid, static_cast<int64_t>(budget / slot_bytes)));
+
+    HIP_TRY(ctx->slots.reserve(static_cast<size_t>(grid) * slot_bytes), "workspace allocation");
+    HIP_TRY(ctx->probs.reserve(sizeof(Problem) * count), "problem table");
+    HIP_TRY(ctx->stats.reserve(sizeof(ProblemStats) * count), "stats table");
+    HIP_TRY(ctx->order.reserve(sizeof(int32_t) * count), "order table");
+    HIP_TRY(ctx->counter.reserve(256), "counter");
+    HIP_TRY(hipMemcpyAsync(ctx->probs.ptr, probs.data(), sizeof(Problem) * count, hipMemcpyHostToDevice, ctx->stream),
+            "problem upload");
+    HIP_TRY(hipMemcpyAsync(ctx->order.ptr, order.data(), sizeof(int32_t) * count, hipMemcpyHostToDevice, ctx->stream),
+            "order upload");
+    HIP_TRY(hipMemsetAsync(ctx->counter.ptr, 0, 256, ctx->stream), "counter reset");
+
+    BatchArgs args{};
+    args.probs = ctx->probs.as<Problem>();
+    args.stats = ctx->stats.as<ProblemStats>();
+    args.order = ctx->order.as<int32_t>();
+    args.next = ctx->counter.as<uint32_t>();
+    args.slots = ctx->slots.as<char>();
+    args.slot_bytes = slot_bytes;
+    args.count = static_cast<int32_t>(count);
+    args.prm.tol = o.tol;
+    args.prm.max_iter = o.max_iter;
+    args.prm.precond = o.preconditioner;
+    args.prm.pre = o.pre_sweeps;
+    args.prm.post = o.post_sweeps;
+    args.prm.gamma = o.cycle_index;
+    args.prm.mixed = mixed ? 1 : 0;
+    args.prm.mode = job.mode;
+    args.prm.record_history = hist ? 1 : 0;
+
+    HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream), "event record");
+    HIP_TRY(darcy_hip_launch_batch(&args, static_cast<int>(grid), mixed, ctx->stream), "solve kernel launch");
+    HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream), "event record");
+
+    std::vector<ProblemStats> st(static_cast<size_t>(count));
+    HIP_TRY(hipMemcpyAsync(st.data(), ctx->stats.ptr, sizeof(ProblemStats) * count, hipMemcpyDeviceToHost, ctx->stream),
+            "stats readback");
+    if (!job.device_ptrs) {
+        for (int64_t i = 0; i < count; ++i)
+            HIP_TRY(hipMemcpyAsync(job.out[i], stage + off_out[i], static_cast<size_t>(job.n[i] * job.n[i]) * 8,
+                                   hipMemcpyDeviceToHost, ctx->stream),
+                    "result readback");
+    }
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "solve");
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "event timing");
+    ctx->last_ms = ms;
+    if (hist && !job.device_ptrs && st[0].status != kInvalid) {
+        const int32_t len = std::min<int32_t>(st[0].history_len, job.history_cap);
+        if (len > 0)
+            HIP_TRY(hipMemcpy(job.history, stage + hist_off, static_cast<size_t>(len) * 8, hipMemcpyDeviceToHost),
+                    "history readback");
+    }
+
+    int64_t first_bad_problem = -1;
+    for (int64_t i = 0; i < count; ++i) {
+        const ProblemStats& s = st[i];
+        if (job.stats) {
+            darcy_stats& d = job.stats[i];
+            d.iterations = s.iterations;
+            d.converged = s.status == kConverged ? 1 : 0;
+            d.levels = s.levels;
+            d.status = s.status;
+            d.residual = s.residual;
+            d.bad_index = s.status == kInvalid ? s.bad_index : -1;
+            d.history_len = std::min<int32_t>(s.history_len, job.history_cap);
+            d.reserved = 0;
+        }
+        if (s.status == kInvalid && first_bad_problem < 0) first_bad_problem = i;
+    }
+    if (first_bad_problem >= 0) {
+        char msg[256];
+        std::snprintf(msg, sizeof msg,
+                      "permeability must be finite and strictly positive everywhere; offending value at flat index "
+                      "%lld%s",
+                      static_cast<long long>(st[first_bad_problem].bad_index),
+                      count > 1 ? (" of problem " + std::to_string(first_bad_problem)).c_str() : "");
+        return fail(ctx, DARCY_INVALID_INPUT, msg);
+    }
+    ctx->error.clear();
+    return DARCY_OK;
+}
+
+}  // namespace
+
+extern "C" void darcy_default_opts(darcy_opts* o) {
+    if (!o) return;
+    o->tol = 1e-10;
+    o->max_iter = 10000;
+    o->preconditioner = DARCY_PC_MULTIGRID;
+    o->pre_sweeps = 1;
+    o->post_sweeps = 1;
+    o->cycle_index = 2;
+    o->record_history = 0;
+    o->precision = DARCY_FP64;
+    o->device_pointers = 0;
+}
+
+extern "C" int darcy_ctx_create(int device, darcy_ctx** out) {
+    if (!out) return DARCY_INVALID_INPUT;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return DARCY_HIP_ERROR;
+    if (device < 0 || device >= ndev) return DARCY_INVALID_INPUT;
+    auto* c = new darcy_ctx();
+    c->device = device;
+    hipDeviceProp_t prop;
+    if (hipSetDevice(device) != hipSuccess || hipGetDeviceProperties(&prop, device) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        darcy_ctx_destroy(c);
+        return DARCY_HIP_ERROR;
+    }
+    c->cus = prop.multiProcessorCount;
+    c->arch = prop.gcnArchName;
+    *out = c;
+    return DARCY_OK;
+}
+
+extern "C" void darcy_ctx_destroy(darcy_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (DevBuf* b : {&c->slots, &c->probs, &c->stats, &c->order, &c->counter, &c->staging}) b->release();
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" const char* darcy_last_error(const darcy_ctx* c) { return c ? c->error.c_str() : "null context"; }
+
+extern "C" int darcy_solve(darcy_ctx* ctx, const double* a, const double* rhs, double f, int64_t n,
+                           const darcy_opts* opts, const double* u0, double* u_out, darcy_stats* stats,
+                           double* history, int32_t history_cap) {
+    if (!ctx) return DARCY_INVALID_INPUT;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (n < 1) return fail(ctx, DARCY_INVALID_INPUT, "grid size must be at least 1");
+    darcy_opts o;
+    darcy_default_opts(&o);
+    if (opts) o = *opts;
+    if (const int rc = check_opts(ctx, o)) return rc;
+    Job job{kModeSolve, &o, 1, &n, &a, &rhs, &f, &u0, &u_out, stats, history, history ? history_cap : 0,
+            o.device_pointers != 0};
+    return run(ctx, job);
+}
+
+extern "C" int darcy_solve_batch(darcy_ctx* ctx, int64_t count, const int64_t* n, const double* const* a,
+                                 const double* const* rhs, const double* f, const darcy_opts* opts,
+                                 const double* const* u0, double* const* u_out, darcy_stats* stats) {
+    if (!ctx) return DARCY_INVALID_INPUT;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (count < 0 || (count > 0 && (!n || !a || !u_out)))
+        return fail(ctx, DARCY_INVALID_INPUT, "a must have shape (batch, n, n)");
+    if (count > INT32_MAX) return fail(ctx, DARCY_INVALID_INPUT, "batch too large");
+    darcy_opts o;
+    darcy_default_opts(&o);
+    if (opts) o = *opts;
+    if (const int rc = check_opts(ctx, o)) return rc;
+    Job job{kModeSolve, &o, count, n, a, rhs, f, u0, u_out, stats, nullptr, 0, o.device_pointers != 0};
+    return run(ctx, job);
+}
+
+extern "C" int darcy_apply_operator(darcy_ctx* ctx, const double* a, int64_t n, const double* x, double* y,
+                                    int32_t device_pointers) {
+    if (!ctx) return DARCY_INVALID_INPUT;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (!x) return fail(ctx, DARCY_INVALID_INPUT, "x must have the same shape as a");
+    darcy_opts o;
+    darcy_default_opts(&o);
+    o.device_pointers = device_pointers;
+    Job job{kModeApplyOperator, &o, 1, &n, &a, nullptr, nullptr, &x, &y, nullptr, nullptr, 0, device_pointers != 0};
+    return run(ctx, job);
+}
+
+extern "C" int darcy_apply_preconditioner(darcy_ctx* ctx, const double* a, int64_t n, const double* r, double* z,
+                                          int32_t pre_sweeps, int32_t post_sweeps, int32_t cycle_index,
+                                          int32_t precision, int32_t device_pointers) {
+    if (!ctx) return DARCY_INVALID_INPUT;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (!r) return fail(ctx, DARCY_INVALID_INPUT, "r must have the same shape as a");
+    darcy_opts o;
+    darcy_default_opts(&o);
+    o.pre_sweeps = pre_sweeps;
+    o.post_sweeps = post_sweeps;
+    o.cycle_index = cycle_index;
+    o.precision = precision;
+    o.device_pointers = device_pointers;
+    if (pre_sweeps < 1 || post_sweeps < 1) return fail(ctx, DARCY_INVALID_INPUT, "sweeps must be at least 1");
+    if (cycle_index < 1 || cycle_index > 2) return fail(ctx, DARCY_INVALID_INPUT, "cycle_index must be 1 or 2");
+    if (const int rc = check_opts(ctx, o)) return rc;
+    Job job{kModeApplyPreconditioner, &o, 1, &n, &a, nullptr, nullptr, &r, &z, nullptr, nullptr, 0,
+            device_pointers != 0};
+    return run(ctx, job);
+}
+
+extern "C" int32_t darcy_num_levels(int64_t n) {
+    if (n < 1) return 0;
+    return level_count(static_cast<int>(std::min<int64_t>(n, INT32_MAX)));
+}
+
+extern "C" int darcy_device_alloc(darcy_ctx* ctx, size_t bytes, void** out) {
+    if (!ctx || !out) return DARCY_INVALID_INPUT;
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    HIP_TRY(hipMalloc(out, bytes), "hipMalloc");
+    return DARCY_OK;
+}
+
+extern "C" int darcy_device_free(darcy_ctx* ctx, void* p) {
+    if (!ctx) return DARCY_INVALID_INPUT;
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    HIP_TRY(hipFree(p), "hipFree");
+    return DARCY_OK;
+}
+
+extern "C" int darcy_copy_to_device(darcy_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx) return DARCY_INVALID_INPUT;
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy H2D");
+    return DARCY_OK;
+}
+
+extern "C" int darcy_copy_to_host(darcy_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx) return DARCY_INVALID_INPUT;
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    return DARCY_OK;
+}
+
+extern "C" int darcy_last_kernel_ms(const darcy_ctx* ctx, double* ms) {
+    if (!ctx || !ms) return DARCY_INVALID_INPUT;
+    *ms = ctx->last_ms;
+    return DARCY_OK;
+}
+
+extern "C" int darcy_device_info(const darcy_ctx* ctx, char* buf, int32_t cap) {
+    if (!ctx || !buf || cap <= 0) return DARCY_INVALID_INPUT;
+    int rt = 0;
+    (void)hipRuntimeGetVersion(&rt);
+    std::snprintf(buf, static_cast<size_t>(cap), "%s, %d CUs, HIP runtime %d", ctx->arch.c_str(), ctx->cus, rt);
+    return DARCY_OK;
+}
