@@ -252,3 +252,18 @@ extern "C" hipError_t darcy_hip_batch_occupancy(int mixed, int* blocks_per_cu) {
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, darcy_hip::k_batch<double>,
                                                         darcy_hip::kBlock, 0);
 }
+
+// Streaming copy of `n` doubles: the reference kernel for achievable HBM
+// bandwidth and for scaling the byte counters of the profiler.
+__global__ __launch_bounds__(256) void k_stream_copy(const double2* __restrict__ src, double2* __restrict__ dst,
+                                                      size_t n2) {
+    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n2;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x)
+        dst[i] = src[i];
+}
+
+extern "C" hipError_t darcy_hip_launch_copy(const void* src, void* dst, size_t bytes, int grid, hipStream_t stream) {
+    hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, stream, static_cast<const double2*>(src),
+                       static_cast<double2*>(dst), bytes / sizeof(double2));
+    return hipGetLastError();
+}

@@ -18,6 +18,7 @@ using namespace darcy_hip;
 
 extern "C" hipError_t darcy_hip_launch_batch(const BatchArgs* args, int grid, int mixed, hipStream_t stream);
 extern "C" hipError_t darcy_hip_batch_occupancy(int mixed, int* blocks_per_cu);
+extern "C" hipError_t darcy_hip_launch_copy(const void* src, void* dst, size_t bytes, int grid, hipStream_t stream);
 
 namespace {
 
@@ -446,5 +447,34 @@ extern "C" int darcy_device_info(const darcy_ctx* ctx, char* buf, int32_t cap) {
     int rt = 0;
     (void)hipRuntimeGetVersion(&rt);
     std::snprintf(buf, static_cast<size_t>(cap), "%s, %d CUs, HIP runtime %d", ctx->arch.c_str(), ctx->cus, rt);
+    return DARCY_OK;
+}
+
+extern "C" int darcy_bandwidth_probe(darcy_ctx* ctx, size_t bytes, int32_t repeats, double* gbps) {
+    if (!ctx || !gbps || repeats < 1 || bytes < 16) return DARCY_INVALID_INPUT;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    bytes &= ~static_cast<size_t>(15);
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    void* src = nullptr;
+    void* dst = nullptr;
+    HIP_TRY(hipMalloc(&src, bytes), "probe allocation");
+    const hipError_t e = hipMalloc(&dst, bytes);
+    if (e != hipSuccess) {
+        (void)hipFree(src);
+        return hip_fail(ctx, e, "probe allocation");
+    }
+    const int grid = ctx->cus * 8;
+    hipError_t err = hipMemsetAsync(src, 0, bytes, ctx->stream);
+    if (err == hipSuccess) err = darcy_hip_launch_copy(src, dst, bytes, grid, ctx->stream);  // warm-up
+    if (err == hipSuccess) err = hipEventRecord(ctx->ev0, ctx->stream);
+    for (int32_t r = 0; r < repeats && err == hipSuccess; ++r) err = darcy_hip_launch_copy(src, dst, bytes, grid, ctx->stream);
+    if (err == hipSuccess) err = hipEventRecord(ctx->ev1, ctx->stream);
+    if (err == hipSuccess) err = hipStreamSynchronize(ctx->stream);
+    float ms = 0.f;
+    if (err == hipSuccess) err = hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    (void)hipFree(src);
+    (void)hipFree(dst);
+    if (err != hipSuccess) return hip_fail(ctx, err, "bandwidth probe");
+    *gbps = 2.0 * static_cast<double>(bytes) * repeats / (ms * 1e-3) / 1e9;
     return DARCY_OK;
 }

@@ -1,0 +1,171 @@
+"""ctypes binding of the C ABI (include/darcy_hip.h).
+
+The pybind11 module covers the reference's API on host arrays.  This binding
+exposes the rest of the boundary -- device-resident inputs, uneven batches
+given as device pointer tables, kernel timing -- for callers that keep data
+in HBM (the benchmark, data generation).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Sequence
+
+import numpy as np
+
+_LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libdarcy_hip.so")
+
+OK, INVALID_INPUT, HIP_ERROR, OUT_OF_MEMORY = 0, 1, 2, 3
+PC = {"none": 0, "jacobi": 1, "multigrid": 2, "mg": 2}
+PRECISION = {"fp64": 0, "mixed": 1}
+
+
+class Opts(C.Structure):
+    _fields_ = [
+        ("tol", C.c_double),
+        ("max_iter", C.c_int32),
+        ("preconditioner", C.c_int32),
+        ("pre_sweeps", C.c_int32),
+        ("post_sweeps", C.c_int32),
+        ("cycle_index", C.c_int32),
+        ("record_history", C.c_int32),
+        ("precision", C.c_int32),
+        ("device_pointers", C.c_int32),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int32),
+        ("converged", C.c_int32),
+        ("levels", C.c_int32),
+        ("status", C.c_int32),
+        ("residual", C.c_double),
+        ("bad_index", C.c_int64),
+        ("history_len", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+def _load() -> C.CDLL:
+    if not os.path.exists(_LIB_PATH):
+        raise ImportError(f"{_LIB_PATH} is not built (python darcy-cpp_amd/build.py)")
+    lib = C.CDLL(_LIB_PATH)
+    vp, i64, i32, dp = C.c_void_p, C.c_int64, C.c_int32, C.POINTER(C.c_double)
+    lib.darcy_default_opts.argtypes = [C.POINTER(Opts)]
+    lib.darcy_ctx_create.argtypes = [C.c_int, C.POINTER(vp)]
+    lib.darcy_ctx_destroy.argtypes = [vp]
+    lib.darcy_last_error.argtypes = [vp]
+    lib.darcy_last_error.restype = C.c_char_p
+    lib.darcy_solve_batch.argtypes = [vp, i64, C.POINTER(i64), C.POINTER(vp), C.POINTER(vp), dp, C.POINTER(Opts),
+                                      C.POINTER(vp), C.POINTER(vp), C.POINTER(Stats)]
+    lib.darcy_device_alloc.argtypes = [vp, C.c_size_t, C.POINTER(vp)]
+    lib.darcy_device_free.argtypes = [vp, vp]
+    lib.darcy_copy_to_device.argtypes = [vp, vp, vp, C.c_size_t]
+    lib.darcy_copy_to_host.argtypes = [vp, vp, vp, C.c_size_t]
+    lib.darcy_last_kernel_ms.argtypes = [vp, dp]
+    lib.darcy_device_info.argtypes = [vp, C.c_char_p, i32]
+    lib.darcy_bandwidth_probe.argtypes = [vp, C.c_size_t, i32, dp]
+    return lib
+
+
+class Context:
+    """A darcy_ctx: one device, one stream, its workspace."""
+
+    def __init__(self, device: int = 0):
+        self.lib = _load()
+        self.ptr = C.c_void_p()
+        rc = self.lib.darcy_ctx_create(device, C.byref(self.ptr))
+        if rc != OK:
+            raise RuntimeError(f"darcy_ctx_create failed (code {rc}): no usable HIP device")
+
+    def close(self) -> None:
+        if self.ptr:
+            self.lib.darcy_ctx_destroy(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __enter__(self) -> "Context":
+        return self
+
+    def __exit__(self, *exc) -> None:
+        self.close()
+
+    def _check(self, rc: int) -> None:
+        if rc == OK:
+            return
+        msg = self.lib.darcy_last_error(self.ptr).decode()
+        if rc == INVALID_INPUT:
+            raise ValueError(msg)
+        raise RuntimeError(msg)
+
+    def info(self) -> str:
+        buf = C.create_string_buffer(256)
+        self.lib.darcy_device_info(self.ptr, buf, 256)
+        return buf.value.decode()
+
+    def bandwidth_probe(self, nbytes: int = 1 << 30, repeats: int = 10) -> float:
+        """GB/s (read + write) of the library's streaming copy kernel."""
+        g = C.c_double()
+        self._check(self.lib.darcy_bandwidth_probe(self.ptr, nbytes, repeats, C.byref(g)))
+        return g.value
+
+    # ----------------------------------------------------------- memory
+    def alloc(self, nbytes: int) -> int:
+        p = C.c_void_p()
+        self._check(self.lib.darcy_device_alloc(self.ptr, nbytes, C.byref(p)))
+        return int(p.value)
+
+    def free(self, dptr: int) -> None:
+        self._check(self.lib.darcy_device_free(self.ptr, C.c_void_p(dptr)))
+
+    def upload(self, dptr: int, host: np.ndarray) -> None:
+        host = np.ascontiguousarray(host)
+        self._check(self.lib.darcy_copy_to_device(self.ptr, C.c_void_p(dptr), host.ctypes.data, host.nbytes))
+
+    def download(self, host: np.ndarray, dptr: int) -> None:
+        assert host.flags.c_contiguous
+        self._check(self.lib.darcy_copy_to_host(self.ptr, host.ctypes.data, C.c_void_p(dptr), host.nbytes))
+
+    def last_kernel_ms(self) -> float:
+        ms = C.c_double()
+        self.lib.darcy_last_kernel_ms(self.ptr, C.byref(ms))
+        return ms.value
+
+    # ------------------------------------------------------------ solve
+    def solve_batch_device(
+        self,
+        sizes: Sequence[int],
+        a_ptrs: Sequence[int],
+        u_ptrs: Sequence[int],
+        *,
+        u0_ptrs: Sequence[int] | None = None,
+        f: float = 1.0,
+        tol: float = 1e-10,
+        max_iter: int = 10000,
+        preconditioner: str = "multigrid",
+        pre_sweeps: int = 1,
+        post_sweeps: int = 1,
+        cycle_index: int = 2,
+        precision: str = "fp64",
+    ) -> dict[str, np.ndarray]:
+        """Solve problems whose a / u (and u0) live in device memory."""
+        count = len(sizes)
+        o = Opts()
+        self.lib.darcy_default_opts(C.byref(o))
+        o.tol, o.max_iter, o.preconditioner = tol, max_iter, PC[preconditioner]
+        o.pre_sweeps, o.post_sweeps, o.cycle_index = pre_sweeps, post_sweeps, cycle_index
+        o.precision, o.device_pointers = PRECISION[precision], 1
+        n = (C.c_int64 * count)(*sizes)
+        a = (C.c_void_p * count)(*a_ptrs)
+        u = (C.c_void_p * count)(*u_ptrs)
+        u0 = (C.c_void_p * count)(*u0_ptrs) if u0_ptrs is not None else None
+        fs = (C.c_double * count)(*([f] * count))
+        st = (Stats * count)()
+        self._check(self.lib.darcy_solve_batch(self.ptr, count, n, a, None, fs, C.byref(o), u0, u, st))
+        return {
+            "iterations": np.array([s.iterations for s in st], dtype=np.int64),
+            "converged": np.array([bool(s.converged) for s in st]),
+            "residual": np.array([s.residual for s in st]),
+            "levels": np.array([s.levels for s in st], dtype=np.int64),
+        }

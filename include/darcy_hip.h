@@ -134,6 +134,11 @@ int darcy_copy_to_host(darcy_ctx* ctx, void* dst, const void* src, size_t bytes)
  * context, on the stream they ran on (milliseconds). */
 int darcy_last_kernel_ms(const darcy_ctx* ctx, double* ms);
 
+/* Achievable HBM bandwidth: a streaming copy of `bytes` repeated `repeats`
+ * times, timed with HIP events; *gbps counts read + write bytes.  No
+ * reference counterpart (measurement support). */
+int darcy_bandwidth_probe(darcy_ctx* ctx, size_t bytes, int32_t repeats, double* gbps);
+
 /* "gfx950, 256 CUs, HIP 7.2" style description of the context's device. */
 int darcy_device_info(const darcy_ctx* ctx, char* buf, int32_t cap);
 
