@@ -18,12 +18,71 @@ namespace darcy_hip {
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / kWave;
 
+// Largest side of the levels kept in LDS: 16 in FP64 (14 KB), 32 in FP32
+// (28 KB), so four workgroups fit on a CU.
+template <class T>
+struct LdsSide;
+template <>
+struct LdsSide<double> {
+    static constexpr int value = 16;
+};
+template <>
+struct LdsSide<float> {
+    static constexpr int value = 32;
+};
+
 struct Shared {
     Hier h;
+    LdsTable t;
     double red[4];
     unsigned long long bad;
     int prob;
 };
+
+// --------------------------------------------------------------- reductions
+
+__device__ inline double wave_sum(double s) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, kWave);
+    return __shfl(s, 0, kWave);
+}
+
+// Canonical chunked reduction.  For each chunk owned by this wave, every
+// lane loads its four cells first (`load(k)` returns a value bundle), then
+// commits them in order (`commit(k, bundle, acc)` adds to acc[0..ND)).
+template <int ND, class Load, class Commit>
+DARCY_INLINE void chunk_pass(int N, int w, int nw, double* const* part, Load&& load, Commit&& commit) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int C = chunks_of(N);
+    for (int c = w; c < C; c += nw) {
+        decltype(load(0)) v[4];
+        bool ok[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const int k = c * kChunk + m * kWave + lane;
+            ok[m] = k < N;
+            if (ok[m]) v[m] = load(k);
+        }
+        double acc[ND];
+#pragma unroll
+        for (int d = 0; d < ND; ++d) acc[d] = 0.0;
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+            if (ok[m]) commit(c * kChunk + m * kWave + lane, v[m], acc);
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+            const double s = wave_sum(acc[d]);
+            if (lane == 0) part[d][c] = s;
+        }
+    }
+}
+
+__device__ inline double final_sum(const double* part, int C) {
+    const int lane = threadIdx.x & (kWave - 1);
+    double s = 0.0;
+    for (int c = lane; c < C; c += kWave) s += part[c];
+    return wave_sum(s);
+}
 
 // Barrier, then one wave combines the chunk sums of `nd` reductions.
 __device__ inline void reduce_parts(Shared& s, int nd, int C) {
@@ -37,10 +96,125 @@ __device__ inline void reduce_parts(Shared& s, int nd, int C) {
     __syncthreads();
 }
 
+// ------------------------------------------------------------ the cycle
+
 template <class T>
-__device__ inline void precondition(const Hier& h, const SolveParams& P, int N, Exec e) {
+DARCY_INLINE LvP<T> global_level(const Hier& h, int l);
+template <>
+DARCY_INLINE LvP<double> global_level<double>(const Hier& h, int l) {
+    return {h.n[l], h.ex[l], h.ny[l], h.id[l], static_cast<double*>(h.x[l]), static_cast<double*>(h.b[l])};
+}
+template <>
+DARCY_INLINE LvP<float> global_level<float>(const Hier& h, int l) {
+    return {h.n[l], h.ex32[l], h.ny32[l], h.id32[l], static_cast<float*>(h.x[l]), static_cast<float*>(h.b[l])};
+}
+
+// The cycle of multigrid.hpp:97-127 run by one workgroup, with the coarse
+// levels in the LDS pool and the smallest ones on a single wave.
+template <class T>
+struct Cycle {
+    const Hier& h;
+    const LdsTable& t;
+    T* pool;
+    CycleOpts o;
+
+    DARCY_INLINE LvP<T> G(int l) const { return global_level<T>(h, l); }
+    DARCY_INLINE LvP<T> S(int l) const {
+        const int* f = t.off[l];
+        return {h.n[l], pool + f[0], pool + f[1], pool + f[2], pool + f[3], pool + f[4]};
+    }
+
+    // --- one wave, everything in LDS (levels >= t.wave)
+    DARCY_INLINE void wave_cycle(int l0, bool fresh0, Exec e) const {
+        const WaveSync sync;
+        const int L = h.L;
+        uint32_t done = 0;
+        int l = l0;
+        bool fresh = fresh0;
+        for (;;) {
+            while (l + 1 < L) {
+                visit_down(S(l), S(l + 1).b, fresh, o, e, sync);
+                done &= ~(1u << l);
+                ++l;
+                fresh = true;
+            }
+            coarsest_solve(S(l), fresh, e, sync);
+            for (;;) {
+                if (l == l0) return;
+                --l;
+                if (o.gamma == 2 && !((done >> l) & 1u)) {
+                    done |= 1u << l;
+                    ++l;
+                    fresh = false;
+                    break;
+                }
+                visit_up(S(l), S(l + 1).x, o, e, sync);
+            }
+        }
+    }
+
+    DARCY_INLINE void down(int l, bool fresh, Exec e) const {
+        const BlockSync sync;
+        if (l >= t.first)
+            visit_down(S(l), S(l + 1).b, fresh, o, e, sync);
+        else if (l + 1 >= t.first)
+            visit_down(G(l), S(l + 1).b, fresh, o, e, sync);
+        else
+            visit_down(G(l), G(l + 1).b, fresh, o, e, sync);
+    }
+
+    DARCY_INLINE void up(int l, Exec e) const {
+        const BlockSync sync;
+        if (l >= t.first)
+            visit_up(S(l), S(l + 1).x, o, e, sync);
+        else if (l + 1 >= t.first)
+            visit_up(G(l), static_cast<const T*>(S(l + 1).x), o, e, sync);
+        else
+            visit_up(G(l), static_cast<const T*>(G(l + 1).x), o, e, sync);
+    }
+
+    // z = M r from level l0 (x_{l0} zero on entry when fresh0).
+    DARCY_INLINE void run(int l0, bool fresh0, Exec e) const {
+        const int L = h.L;
+        uint32_t done = 0;
+        int l = l0;
+        bool fresh = fresh0;
+        for (;;) {
+            while (l + 1 < L && l < t.wave) {
+                down(l, fresh, e);
+                done &= ~(1u << l);
+                ++l;
+                fresh = true;
+            }
+            if (l >= t.wave) {
+                if (threadIdx.x < kWave) wave_cycle(l, fresh, Exec{static_cast<int>(threadIdx.x), kWave});
+                __syncthreads();
+            } else if (l >= t.first) {
+                coarsest_solve(S(l), fresh, e, BlockSync{});
+            } else {
+                coarsest_solve(G(l), fresh, e, BlockSync{});
+            }
+            for (;;) {
+                if (l == l0) return;
+                --l;
+                if (o.gamma == 2 && !((done >> l) & 1u)) {
+                    done |= 1u << l;
+                    ++l;
+                    fresh = false;
+                    break;
+                }
+                up(l, e);
+            }
+        }
+    }
+};
+
+// --------------------------------------------------------------- a problem
+
+template <class T>
+DARCY_INLINE void precondition(const Cycle<T>& cyc, const Hier& h, const SolveParams& P, int N, Exec e) {
     if (P.precond == kPcMultigrid || P.mode == kModeApplyPreconditioner) {
-        block_cycle<T>(h, 0, true, CycleOpts{P.pre, P.post, P.gamma}, e);
+        cyc.run(0, true, e);
     } else if (P.precond == kPcJacobi) {  // solver.hpp:127-132
         for (int k = e.tid; k < N; k += e.nthr) h.z[k] = h.r[k] * h.id[0][k];
         __syncthreads();
@@ -51,7 +225,8 @@ __device__ inline void precondition(const Hier& h, const SolveParams& P, int N, 
 }
 
 template <class T>
-__device__ void run_problem(Shared& s, const Problem& pb, ProblemStats& st, const SolveParams& P, char* slot) {
+__device__ void run_problem(Shared& s, T* pool, const Problem& pb, ProblemStats& st, const SolveParams& P,
+                            char* slot) {
     const Exec e{static_cast<int>(threadIdx.x), kBlock};
     const int wave = threadIdx.x / kWave;
     const int n = pb.n, N = n * n, C = chunks_of(N);
@@ -69,7 +244,10 @@ __device__ void run_problem(Shared& s, const Problem& pb, ProblemStats& st, cons
     }
 
     const bool mg = P.mode == kModeApplyPreconditioner || (P.mode == kModeSolve && P.precond == kPcMultigrid);
-    if (threadIdx.x == 0) plan_slot(slot, n, mixed, &s.h);
+    if (threadIdx.x == 0) {
+        plan_slot(slot, n, mixed, &s.h);
+        plan_lds<T, LdsSide<T>::value>(s.h, s.t);
+    }
     __syncthreads();
     const Hier& h = s.h;
 
@@ -89,8 +267,21 @@ __device__ void run_problem(Shared& s, const Problem& pb, ProblemStats& st, cons
             to_fp32(h.n[l], h.ex[l], h.ny[l], h.id[l], h.ex32[l], h.ny32[l], h.id32[l], e);
             __syncthreads();
         }
+        // Coarse operators into LDS for the whole solve.
+        for (int l = s.t.first; l < h.L; ++l) {
+            const LvP<T> g = global_level<T>(h, l);
+            const int* f = s.t.off[l];
+            const int nf = g.n * (g.n + 1), nc = g.n * g.n;
+            for (int q = e.tid; q < nf; q += e.nthr) {
+                pool[f[0] + q] = g.ex[q];
+                pool[f[1] + q] = g.ny[q];
+            }
+            for (int q = e.tid; q < nc; q += e.nthr) pool[f[2] + q] = g.id[q];
+        }
+        __syncthreads();
     }
-    const Lv<double> A0 = level_view<double>(h, 0);
+    const Cycle<T> cyc{h, s.t, pool, CycleOpts{P.pre, P.post, P.gamma}};
+    const LvP<double> A0 = global_level<double>(h, 0);
     float* const b0f = mixed ? static_cast<float*>(h.b[0]) : nullptr;
     const float* const x0f = mixed ? static_cast<const float*>(h.x[0]) : nullptr;
     auto zval = [&](int k) -> double { return mixed ? static_cast<double>(x0f[k]) : h.z[k]; };
@@ -98,7 +289,7 @@ __device__ void run_problem(Shared& s, const Problem& pb, ProblemStats& st, cons
     if (P.mode == kModeApplyOperator) {  // bindings.cpp:199-216
         for (int k = e.tid; k < N; k += e.nthr) {
             const int i = k / n, j = k - i * n;
-            pb.out[k] = apply_at(A0, i, j, [&](int t) { return pb.in[t]; });
+            pb.out[k] = apply_at(n, A0.ex, A0.ny, i, j, [&](int t) { return pb.in[t]; });
         }
         if (threadIdx.x == 0) st = ProblemStats{0, kConverged, 1, 0, 0.0, -1};
         return;
@@ -109,7 +300,7 @@ __device__ void run_problem(Shared& s, const Problem& pb, ProblemStats& st, cons
             if (mixed) b0f[k] = static_cast<float>(pb.in[k]);
         }
         __syncthreads();
-        precondition<T>(h, P, N, e);
+        precondition(cyc, h, P, N, e);
         for (int k = e.tid; k < N; k += e.nthr) pb.out[k] = zval(k);
         if (threadIdx.x == 0) st = ProblemStats{0, kConverged, h.L, 0, 0.0, -1};
         return;
@@ -119,21 +310,30 @@ __device__ void run_problem(Shared& s, const Problem& pb, ProblemStats& st, cons
     const double hh = 1.0 / static_cast<double>(n + 1);
     const double h2 = hh * hh;
     double* const u = pb.out;
-    chunk_sums<2>(N, wave, kWaves, h.part, [&](int k, double* acc) {
-        const double bk = (pb.rhs ? pb.rhs[k] : pb.f) * h2;
-        acc[0] += bk * bk;
-        double rk = bk;
-        if (pb.in) {
-            const int i = k / n, j = k - i * n;
-            rk = bk - apply_at(A0, i, j, [&](int t) { return pb.in[t]; });
-            u[k] = pb.in[k];
-        } else {
-            u[k] = 0.0;
-        }
-        h.r[k] = rk;
-        acc[1] += rk * rk;
-        if (mixed) b0f[k] = static_cast<float>(rk);
-    });
+    struct Init {
+        double bk, rk, u0;
+    };
+    chunk_pass<2>(
+        N, wave, kWaves, h.part,
+        [&](int k) {
+            Init v;
+            v.bk = (pb.rhs ? pb.rhs[k] : pb.f) * h2;
+            v.rk = v.bk;
+            v.u0 = 0.0;
+            if (pb.in) {
+                const int i = k / n, j = k - i * n;
+                v.rk = v.bk - apply_at(n, A0.ex, A0.ny, i, j, [&](int t) { return pb.in[t]; });
+                v.u0 = pb.in[k];
+            }
+            return v;
+        },
+        [&](int k, const Init& v, double* acc) {
+            acc[0] += v.bk * v.bk;
+            u[k] = v.u0;
+            h.r[k] = v.rk;
+            acc[1] += v.rk * v.rk;
+            if (mixed) b0f[k] = static_cast<float>(v.rk);
+        });
     reduce_parts(s, 2, C);
     const double bnorm = sqrt(s.red[0]);
     const int levels = mg ? h.L : 1;
@@ -150,8 +350,10 @@ __device__ void run_problem(Shared& s, const Problem& pb, ProblemStats& st, cons
         return;
     }
 
-    precondition<T>(h, P, N, e);
-    chunk_sums<1>(N, wave, kWaves, h.part, [&](int k, double* acc) { acc[0] += h.r[k] * zval(k); });
+    precondition(cyc, h, P, N, e);
+    chunk_pass<1>(
+        N, wave, kWaves, h.part, [&](int k) { return h.r[k] * zval(k); },
+        [&](int, double v, double* acc) { acc[0] += v; });
     reduce_parts(s, 1, C);
     double rz = s.red[0];
 
@@ -160,18 +362,33 @@ __device__ void run_problem(Shared& s, const Problem& pb, ProblemStats& st, cons
     double beta = 0.0;
     bool first = true;
     int status = kMaxIter, iters = 0;
+    struct PAp {
+        double p, ap;
+    };
+    struct Upd {
+        double u, p, r, ap;
+    };
+    struct Rz {
+        double r, z, ap;
+    };
     for (int it = 0; it < P.max_iter; ++it) {
         // p = z + beta p (solver.hpp:171, blas.hpp:50-56), fused with
         // ap = A p (solver.hpp:147) and <p, ap> (solver.hpp:148).
-        chunk_sums<1>(N, wave, kWaves, h.part, [&](int k, double* acc) {
-            const int i = k / n, j = k - i * n;
-            auto pv = [&](int t) -> double { return first ? zval(t) : zval(t) + beta * pold[t]; };
-            const double v = apply_at(A0, i, j, pv);
-            const double pk = pv(k);
-            pnew[k] = pk;
-            h.ap[k] = v;
-            acc[0] += pk * v;
-        });
+        chunk_pass<1>(
+            N, wave, kWaves, h.part,
+            [&](int k) {
+                const int i = k / n, j = k - i * n;
+                auto pv = [&](int t) -> double { return first ? zval(t) : zval(t) + beta * pold[t]; };
+                PAp v;
+                v.ap = apply_at(n, A0.ex, A0.ny, i, j, pv);
+                v.p = pv(k);
+                return v;
+            },
+            [&](int k, const PAp& v, double* acc) {
+                pnew[k] = v.p;
+                h.ap[k] = v.ap;
+                acc[0] += v.p * v.ap;
+            });
         reduce_parts(s, 1, C);
         const double pap = s.red[0];
         if (!(fabs(pap) > 0.0) || !isfinite(pap)) {  // solver.hpp:149-155
@@ -181,13 +398,15 @@ __device__ void run_problem(Shared& s, const Problem& pb, ProblemStats& st, cons
         }
         const double alpha = rz / pap;
         // u += alpha p; r -= alpha ap; <r, r>  (blas.hpp:62-77)
-        chunk_sums<1>(N, wave, kWaves, h.part, [&](int k, double* acc) {
-            u[k] += alpha * pnew[k];
-            const double ri = h.r[k] - alpha * h.ap[k];
-            h.r[k] = ri;
-            acc[0] += ri * ri;
-            if (mixed) b0f[k] = static_cast<float>(ri);
-        });
+        chunk_pass<1>(
+            N, wave, kWaves, h.part, [&](int k) { return Upd{u[k], pnew[k], h.r[k], h.ap[k]}; },
+            [&](int k, const Upd& v, double* acc) {
+                u[k] = v.u + alpha * v.p;
+                const double ri = v.r - alpha * v.ap;
+                h.r[k] = ri;
+                acc[0] += ri * ri;
+                if (mixed) b0f[k] = static_cast<float>(ri);
+            });
         reduce_parts(s, 1, C);
         res = sqrt(s.red[0]) / bnorm;
         iters = it + 1;
@@ -197,12 +416,13 @@ __device__ void run_problem(Shared& s, const Problem& pb, ProblemStats& st, cons
             break;
         }
         if (iters == P.max_iter) break;
-        precondition<T>(h, P, N, e);
-        chunk_sums<2>(N, wave, kWaves, h.part, [&](int k, double* acc) {
-            const double zk = zval(k);
-            acc[0] += h.r[k] * zk;
-            if (mixed) acc[1] += zk * h.ap[k];
-        });
+        precondition(cyc, h, P, N, e);
+        chunk_pass<2>(
+            N, wave, kWaves, h.part, [&](int k) { return Rz{h.r[k], zval(k), mixed ? h.ap[k] : 0.0}; },
+            [&](int, const Rz& v, double* acc) {
+                acc[0] += v.r * v.z;
+                if (mixed) acc[1] += v.z * v.ap;
+            });
         reduce_parts(s, mixed ? 2 : 1, C);
         beta = mixed ? (-alpha * s.red[1]) / rz : s.red[0] / rz;
         rz = s.red[0];
@@ -211,13 +431,13 @@ __device__ void run_problem(Shared& s, const Problem& pb, ProblemStats& st, cons
         pnew = t;
         first = false;
     }
-    if (threadIdx.x == 0)
-        st = ProblemStats{iters, status, levels, hist ? iters + 1 : 0, res, -1};
+    if (threadIdx.x == 0) st = ProblemStats{iters, status, levels, hist ? iters + 1 : 0, res, -1};
 }
 
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_batch(BatchArgs A) {
     __shared__ Shared s;
+    __shared__ T pool[LdsPool<T, LdsSide<T>::value>::kElems];
     char* slot = A.slots + static_cast<size_t>(blockIdx.x) * A.slot_bytes;
     for (;;) {
         if (threadIdx.x == 0) s.prob = static_cast<int>(atomicAdd(A.next, 1u));
@@ -225,13 +445,22 @@ __global__ __launch_bounds__(kBlock) void k_batch(BatchArgs A) {
         const int t = s.prob;
         if (t >= A.count) return;
         const int p = A.order ? A.order[t] : t;
-        run_problem<T>(s, A.probs[p], A.stats[p], A.prm, slot);
+        run_problem<T>(s, pool, A.probs[p], A.stats[p], A.prm, slot);
         __syncthreads();
     }
 }
 
 template __global__ void k_batch<double>(BatchArgs);
 template __global__ void k_batch<float>(BatchArgs);
+
+// Streaming copy of `n` doubles: the reference kernel for achievable HBM
+// bandwidth and for scaling the byte counters of the profiler.
+__global__ __launch_bounds__(256) void k_stream_copy(const double2* __restrict__ src, double2* __restrict__ dst,
+                                                      size_t n2) {
+    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n2;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x)
+        dst[i] = src[i];
+}
 
 }  // namespace darcy_hip
 
@@ -253,17 +482,8 @@ extern "C" hipError_t darcy_hip_batch_occupancy(int mixed, int* blocks_per_cu) {
                                                         darcy_hip::kBlock, 0);
 }
 
-// Streaming copy of `n` doubles: the reference kernel for achievable HBM
-// bandwidth and for scaling the byte counters of the profiler.
-__global__ __launch_bounds__(256) void k_stream_copy(const double2* __restrict__ src, double2* __restrict__ dst,
-                                                      size_t n2) {
-    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n2;
-         i += static_cast<size_t>(gridDim.x) * blockDim.x)
-        dst[i] = src[i];
-}
-
 extern "C" hipError_t darcy_hip_launch_copy(const void* src, void* dst, size_t bytes, int grid, hipStream_t stream) {
-    hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, stream, static_cast<const double2*>(src),
+    hipLaunchKernelGGL(darcy_hip::k_stream_copy, dim3(grid), dim3(256), 0, stream, static_cast<const double2*>(src),
                        static_cast<double2*>(dst), bytes / sizeof(double2));
     return hipGetLastError();
 }

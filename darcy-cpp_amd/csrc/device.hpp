@@ -1,23 +1,32 @@
 // Device building blocks of the solve path for gfx950 (CDNA4, wave64).
 //
-// Everything here is written once and used both by the persistent batch
-// kernel (one workgroup owns a whole problem) and by the grid kernels of the
-// large single solves (many workgroups share one level), so the per-cell
-// arithmetic -- and therefore every result -- is identical on both paths.
+// Written once, used by the persistent batch kernel (one workgroup owns a
+// whole problem), so the per-cell arithmetic -- and every result -- is the
+// same whichever way a problem is scheduled.
 //
 // Per-cell floating-point operations follow the reference's operation order
 // exactly (include/darcy/level.hpp), so that with -ffp-contract=off the FP64
 // cycle reproduces the reference bit for bit:
 //   apply/residual  v = d*x_k - e*x_E - w*x_W - n*x_N - s*x_S   (level.hpp:111-116, 129-134)
-//   GS update       x_k = (((b_k + e*x_E) + w*x_W) + n*x_N + s*x_S) * inv_d   (level.hpp:153-158)
+//   GS update       x_k = ((((b_k + e*x_E) + w*x_W) + n*x_N) + s*x_S) * inv_d   (level.hpp:153-158)
 //   restriction     sum over (i0,j0),(i0,j1),(i1,j0),(i1,j1)   (level.hpp:221-224)
+// Fusions keep those values: a value the reference stores and reloads (the
+// first red sweep from zero, a prolongated cell) is recomputed in the same
+// operation order where a fused phase needs it.
+//
+// Memory placement of the hierarchy inside one workgroup:
+//   levels with side > NL ........ global memory (L2/HBM), whole workgroup
+//   levels with side <= NL ....... LDS, whole workgroup
+//   levels with side <= 8 ........ LDS, the first wave alone, with no
+//                                  workgroup barrier (the W-cycle visits the
+//                                  coarse levels 2^l times: this is where the
+//                                  barriers were)
 //
 // Reductions (the CG dot products) use one canonical order that does not
 // depend on launch shape: the flat vector is cut into chunks of 256 cells;
 // lane l of the wave that owns a chunk adds cells l, l+64, l+128, l+192 in
 // turn, the 64 lane sums are combined by a fixed xor butterfly, and the chunk
-// sums are combined the same way by one wave.  Any number of workgroups or
-// waves yields the same bits.
+// sums are combined the same way by one wave.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -26,45 +35,55 @@
 
 #include "layout.hpp"
 
+#define DARCY_INLINE __device__ __forceinline__
+
 namespace darcy_hip {
 
 constexpr int kWave = 64;
+constexpr int kWaveTail = 8;  // levels with side <= 8 run on one wave
 
-// Operator view of one level in the cycle's arithmetic type T.
+// ------------------------------------------------------------ level views
+
+// Operator and vectors of one level in the cycle's arithmetic type T.
 template <class T>
-struct Lv {
+struct LvP {
     int n;
-    const T* __restrict__ ex;
-    const T* __restrict__ ny;
-    const T* __restrict__ id;
+    const T* ex;  // n x (n+1) faces
+    const T* ny;  // (n+1) x n faces
+    const T* id;  // n x n reciprocal diagonal
+    T* x;
+    T* b;
 };
 
-template <class T>
-__device__ inline Lv<T> level_view(const Hier& h, int l);
-template <>
-__device__ inline Lv<double> level_view<double>(const Hier& h, int l) {
-    return {h.n[l], h.ex[l], h.ny[l], h.id[l]};
-}
-template <>
-__device__ inline Lv<float> level_view<float>(const Hier& h, int l) {
-    return {h.n[l], h.ex32[l], h.ny32[l], h.id32[l]};
-}
+// ------------------------------------------------------------------ sync
+
+struct BlockSync {
+    DARCY_INLINE void operator()() const { __syncthreads(); }
+};
+
+// One wave: LDS operations of a wave complete in order; the fences keep the
+// compiler from moving memory operations across the barrier.
+struct WaveSync {
+    DARCY_INLINE void operator()() const {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+};
+
+struct Exec {
+    int tid;
+    int nthr;
+};
 
 // ---------------------------------------------------------------- per cell
-
-// Diagonal ((e + w) + n) + s, exactly as level.hpp:92 / :205 build it.
-template <class T>
-__device__ inline T diag_at(const Lv<T>& L, int i, int j) {
-    const int n = L.n, f = i * (n + 1) + j;
-    return ((L.ex[f + 1] + L.ex[f]) + L.ny[(i + 1) * n + j]) + L.ny[i * n + j];
-}
 
 // (A x)_k with off-grid neighbours dropped (level.hpp:105-119).  `X` maps a
 // flat index to the value of x there, so callers can form x on the fly.
 template <class T, class X>
-__device__ inline T apply_at(const Lv<T>& L, int i, int j, X&& x) {
-    const int n = L.n, k = i * n + j, f = i * (n + 1) + j;
-    const T e = L.ex[f + 1], w = L.ex[f], nn = L.ny[(i + 1) * n + j], s = L.ny[i * n + j];
+DARCY_INLINE T apply_at(int n, const T* ex, const T* ny, int i, int j, X&& x) {
+    const int k = i * n + j, f = i * (n + 1) + j;
+    const T e = ex[f + 1], w = ex[f], nn = ny[(i + 1) * n + j], s = ny[i * n + j];
     const T d = ((e + w) + nn) + s;
     T v = d * x(k);
     if (j + 1 < n) v -= e * x(k + 1);
@@ -75,150 +94,162 @@ __device__ inline T apply_at(const Lv<T>& L, int i, int j, X&& x) {
 }
 
 // Gauss-Seidel value of cell (i, j) from its neighbours (level.hpp:153-158).
+// `X` maps grid coordinates (i, j) to the value of x there.
 template <class T, class X>
-__device__ inline T gs_at(const Lv<T>& L, int i, int j, T bk, X&& x) {
-    const int n = L.n, k = i * n + j, f = i * (n + 1) + j;
+DARCY_INLINE T gs_at(const LvP<T>& L, int i, int j, T bk, X&& x) {
+    const int n = L.n, f = i * (n + 1) + j;
     T v = bk;
-    if (j + 1 < n) v += L.ex[f + 1] * x(k + 1);
-    if (j > 0) v += L.ex[f] * x(k - 1);
-    if (i + 1 < n) v += L.ny[(i + 1) * n + j] * x(k + n);
-    if (i > 0) v += L.ny[i * n + j] * x(k - n);
-    return v * L.id[k];
+    if (j + 1 < n) v += L.ex[f + 1] * x(i, j + 1);
+    if (j > 0) v += L.ex[f] * x(i, j - 1);
+    if (i + 1 < n) v += L.ny[(i + 1) * n + j] * x(i + 1, j);
+    if (i > 0) v += L.ny[i * n + j] * x(i - 1, j);
+    return v * L.id[i * n + j];
 }
 
 // ------------------------------------------------------------------ phases
-// A phase is executed by `nthr` cooperating threads; `tid` is this thread's
-// rank among them.  Callers separate dependent phases with a barrier.
+// A phase is run by `nthr` cooperating threads; `tid` is this thread's rank.
+// Threads handle up to four cells per step with all loads issued before the
+// stores, so each thread keeps several memory requests in flight.
+// Callers separate dependent phases with a barrier.
 
-struct Exec {
-    int tid;
-    int nthr;
-};
+// Colour `c` cell number q of a level: row i, column 2*(q mod h) + ((i+c)&1).
+DARCY_INLINE bool colour_cell(int q, int n, int h, int c, int& i, int& j) {
+    i = q / h;
+    j = 2 * (q - i * h) + ((i + c) & 1);
+    return j < n;
+}
 
-// One colour of red-black GS (level.hpp:145-161).  Cells of row i with
-// (i + j) % 2 == colour are enumerated as j = 2q + ((i + colour) & 1).
+// One colour of red-black GS (level.hpp:145-161).
 template <class T>
-__device__ inline void gs_colour(const Lv<T>& L, T* __restrict__ x, const T* __restrict__ b, int colour,
-                                 Exec e) {
+DARCY_INLINE void gs_colour(const LvP<T>& L, int colour, Exec e) {
     const int n = L.n, h = (n + 1) >> 1, total = n * h;
-    for (int q = e.tid; q < total; q += e.nthr) {
-        const int i = q / h;
-        const int j = 2 * (q - i * h) + ((i + colour) & 1);
-        if (j >= n) continue;
-        const int k = i * n + j;
-        x[k] = gs_at(L, i, j, b[k], [&](int m) { return x[m]; });
+    T* x = L.x;
+    for (int q0 = e.tid; q0 < total; q0 += 4 * e.nthr) {
+        T v[4];
+        int kk[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            int i, j;
+            const int q = q0 + u * e.nthr;
+            kk[u] = -1;
+            if (q < total && colour_cell(q, n, h, colour, i, j)) {
+                kk[u] = i * n + j;
+                v[u] = gs_at(L, i, j, L.b[kk[u]], [&](int a, int c) { return x[a * n + c]; });
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (kk[u] >= 0) x[kk[u]] = v[u];
     }
 }
 
-// The first red sweep on a level whose x is zero: every neighbour term is
-// e * 0, so the update is b * inv_d (the reference's value, up to the sign of
-// a zero).  Black cells are left untouched: they are still zero and the black
-// sweep that follows reads only red cells.
+// Pre-smoothing red + black on a level whose x is zero, in one phase.  The
+// red sweep from zero gives b * inv_d at every red cell (each neighbour term
+// is e * 0); the black update reads those red values, recomputed here in the
+// same operations instead of being reloaded.
 template <class T>
-__device__ inline void gs_red_from_zero(const Lv<T>& L, T* __restrict__ x, const T* __restrict__ b, Exec e) {
+DARCY_INLINE void presmooth_from_zero(const LvP<T>& L, Exec e) {
     const int n = L.n, h = (n + 1) >> 1, total = n * h;
-    for (int q = e.tid; q < total; q += e.nthr) {
-        const int i = q / h;
-        const int j = 2 * (q - i * h) + (i & 1);
-        if (j >= n) continue;
-        const int k = i * n + j;
-        x[k] = b[k] * L.id[k];
+    T* x = L.x;
+    const T* b = L.b;
+    const T* id = L.id;
+    auto red = [&](int a, int c) { return b[a * n + c] * id[a * n + c]; };
+    for (int q0 = e.tid; q0 < total; q0 += 4 * e.nthr) {
+        T vr[4], vb[4];
+        int kr[4], kb[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            int i, j;
+            const int q = q0 + u * e.nthr;
+            kr[u] = kb[u] = -1;
+            if (q < total && colour_cell(q, n, h, 0, i, j)) {
+                kr[u] = i * n + j;
+                vr[u] = red(i, j);
+            }
+            if (q < total && colour_cell(q, n, h, 1, i, j)) {
+                kb[u] = i * n + j;
+                vb[u] = gs_at(L, i, j, b[kb[u]], red);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (kr[u] >= 0) x[kr[u]] = vr[u];
+            if (kb[u] >= 0) x[kb[u]] = vb[u];
+        }
     }
 }
 
-// Black sweep right after gs_red_from_zero: black cells were zero and are
-// never read, so only red neighbours contribute -- the general update.
+// First post-smoothing sweep (black) fused with the prolongation
+// x += P x_c (level.hpp:230-237): red neighbours are read as x + x_c of their
+// aggregate, the value the reference stores before smoothing.  Red cells keep
+// their unprolongated values: the red sweep that follows overwrites them
+// without reading them.
 template <class T>
-__device__ inline void zero_colour(int n, T* __restrict__ x, int colour, Exec e) {
-    const int h = (n + 1) >> 1, total = n * h;
-    for (int q = e.tid; q < total; q += e.nthr) {
-        const int i = q / h;
-        const int j = 2 * (q - i * h) + ((i + colour) & 1);
-        if (j < n) x[i * n + j] = T(0);
+DARCY_INLINE void black_with_prolong(const LvP<T>& L, const T* xc, Exec e) {
+    const int n = L.n, h = (n + 1) >> 1, total = n * h, m = (n + 1) >> 1;
+    T* x = L.x;
+    auto prolonged = [&](int a, int c) { return x[a * n + c] + xc[(a >> 1) * m + (c >> 1)]; };
+    for (int q0 = e.tid; q0 < total; q0 += 4 * e.nthr) {
+        T v[4];
+        int kk[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            int i, j;
+            const int q = q0 + u * e.nthr;
+            kk[u] = -1;
+            if (q < total && colour_cell(q, n, h, 1, i, j)) {
+                kk[u] = i * n + j;
+                v[u] = gs_at(L, i, j, L.b[kk[u]], prolonged);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (kk[u] >= 0) x[kk[u]] = v[u];
     }
 }
 
 // b_c = P^T (b - A x) over each <= 2x2 aggregate (level.hpp:122-137 then
 // :214-227, fused: the fine residual is never stored).
 template <class T>
-__device__ inline void residual_restrict(const Lv<T>& L, const T* __restrict__ x, const T* __restrict__ b,
-                                         T* __restrict__ bc, Exec e) {
+DARCY_INLINE void residual_restrict(const LvP<T>& L, T* bc, Exec e) {
     const int n = L.n, m = (n + 1) >> 1, total = m * m;
+    const T* x = L.x;
     for (int q = e.tid; q < total; q += e.nthr) {
         const int I = q / m, J = q - I * m;
         const int i0 = 2 * I, i1 = (2 * I + 1 < n) ? 2 * I + 1 : 2 * I;
         const int j0 = 2 * J, j1 = (2 * J + 1 < n) ? 2 * J + 1 : 2 * J;
         T s = T(0);
         for (int i = i0; i <= i1; ++i)
-            for (int j = j0; j <= j1; ++j) {
-                const int k = i * n + j;
-                s += b[k] - apply_at(L, i, j, [&](int t) { return x[t]; });
-            }
+            for (int j = j0; j <= j1; ++j)
+                s += L.b[i * n + j] - apply_at(n, L.ex, L.ny, i, j, [&](int t) { return x[t]; });
         bc[q] = s;
     }
 }
 
-// x += P x_c (level.hpp:230-237).
-template <class T>
-__device__ inline void prolong_add(int n, T* __restrict__ x, const T* __restrict__ xc, Exec e) {
-    const int m = (n + 1) >> 1, total = n * n;
-    for (int k = e.tid; k < total; k += e.nthr) {
-        const int i = k / n, j = k - i * n;
-        x[k] += xc[(i >> 1) * m + (j >> 1)];
-    }
-}
-
-// ------------------------------------------------------ workgroup W/V cycle
-
-__device__ inline void block_sync() { __syncthreads(); }
-
 // Coarsest solve (multigrid.hpp:129-144): exact for 1x1, otherwise 8 + 8
 // symmetric sweeps (reached only when the 32-level cap stops coarsening).
-template <class T>
-__device__ inline void coarsest_solve(const Lv<T>& L, T* x, const T* b, bool fresh, Exec e) {
+template <class T, class S>
+DARCY_INLINE void coarsest_solve(const LvP<T>& L, bool fresh, Exec e, S sync) {
     if (L.n == 1) {
-        if (e.tid == 0) x[0] = b[0] * L.id[0];
-        block_sync();
+        if (e.tid == 0) L.x[0] = L.b[0] * L.id[0];
+        sync();
         return;
     }
     if (fresh) {
-        for (int k = e.tid; k < L.n * L.n; k += e.nthr) x[k] = T(0);
-        block_sync();
+        for (int k = e.tid; k < L.n * L.n; k += e.nthr) L.x[k] = T(0);
+        sync();
     }
     for (int s = 0; s < 8; ++s) {
-        gs_colour(L, x, b, 0, e);
-        block_sync();
-        gs_colour(L, x, b, 1, e);
-        block_sync();
+        gs_colour(L, 0, e);
+        sync();
+        gs_colour(L, 1, e);
+        sync();
     }
     for (int s = 0; s < 8; ++s) {
-        gs_colour(L, x, b, 1, e);
-        block_sync();
-        gs_colour(L, x, b, 0, e);
-        block_sync();
-    }
-}
-
-template <class T>
-__device__ inline void pre_smooth(const Lv<T>& L, T* x, const T* b, int sweeps, bool fresh, Exec e) {
-    for (int s = 0; s < sweeps; ++s) {
-        if (s == 0 && fresh)
-            gs_red_from_zero(L, x, b, e);
-        else
-            gs_colour(L, x, b, 0, e);
-        block_sync();
-        gs_colour(L, x, b, 1, e);
-        block_sync();
-    }
-}
-
-template <class T>
-__device__ inline void post_smooth(const Lv<T>& L, T* x, const T* b, int sweeps, Exec e) {
-    for (int s = 0; s < sweeps; ++s) {
-        gs_colour(L, x, b, 1, e);
-        block_sync();
-        gs_colour(L, x, b, 0, e);
-        block_sync();
+        gs_colour(L, 1, e);
+        sync();
+        gs_colour(L, 0, e);
+        sync();
     }
 }
 
@@ -228,85 +259,94 @@ struct CycleOpts {
     int gamma;  // 1 = V-cycle, 2 = W-cycle
 };
 
-// The recursive cycle of multigrid.hpp:97-127, run iteratively by one
-// workgroup from level l0 down to the coarsest level L-1.  `fresh0` says
-// whether x_{l0} is zero on entry.  Bit k of `done` counts the completed
-// sub-cycles of level k (gamma <= 2).
-template <class T>
-__device__ void block_cycle(const Hier& h, int l0, bool fresh0, const CycleOpts& o, Exec e) {
-    const int L = h.L;
-    uint32_t done = 0;
-    int l = l0;
-    bool fresh = fresh0;
-    for (;;) {
-        while (l + 1 < L) {
-            const Lv<T> A = level_view<T>(h, l);
-            T* x = static_cast<T*>(h.x[l]);
-            const T* b = static_cast<const T*>(h.b[l]);
-            pre_smooth(A, x, b, o.pre, fresh, e);
-            residual_restrict(A, x, b, static_cast<T*>(h.b[l + 1]), e);
-            block_sync();
-            done &= ~(1u << l);
-            ++l;
-            fresh = true;
+// Going down through level l: pre-smoothing (red, black) then the restricted
+// residual into the next level's b (multigrid.hpp:107-115).
+template <class T, class S>
+DARCY_INLINE void visit_down(const LvP<T>& L, T* bc, bool fresh, const CycleOpts& o, Exec e, S sync) {
+    for (int s = 0; s < o.pre; ++s) {
+        if (s == 0 && fresh) {
+            presmooth_from_zero(L, e);
+        } else {
+            gs_colour(L, 0, e);
+            sync();
+            gs_colour(L, 1, e);
         }
-        coarsest_solve(level_view<T>(h, l), static_cast<T*>(h.x[l]), static_cast<const T*>(h.b[l]), fresh, e);
-        for (;;) {
-            if (l == l0) return;
-            --l;
-            const bool first_done = (done >> l) & 1u;
-            if (!first_done && o.gamma == 2) {
-                done |= 1u << l;
-                ++l;
-                fresh = false;
-                break;
-            }
-            const Lv<T> A = level_view<T>(h, l);
-            T* x = static_cast<T*>(h.x[l]);
-            prolong_add(A.n, x, static_cast<const T*>(h.x[l + 1]), e);
-            block_sync();
-            post_smooth(A, x, static_cast<const T*>(h.b[l]), o.post, e);
-        }
+        sync();
+    }
+    residual_restrict(L, bc, e);
+    sync();
+}
+
+// Coming up through level l: prolongation and post-smoothing (black, red)
+// (multigrid.hpp:120-126).
+template <class T, class S>
+DARCY_INLINE void visit_up(const LvP<T>& L, const T* xc, const CycleOpts& o, Exec e, S sync) {
+    black_with_prolong(L, xc, e);
+    sync();
+    gs_colour(L, 0, e);
+    sync();
+    for (int s = 1; s < o.post; ++s) {
+        gs_colour(L, 1, e);
+        sync();
+        gs_colour(L, 0, e);
+        sync();
     }
 }
 
-// ------------------------------------------------------------ reductions
+// ---------------------------------------------------------- LDS hierarchy
 
-__device__ inline double wave_sum(double s) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, kWave);
-    return __shfl(s, 0, kWave);
-}
-
-// Runs `cell(k, acc)` over the chunks owned by wave `w` of `nw` waves and
-// stores each chunk's canonical sums in part[d][chunk].
-template <int ND, class F>
-__device__ inline void chunk_sums(int N, int w, int nw, double* const* part, F&& cell) {
-    const int lane = threadIdx.x & (kWave - 1);
-    const int C = chunks_of(N);
-    for (int c = w; c < C; c += nw) {
-        double acc[ND];
-#pragma unroll
-        for (int d = 0; d < ND; ++d) acc[d] = 0.0;
-#pragma unroll
-        for (int m = 0; m < 4; ++m) {
-            const int k = c * kChunk + m * kWave + lane;
-            if (k < N) cell(k, acc);
-        }
-#pragma unroll
-        for (int d = 0; d < ND; ++d) {
-            const double s = wave_sum(acc[d]);
-            if (lane == 0) part[d][c] = s;
-        }
+// LDS pool of the coarse levels: operators and vectors of every level with
+// side <= NL.  Sized for the worst case (sides NL, NL/2, ..., 1).
+template <class T, int NL>
+struct LdsPool {
+    static constexpr int elems_of(int s) { return s > 0 ? 2 * s * (s + 1) + 3 * s * s + 20 : 0; }
+    static constexpr int total_elems() {
+        int t = 0;
+        for (int s = NL; s >= 1; s = (s == 1) ? 0 : (s + 1) / 2) t += elems_of(s);
+        return t;
     }
-}
+    static constexpr int kElems = total_elems();
+};
 
-// Combine C chunk sums (one full wave must call this).
-__device__ inline double final_sum(const double* part, int C) {
-    const int lane = threadIdx.x & (kWave - 1);
-    double s = 0.0;
-    for (int c = lane; c < C; c += kWave) s += part[c];
-    return wave_sum(s);
+// Offsets (in elements of T) of the LDS levels, filled by one thread.
+struct LdsTable {
+    int first;  // first level held in LDS (L when none)
+    int wave;   // first level run by one wave
+    int off[kMaxLevels][5];
+};
+
+DARCY_INLINE int align4(int v) { return (v + 3) & ~3; }
+
+template <class T, int NL>
+DARCY_INLINE void plan_lds(const Hier& h, LdsTable& t) {
+    int first = h.L;
+    for (int l = 1; l < h.L; ++l)
+        if (h.n[l] <= NL) {
+            first = l;
+            break;
+        }
+    t.first = first;
+    int wave = h.L;
+    for (int l = first; l < h.L; ++l)
+        if (h.n[l] <= kWaveTail) {
+            wave = l;
+            break;
+        }
+    t.wave = wave;
+    int o = 0;
+    for (int l = first; l < h.L; ++l) {
+        const int s = h.n[l];
+        t.off[l][0] = o;
+        o = align4(o + s * (s + 1));
+        t.off[l][1] = o;
+        o = align4(o + s * (s + 1));
+        t.off[l][2] = o;
+        o = align4(o + s * s);
+        t.off[l][3] = o;
+        o = align4(o + s * s);
+        t.off[l][4] = o;
+        o = align4(o + s * s);
+    }
 }
 
 }  // namespace darcy_hip

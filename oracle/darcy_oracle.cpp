@@ -585,3 +585,11 @@ extern "C" int32_t oracle_max_threads(void) {
     return 1;
 #endif
 }
+
+extern "C" void oracle_set_num_threads(int32_t threads) {
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#else
+    (void)threads;
+#endif
+}

@@ -74,8 +74,10 @@ int32_t oracle_num_levels(int64_t n);
  * number required when out == NULL. */
 int64_t oracle_hierarchy(const double* a, int64_t n, double* out);
 
-/* Maximum OpenMP threads of this build. */
+/* Maximum OpenMP threads of this build / set them (omp_set_num_threads of
+ * the calling thread; 1 makes every reduction serial and deterministic). */
 int32_t oracle_max_threads(void);
+void oracle_set_num_threads(int32_t threads);
 
 #ifdef __cplusplus
 }

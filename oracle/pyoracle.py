@@ -97,6 +97,7 @@ class Oracle:
         L.oracle_hierarchy.argtypes = [dp, i64, dp]
         L.oracle_hierarchy.restype = i64
         L.oracle_max_threads.restype = i32
+        L.oracle_set_num_threads.argtypes = [i32]
 
     def solve(self, a, f=1.0, *, tol=1e-10, max_iter=10000, preconditioner="multigrid", pre_sweeps=1,
               post_sweeps=1, cycle_index=2, record_history=False, fp32=False, u0=None) -> Result:
@@ -177,6 +178,11 @@ class Oracle:
 
     def max_threads(self) -> int:
         return int(self.lib.oracle_max_threads())
+
+    def set_num_threads(self, threads: int) -> None:
+        """OpenMP threads of the calling thread (shared by every OpenMP
+        library of the process, the reference build included)."""
+        self.lib.oracle_set_num_threads(threads)
 
 
 class Reference:
