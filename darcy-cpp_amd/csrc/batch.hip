@@ -18,18 +18,27 @@ namespace darcy_hip {
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / kWave;
 
-// Largest side of the levels kept in LDS: 16 in FP64 (14 KB), 32 in FP32
-// (28 KB), so four workgroups fit on a CU.
+// Largest side of the levels kept in LDS: 16 (14 KB in FP64, 7.5 KB in FP32;
+// the FP32 kernel also keeps the 16 KB tail matrix), so four workgroups fit
+// on a CU.
 template <class T>
-struct LdsSide;
-template <>
-struct LdsSide<double> {
+struct LdsSide {
     static constexpr int value = 16;
 };
-template <>
-struct LdsSide<float> {
-    static constexpr int value = 32;
-};
+
+// LDS elements of the coarse-level pool.  The FP32 pool doubles as FP64
+// scratch while the tail matrix is built: FP64 operators of the tail levels
+// (<= 285 doubles) and x, b for one unit vector per wave (4 x 170 doubles).
+constexpr int kTailScratchDoubles = 285 + 4 * 170;
+template <class T>
+constexpr int pool_elems() {
+    constexpr int need = LdsPool<T, LdsSide<T>::value>::kElems;
+    constexpr int scratch = sizeof(T) == 4 ? 2 * kTailScratchDoubles : 0;
+    return need > scratch ? need : scratch;
+}
+// FP32 tail matrix, stored by columns (64 x 64 at most), and a residual.
+constexpr int kTailMat = 64 * 64;
+constexpr int kTailLds = kTailMat + 64;
 
 struct Shared {
     Hier h;
@@ -116,6 +125,7 @@ struct Cycle {
     const Hier& h;
     const LdsTable& t;
     T* pool;
+    float* tail;  // FP32 kernel: tail matrix by columns, then a residual vector
     CycleOpts o;
 
     DARCY_INLINE LvP<T> G(int l) const { return global_level<T>(h, l); }
@@ -124,43 +134,46 @@ struct Cycle {
         return {h.n[l], pool + f[0], pool + f[1], pool + f[2], pool + f[3], pool + f[4]};
     }
 
-    // --- one wave, everything in LDS (levels >= t.wave)
+    // --- one wave, everything in LDS (levels >= t.wave): the recursive cycle
     DARCY_INLINE void wave_cycle(int l0, bool fresh0, Exec e) const {
-        const WaveSync sync;
-        const int L = h.L;
-        uint32_t done = 0;
-        int l = l0;
-        bool fresh = fresh0;
-        for (;;) {
-            while (l + 1 < L) {
-                visit_down(S(l), S(l + 1).b, fresh, o, e, sync);
-                done &= ~(1u << l);
-                ++l;
-                fresh = true;
-            }
-            coarsest_solve(S(l), fresh, e, sync);
-            for (;;) {
-                if (l == l0) return;
-                --l;
-                if (o.gamma == 2 && !((done >> l) & 1u)) {
-                    done |= 1u << l;
-                    ++l;
-                    fresh = false;
-                    break;
-                }
-                visit_up(S(l), S(l + 1).x, o, e, sync);
-            }
-        }
+        cycle_levels<T>(l0, h.L, fresh0, o, e, WaveSync{}, [&](int l) { return S(l); });
     }
 
+    // --- FP32 cycle: the tail as its matrix.  Fresh: x = M b; repeated
+    // W-cycle visit: x += M (b - A x).  Lane i owns row i; the row sum runs
+    // over k in order.
+    DARCY_INLINE void dense_tail(int l, bool fresh, Exec e) const {
+        const LvP<T> A = S(l);
+        const int n = A.n, dim = n * n, i = e.tid;
+        const WaveSync sync;
+        float* res = tail + kTailMat;
+        const T* rhs = A.b;
+        if (!fresh) {
+            if (i < dim) {
+                const int r = i / n, c = i - r * n;
+                res[i] = A.b[i] - apply_at(n, A.ex, A.ny, r, c, [&](int q) { return A.x[q]; });
+            }
+            sync();
+            rhs = res;
+        }
+        if (i < dim) {
+            float s = 0.0f;
+            for (int k = 0; k < dim; ++k) s += tail[k * dim + i] * rhs[k];
+            A.x[i] = fresh ? s : A.x[i] + s;
+        }
+        sync();
+    }
+
+    // Global-memory levels stream one pass per visit when one sweep is
+    // asked for (the default); otherwise they run phase by phase.
     DARCY_INLINE void down(int l, bool fresh, Exec e) const {
         const BlockSync sync;
         if (l >= t.first)
             visit_down(S(l), S(l + 1).b, fresh, o, e, sync);
         else if (l + 1 >= t.first)
-            visit_down(G(l), S(l + 1).b, fresh, o, e, sync);
+            o.pre == 1 ? down_stream(G(l), S(l + 1).b, fresh, e) : visit_down(G(l), S(l + 1).b, fresh, o, e, sync);
         else
-            visit_down(G(l), G(l + 1).b, fresh, o, e, sync);
+            o.pre == 1 ? down_stream(G(l), G(l + 1).b, fresh, e) : visit_down(G(l), G(l + 1).b, fresh, o, e, sync);
     }
 
     DARCY_INLINE void up(int l, Exec e) const {
@@ -168,9 +181,11 @@ struct Cycle {
         if (l >= t.first)
             visit_up(S(l), S(l + 1).x, o, e, sync);
         else if (l + 1 >= t.first)
-            visit_up(G(l), static_cast<const T*>(S(l + 1).x), o, e, sync);
+            o.post == 1 ? up_stream(G(l), static_cast<const T*>(S(l + 1).x), e)
+                        : visit_up(G(l), static_cast<const T*>(S(l + 1).x), o, e, sync);
         else
-            visit_up(G(l), static_cast<const T*>(G(l + 1).x), o, e, sync);
+            o.post == 1 ? up_stream(G(l), static_cast<const T*>(G(l + 1).x), e)
+                        : visit_up(G(l), static_cast<const T*>(G(l + 1).x), o, e, sync);
     }
 
     // z = M r from level l0 (x_{l0} zero on entry when fresh0).
@@ -187,7 +202,13 @@ struct Cycle {
                 fresh = true;
             }
             if (l >= t.wave) {
-                if (threadIdx.x < kWave) wave_cycle(l, fresh, Exec{static_cast<int>(threadIdx.x), kWave});
+                if (threadIdx.x < kWave) {
+                    const Exec we{static_cast<int>(threadIdx.x), kWave};
+                    if constexpr (sizeof(T) == 4)
+                        dense_tail(l, fresh, we);
+                    else
+                        wave_cycle(l, fresh, we);
+                }
                 __syncthreads();
             } else if (l >= t.first) {
                 coarsest_solve(S(l), fresh, e, BlockSync{});
@@ -209,6 +230,51 @@ struct Cycle {
     }
 };
 
+// Tail matrix of the FP32 cycle.  Column k is the FP64 recursive cycle from
+// level tl -- the reference's arithmetic, bit-exact with the oracle -- applied
+// to the unit vector e_k, rounded to FP32 and stored by columns.  The four
+// waves take columns in turn; `scratch` (the LDS pool, before it receives the
+// FP32 operators) holds the FP64 operators of the tail levels and each wave's
+// x and b.
+DARCY_INLINE void build_tail_matrix(const Hier& h, int tl, const CycleOpts& o, void* scratch, float* tail) {
+    double* dp = static_cast<double*>(scratch);
+    const int L = h.L, lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    auto op_off = [&](int l) {
+        int off = 0;
+        for (int q = tl; q < l; ++q) off += 2 * h.n[q] * (h.n[q] + 1) + h.n[q] * h.n[q];
+        return off;
+    };
+    const int ops_end = op_off(L);
+    for (int l = tl; l < L; ++l) {
+        const int n = h.n[l], nf = n * (n + 1), base = op_off(l);
+        for (int q = threadIdx.x; q < nf; q += kBlock) {
+            dp[base + q] = h.ex[l][q];
+            dp[base + nf + q] = h.ny[l][q];
+        }
+        for (int q = threadIdx.x; q < n * n; q += kBlock) dp[base + 2 * nf + q] = h.id[l][q];
+    }
+    __syncthreads();
+    double* vec = dp + ops_end + wave * 170;
+    auto view = [&](int l) {
+        int off = 0;
+        for (int q = tl; q < l; ++q) off += 2 * h.n[q] * h.n[q];
+        const int n = h.n[l], base = op_off(l);
+        return LvP<double>{n, dp + base, dp + base + n * (n + 1), dp + base + 2 * n * (n + 1), vec + off,
+                           vec + off + n * n};
+    };
+    const int dim = h.n[tl] * h.n[tl];
+    const WaveSync sync;
+    for (int k = wave; k < dim; k += kWaves) {
+        const LvP<double> top = view(tl);
+        for (int q = lane; q < dim; q += kWave) top.b[q] = q == k ? 1.0 : 0.0;
+        sync();
+        cycle_levels<double>(tl, L, true, o, Exec{lane, kWave}, sync, view);
+        for (int q = lane; q < dim; q += kWave) tail[k * dim + q] = static_cast<float>(top.x[q]);
+        sync();
+    }
+    __syncthreads();
+}
+
 // --------------------------------------------------------------- a problem
 
 template <class T>
@@ -225,8 +291,8 @@ DARCY_INLINE void precondition(const Cycle<T>& cyc, const Hier& h, const SolvePa
 }
 
 template <class T>
-__device__ void run_problem(Shared& s, T* pool, const Problem& pb, ProblemStats& st, const SolveParams& P,
-                            char* slot) {
+DARCY_INLINE void run_problem(Shared& s, T* pool, float* tail, const Problem& pb, ProblemStats& st,
+                              const SolveParams& P, char* slot) {
     const Exec e{static_cast<int>(threadIdx.x), kBlock};
     const int wave = threadIdx.x / kWave;
     const int n = pb.n, N = n * n, C = chunks_of(N);
@@ -267,6 +333,9 @@ __device__ void run_problem(Shared& s, T* pool, const Problem& pb, ProblemStats&
             to_fp32(h.n[l], h.ex[l], h.ny[l], h.id[l], h.ex32[l], h.ny32[l], h.id32[l], e);
             __syncthreads();
         }
+        if constexpr (mixed) {
+            if (s.t.wave < h.L) build_tail_matrix(h, s.t.wave, CycleOpts{P.pre, P.post, P.gamma}, pool, tail);
+        }
         // Coarse operators into LDS for the whole solve.
         for (int l = s.t.first; l < h.L; ++l) {
             const LvP<T> g = global_level<T>(h, l);
@@ -280,7 +349,7 @@ __device__ void run_problem(Shared& s, T* pool, const Problem& pb, ProblemStats&
         }
         __syncthreads();
     }
-    const Cycle<T> cyc{h, s.t, pool, CycleOpts{P.pre, P.post, P.gamma}};
+    const Cycle<T> cyc{h, s.t, pool, tail, CycleOpts{P.pre, P.post, P.gamma}};
     const LvP<double> A0 = global_level<double>(h, 0);
     float* const b0f = mixed ? static_cast<float*>(h.b[0]) : nullptr;
     const float* const x0f = mixed ? static_cast<const float*>(h.x[0]) : nullptr;
@@ -435,9 +504,10 @@ __device__ void run_problem(Shared& s, T* pool, const Problem& pb, ProblemStats&
 }
 
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_batch(BatchArgs A) {
+__global__ __launch_bounds__(kBlock, 4) void k_batch(BatchArgs A) {
     __shared__ Shared s;
-    __shared__ T pool[LdsPool<T, LdsSide<T>::value>::kElems];
+    __shared__ __align__(16) T pool[pool_elems<T>()];
+    __shared__ __align__(16) float tail[sizeof(T) == 4 ? kTailLds : 1];
     char* slot = A.slots + static_cast<size_t>(blockIdx.x) * A.slot_bytes;
     for (;;) {
         if (threadIdx.x == 0) s.prob = static_cast<int>(atomicAdd(A.next, 1u));
@@ -445,7 +515,7 @@ __global__ __launch_bounds__(kBlock) void k_batch(BatchArgs A) {
         const int t = s.prob;
         if (t >= A.count) return;
         const int p = A.order ? A.order[t] : t;
-        run_problem<T>(s, pool, A.probs[p], A.stats[p], A.prm, slot);
+        run_problem<T>(s, pool, tail, A.probs[p], A.stats[p], A.prm, slot);
         __syncthreads();
     }
 }

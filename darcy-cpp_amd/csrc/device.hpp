@@ -293,6 +293,158 @@ DARCY_INLINE void visit_up(const LvP<T>& L, const T* xc, const CycleOpts& o, Exe
     }
 }
 
+// The cycle of multigrid.hpp:97-127 from level l0 to the coarsest level
+// L-1, run iteratively by the threads of `e` with barrier `sync`; `view(l)`
+// gives the LvP of level l.  Bit k of `done` marks that level k has had its
+// first coarse-grid visit (gamma <= 2).
+template <class T, class S, class V>
+DARCY_INLINE void cycle_levels(int l0, int L, bool fresh0, const CycleOpts& o, Exec e, S sync, V&& view) {
+    uint32_t done = 0;
+    int l = l0;
+    bool fresh = fresh0;
+    for (;;) {
+        while (l + 1 < L) {
+            visit_down(view(l), view(l + 1).b, fresh, o, e, sync);
+            done &= ~(1u << l);
+            ++l;
+            fresh = true;
+        }
+        coarsest_solve(view(l), fresh, e, sync);
+        for (;;) {
+            if (l == l0) return;
+            --l;
+            if (o.gamma == 2 && !((done >> l) & 1u)) {
+                done |= 1u << l;
+                ++l;
+                fresh = false;
+                break;
+            }
+            visit_up(view(l), static_cast<const T*>(view(l + 1).x), o, e, sync);
+        }
+    }
+}
+
+// ------------------------------------------- row-pipelined (streamed) visits
+// For the levels in global memory a visit is one pass over the rows instead
+// of one pass per phase: the workgroup walks down the grid R rows at a time,
+// the black update trails the red one by a row and the residual/restriction
+// trails the black one, so every row is finished while it is still in cache.
+// Values and operation order are those of the phase-by-phase visit: a black
+// cell is updated only after both red neighbours rows are, a residual only
+// after its whole 3-row neighbourhood is final.
+
+// Rows per pipeline step: about 2048 cells, even, at least 2.
+DARCY_INLINE int rows_per_step(int n) {
+    int r = 2048 / n;
+    r &= ~1;
+    return r < 2 ? 2 : r;
+}
+
+// x[k] = value(i, j) for the cells of colour `colour` in rows [ra, rb); four
+// cells per thread and step, loads before stores.
+template <class T, class F>
+DARCY_INLINE void update_rows(T* x, int n, int colour, int ra, int rb, Exec e, F&& value) {
+    const int h = (n + 1) >> 1, qa = ra * h, qb = rb * h;
+    for (int q0 = qa + e.tid; q0 < qb; q0 += 4 * e.nthr) {
+        T v[4];
+        int kk[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int q = q0 + u * e.nthr;
+            kk[u] = -1;
+            if (q < qb) {
+                const int i = q / h;
+                const int j = 2 * (q - i * h) + ((i + colour) & 1);
+                if (j < n) {
+                    kk[u] = i * n + j;
+                    v[u] = value(i, j);
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (kk[u] >= 0) x[kk[u]] = v[u];
+    }
+}
+
+// Restricted residual of the coarse rows [Ia, Ib) (level.hpp:122-137, :214-227).
+template <class T>
+DARCY_INLINE void restrict_rows(const LvP<T>& L, T* bc, int Ia, int Ib, Exec e) {
+    const int n = L.n, m = (n + 1) >> 1;
+    const T* x = L.x;
+    for (int q = Ia * m + e.tid; q < Ib * m; q += e.nthr) {
+        const int I = q / m, J = q - I * m;
+        const int i0 = 2 * I, i1 = (2 * I + 1 < n) ? 2 * I + 1 : 2 * I;
+        const int j0 = 2 * J, j1 = (2 * J + 1 < n) ? 2 * J + 1 : 2 * J;
+        T s = T(0);
+        for (int i = i0; i <= i1; ++i)
+            for (int j = j0; j <= j1; ++j)
+                s += L.b[i * n + j] - apply_at(n, L.ex, L.ny, i, j, [&](int t) { return x[t]; });
+        bc[q] = s;
+    }
+}
+
+// One pre-smoothing sweep (red, black) and the restricted residual in a
+// single pass (multigrid.hpp:107-115 with pre_sweeps = 1).
+template <class T>
+DARCY_INLINE void down_stream(const LvP<T>& L, T* bc, bool fresh, Exec e) {
+    const int n = L.n, m = (n + 1) >> 1, R = rows_per_step(n);
+    T* x = L.x;
+    const T* b = L.b;
+    const T* id = L.id;
+    auto xv = [&](int a, int c) { return x[a * n + c]; };
+    int coarse_done = 0;
+    for (int r0 = 0;; r0 += R) {
+        const int red_hi = r0 + R < n ? r0 + R : n;
+        if (r0 < n) {
+            if (fresh)  // the red sweep from zero: b * inv_d (each neighbour term is e * 0)
+                update_rows(x, n, 0, r0, red_hi, e, [&](int i, int j) { return b[i * n + j] * id[i * n + j]; });
+            else
+                update_rows(x, n, 0, r0, red_hi, e, [&](int i, int j) { return gs_at(L, i, j, b[i * n + j], xv); });
+        }
+        __syncthreads();
+        // black row i needs red rows i-1 and i+1
+        const int black_lo = r0 > 0 ? r0 - 1 : 0;
+        const int black_hi = red_hi == n ? n : red_hi - 1;
+        update_rows(x, n, 1, black_lo, black_hi, e, [&](int i, int j) { return gs_at(L, i, j, b[i * n + j], xv); });
+        __syncthreads();
+        // residual row i needs final rows i-1..i+1; coarse row I needs fine rows 2I, 2I+1
+        const int ready = black_hi == n ? n : black_hi - 1;
+        const int coarse_hi = ready == n ? m : ready / 2;
+        restrict_rows(L, bc, coarse_done, coarse_hi, e);
+        coarse_done = coarse_hi;
+        if (coarse_done == m) break;
+    }
+    __syncthreads();
+}
+
+// Prolongation and one post-smoothing sweep (black, red) in a single pass
+// (multigrid.hpp:120-126 with post_sweeps = 1).  Black cells read their red
+// neighbours as x + x_c (the prolongated values); the red update trails the
+// black one by a row.
+template <class T>
+DARCY_INLINE void up_stream(const LvP<T>& L, const T* xc, Exec e) {
+    const int n = L.n, m = (n + 1) >> 1, R = rows_per_step(n);
+    T* x = L.x;
+    const T* b = L.b;
+    auto xv = [&](int a, int c) { return x[a * n + c]; };
+    auto prolonged = [&](int a, int c) { return x[a * n + c] + xc[(a >> 1) * m + (c >> 1)]; };
+    for (int r0 = 0;; r0 += R) {
+        const int black_hi = r0 + R < n ? r0 + R : n;
+        if (r0 < n)
+            update_rows(x, n, 1, r0, black_hi, e,
+                        [&](int i, int j) { return gs_at(L, i, j, b[i * n + j], prolonged); });
+        __syncthreads();
+        // red row i needs black rows i-1 and i+1; red rows of step t never
+        // meet the black rows of step t+1, so no barrier follows.
+        const int red_lo = r0 > 0 ? r0 - 1 : 0;
+        const int red_hi = black_hi == n ? n : black_hi - 1;
+        update_rows(x, n, 0, red_lo, red_hi, e, [&](int i, int j) { return gs_at(L, i, j, b[i * n + j], xv); });
+        if (red_hi == n) break;
+    }
+    __syncthreads();
+}
+
 // ---------------------------------------------------------- LDS hierarchy
 
 // LDS pool of the coarse levels: operators and vectors of every level with

@@ -298,8 +298,56 @@ struct Cycle {
         }
     }
 
-    void visit(std::size_t l) {
+    // Dense coarse tail of the FP32 cycle (no reference counterpart).  From
+    // level `tail` -- the first level below the finest with side <= 8 -- the
+    // recursive cycle is a linear map of b (x fresh) and, on a repeated
+    // W-cycle visit, x + M (b - A x).  Its matrix M is built once in FP64 by
+    // running the reference's recursive cycle on unit vectors, rounded to T,
+    // and applied with a sequential row sum.
+    std::size_t tail = SIZE_MAX;
+    std::size_t tdim = 0;
+    std::vector<T> M;  // tdim x tdim, M[i * tdim + k]
+
+    void build_tail(const std::vector<Op<double>>& hier) {
+        for (std::size_t l = 1; l < hier.size(); ++l)
+            if (hier[l].n <= 8) {
+                tail = l;
+                break;
+            }
+        if (tail == SIZE_MAX) return;
+        Cycle<double> c;
+        c.ops.assign(hier.begin() + static_cast<std::ptrdiff_t>(tail), hier.end());
+        c.init(pre, post, gamma);
+        tdim = static_cast<std::size_t>(hier[tail].n * hier[tail].n);
+        M.assign(tdim * tdim, T(0));
+        for (std::size_t k = 0; k < tdim; ++k) {
+            for (std::size_t i = 0; i < tdim; ++i) c.b[0][i] = i == k ? 1.0 : 0.0;
+            vzero(c.x[0]);
+            c.visit(0, true);
+            for (std::size_t i = 0; i < tdim; ++i) M[i * tdim + k] = static_cast<T>(c.x[0][i]);
+        }
+    }
+
+    void dense_tail(std::size_t l, bool fresh) {
+        const T* rhs = b[l].data();
+        if (!fresh) {
+            stencil(ops[l], x[l].data(), b[l].data(), r[l].data());
+            rhs = r[l].data();
+        }
+        for (std::size_t i = 0; i < tdim; ++i) {
+            T s = T(0);
+            for (std::size_t k = 0; k < tdim; ++k) s += M[i * tdim + k] * rhs[k];
+            x[l][i] = fresh ? s : x[l][i] + s;
+        }
+    }
+
+    // `fresh`: x_l is zero on entry (the first of the gamma coarse visits).
+    void visit(std::size_t l, bool fresh) {
         const Op<T>& L = ops[l];
+        if (l == tail) {
+            dense_tail(l, fresh);
+            return;
+        }
         if (l + 1 == ops.size()) {
             coarsest(l);
             return;
@@ -311,7 +359,7 @@ struct Cycle {
         stencil(L, x[l].data(), b[l].data(), r[l].data());
         restrict_sum(L.n, r[l].data(), b[l + 1].data());
         vzero(x[l + 1]);
-        for (int g = 0; g < gamma; ++g) visit(l + 1);
+        for (int g = 0; g < gamma; ++g) visit(l + 1, g == 0);
         prolong_inject(L.n, x[l + 1].data(), x[l].data());
         for (int s = 0; s < post; ++s) {
             gs_colour(L, x[l].data(), b[l].data(), 1);
@@ -325,7 +373,7 @@ struct Cycle {
         const std::size_t m = x[0].size();
         for (std::size_t k = 0; k < m; ++k) b[0][k] = static_cast<T>(rin[k]);
         vzero(x[0]);
-        visit(0);
+        visit(0, true);
         for (std::size_t k = 0; k < m; ++k) z[k] = static_cast<double>(x[0][k]);
     }
 };
@@ -380,6 +428,7 @@ int solve_one(const double* a, const double* rhs, idx n, const oracle_opts& o, c
         if (fp32) {
             for (const auto& L : hier) c32.ops.push_back(convert<float>(L));
             c32.init(o.pre_sweeps, o.post_sweeps, o.cycle_index);
+            c32.build_tail(hier);
         }
     } else {
         hier.push_back(std::move(fine));
@@ -538,6 +587,7 @@ extern "C" int oracle_apply_preconditioner(const double* a, int64_t n, const dou
         Cycle<float> c;
         for (const auto& L : hier) c.ops.push_back(convert<float>(L));
         c.init(pre_sweeps, post_sweeps, cycle_index);
+        c.build_tail(hier);
         c.apply(r, z);
     } else {
         Cycle<double> c;
