@@ -59,29 +59,35 @@ __device__ inline double wave_sum(double s) {
 // Canonical chunked reduction.  For each chunk owned by this wave, every
 // lane loads its four cells first (`load(k)` returns a value bundle), then
 // commits them in order (`commit(k, bundle, acc)` adds to acc[0..ND)).
-template <int ND, class Load, class Commit>
+// A wave takes CPR chunks per round, issuing all their loads first.
+template <int ND, int CPR = 1, class Load, class Commit>
 DARCY_INLINE void chunk_pass(int N, int w, int nw, double* const* part, Load&& load, Commit&& commit) {
     const int lane = threadIdx.x & (kWave - 1);
     const int C = chunks_of(N);
-    for (int c = w; c < C; c += nw) {
-        decltype(load(0)) v[4];
-        bool ok[4];
+    for (int c0 = w * CPR; c0 < C; c0 += nw * CPR) {
+        decltype(load(0)) v[4 * CPR];
+        bool ok[4 * CPR];
 #pragma unroll
-        for (int m = 0; m < 4; ++m) {
-            const int k = c * kChunk + m * kWave + lane;
-            ok[m] = k < N;
-            if (ok[m]) v[m] = load(k);
+        for (int u = 0; u < 4 * CPR; ++u) {
+            const int k = (c0 + u / 4) * kChunk + (u % 4) * kWave + lane;
+            ok[u] = k < N;
+            if (ok[u]) v[u] = load(k);
         }
-        double acc[ND];
 #pragma unroll
-        for (int d = 0; d < ND; ++d) acc[d] = 0.0;
+        for (int cc = 0; cc < CPR; ++cc) {
+            double acc[ND];
 #pragma unroll
-        for (int m = 0; m < 4; ++m)
-            if (ok[m]) commit(c * kChunk + m * kWave + lane, v[m], acc);
+            for (int d = 0; d < ND; ++d) acc[d] = 0.0;
 #pragma unroll
-        for (int d = 0; d < ND; ++d) {
-            const double s = wave_sum(acc[d]);
-            if (lane == 0) part[d][c] = s;
+            for (int m = 0; m < 4; ++m)
+                if (ok[4 * cc + m]) commit((c0 + cc) * kChunk + m * kWave + lane, v[4 * cc + m], acc);
+            if (c0 + cc < C) {
+#pragma unroll
+                for (int d = 0; d < ND; ++d) {
+                    const double s = wave_sum(acc[d]);
+                    if (lane == 0) part[d][c0 + cc] = s;
+                }
+            }
         }
     }
 }

@@ -211,19 +211,12 @@ DARCY_INLINE void black_with_prolong(const LvP<T>& L, const T* xc, Exec e) {
 // b_c = P^T (b - A x) over each <= 2x2 aggregate (level.hpp:122-137 then
 // :214-227, fused: the fine residual is never stored).
 template <class T>
+DARCY_INLINE T restricted_residual(const LvP<T>& L, int q, int m);
+
+template <class T>
 DARCY_INLINE void residual_restrict(const LvP<T>& L, T* bc, Exec e) {
-    const int n = L.n, m = (n + 1) >> 1, total = m * m;
-    const T* x = L.x;
-    for (int q = e.tid; q < total; q += e.nthr) {
-        const int I = q / m, J = q - I * m;
-        const int i0 = 2 * I, i1 = (2 * I + 1 < n) ? 2 * I + 1 : 2 * I;
-        const int j0 = 2 * J, j1 = (2 * J + 1 < n) ? 2 * J + 1 : 2 * J;
-        T s = T(0);
-        for (int i = i0; i <= i1; ++i)
-            for (int j = j0; j <= j1; ++j)
-                s += L.b[i * n + j] - apply_at(n, L.ex, L.ny, i, j, [&](int t) { return x[t]; });
-        bc[q] = s;
-    }
+    const int m = (L.n + 1) >> 1, total = m * m;
+    for (int q = e.tid; q < total; q += e.nthr) bc[q] = restricted_residual(L, q, m);
 }
 
 // Coarsest solve (multigrid.hpp:129-144): exact for 1x1, otherwise 8 + 8
@@ -367,21 +360,46 @@ DARCY_INLINE void update_rows(T* x, int n, int colour, int ra, int rb, Exec e, F
     }
 }
 
-// Restricted residual of the coarse rows [Ia, Ib) (level.hpp:122-137, :214-227).
+// Restricted residual of coarse cell q: the fine residuals of its <= 2x2
+// aggregate summed in the order (i0,j0),(i0,j1),(i1,j0),(i1,j1)
+// (level.hpp:122-137, :214-227).  The aggregate is unrolled with predicates
+// so that all loads issue before the sums.
+template <class T>
+DARCY_INLINE T restricted_residual(const LvP<T>& L, int q, int m) {
+    const int n = L.n, I = q / m, J = q - I * m;
+    const T* x = L.x;
+    T r[4];
+    bool ok[4];
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+        const int i = 2 * I + (d >> 1), j = 2 * J + (d & 1);
+        ok[d] = i < n && j < n;
+        if (ok[d]) r[d] = L.b[i * n + j] - apply_at(n, L.ex, L.ny, i, j, [&](int t) { return x[t]; });
+    }
+    T s = T(0);
+#pragma unroll
+    for (int d = 0; d < 4; ++d)
+        if (ok[d]) s += r[d];
+    return s;
+}
+
+// Restricted residual of the coarse rows [Ia, Ib), two coarse cells per
+// thread and step.
 template <class T>
 DARCY_INLINE void restrict_rows(const LvP<T>& L, T* bc, int Ia, int Ib, Exec e) {
-    const int n = L.n, m = (n + 1) >> 1;
-    const T* x = L.x;
-    for (int q = Ia * m + e.tid; q < Ib * m; q += e.nthr) {
-        const int I = q / m, J = q - I * m;
-        const int i0 = 2 * I, i1 = (2 * I + 1 < n) ? 2 * I + 1 : 2 * I;
-        const int j0 = 2 * J, j1 = (2 * J + 1 < n) ? 2 * J + 1 : 2 * J;
-        T s = T(0);
-        for (int i = i0; i <= i1; ++i)
-            for (int j = j0; j <= j1; ++j)
-                s += L.b[i * n + j] - apply_at(n, L.ex, L.ny, i, j, [&](int t) { return x[t]; });
-        bc[q] = s;
+    const int n = L.n, m = (n + 1) >> 1, qb = Ib * m;
+    for (int q0 = Ia * m + e.tid; q0 < qb; q0 += 2 * e.nthr) {
+        T v[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int q = q0 + u * e.nthr;
+            if (q < qb) v[u] = restricted_residual(L, q, m);
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+            if (q0 + u * e.nthr < qb) bc[q0 + u * e.nthr] = v[u];
     }
+    (void)n;
 }
 
 // One pre-smoothing sweep (red, black) and the restricted residual in a
