@@ -61,4 +61,25 @@ struct BatchArgs {
     SolveParams prm;
 };
 
+// CG scalars of one solve, resident in device memory.
+struct GridState {
+    double bnorm, rz, alpha, beta, res, pap;
+    int it, status, first, pad;
+    unsigned int arrive;
+};
+
+struct GridCg {
+    const Hier* hd;
+    GridState* st;
+    const double* a;
+    const double* rhs;  // or nullptr: constant f
+    double f;
+    const double* u0;  // or nullptr
+    double* u;         // solution (output buffer)
+    double* history;   // or nullptr
+    double tol;
+    int max_iter;
+    int mixed;
+};
+
 }  // namespace darcy_hip

@@ -1,0 +1,294 @@
+// Workgroup-level building blocks shared by the persistent batch kernel
+// (batch.hip) and the coarse-tail kernel of the grid path (grid.hip):
+// canonical reductions, the LDS-resident coarse levels, the dense FP32 tail,
+// and the W/V cycle run by one workgroup (multigrid.hpp:97-127 of the
+// reference).
+#pragma once
+
+#include "abi_types.hpp"
+#include "setup.hpp"
+
+namespace darcy_hip {
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / kWave;
+
+// Largest side of the levels kept in LDS: 16 (14 KB in FP64, 7.5 KB in FP32;
+// the FP32 kernel also keeps the 16 KB tail matrix), so four workgroups fit
+// on a CU.
+template <class T>
+struct LdsSide {
+    static constexpr int value = 16;
+};
+
+// LDS elements of the coarse-level pool.  The FP32 pool doubles as FP64
+// scratch while the tail matrix is built: FP64 operators of the tail levels
+// (<= 285 doubles) and x, b for one unit vector per wave (4 x 170 doubles).
+constexpr int kTailScratchDoubles = 285 + 4 * 170;
+template <class T>
+constexpr int pool_elems() {
+    constexpr int need = LdsPool<T, LdsSide<T>::value>::kElems;
+    constexpr int scratch = sizeof(T) == 4 ? 2 * kTailScratchDoubles : 0;
+    return need > scratch ? need : scratch;
+}
+// FP32 tail matrix, stored by columns (64 x 64 at most), and a residual.
+constexpr int kTailMat = 64 * 64;
+constexpr int kTailLds = kTailMat + 64;
+
+struct Shared {
+    Hier h;
+    LdsTable t;
+    double red[4];
+    unsigned long long bad;
+    int prob;
+};
+
+// --------------------------------------------------------------- reductions
+
+__device__ inline double wave_sum(double s) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, kWave);
+    return __shfl(s, 0, kWave);
+}
+
+// Canonical chunked reduction.  For each chunk owned by this wave, every
+// lane loads its four cells first (`load(k)` returns a value bundle), then
+// commits them in order (`commit(k, bundle, acc)` adds to acc[0..ND)).
+// A wave takes CPR chunks per round, issuing all their loads first.
+template <int ND, int CPR = 1, class Load, class Commit>
+DARCY_INLINE void chunk_pass(int N, int w, int nw, double* const* part, Load&& load, Commit&& commit) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int C = chunks_of(N);
+    for (int c0 = w * CPR; c0 < C; c0 += nw * CPR) {
+        decltype(load(0)) v[4 * CPR];
+        bool ok[4 * CPR];
+#pragma unroll
+        for (int u = 0; u < 4 * CPR; ++u) {
+            const int k = (c0 + u / 4) * kChunk + (u % 4) * kWave + lane;
+            ok[u] = k < N;
+            if (ok[u]) v[u] = load(k);
+        }
+#pragma unroll
+        for (int cc = 0; cc < CPR; ++cc) {
+            double acc[ND];
+#pragma unroll
+            for (int d = 0; d < ND; ++d) acc[d] = 0.0;
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+                if (ok[4 * cc + m]) commit((c0 + cc) * kChunk + m * kWave + lane, v[4 * cc + m], acc);
+            if (c0 + cc < C) {
+#pragma unroll
+                for (int d = 0; d < ND; ++d) {
+                    const double s = wave_sum(acc[d]);
+                    if (lane == 0) part[d][c0 + cc] = s;
+                }
+            }
+        }
+    }
+}
+
+__device__ inline double final_sum(const double* part, int C) {
+    const int lane = threadIdx.x & (kWave - 1);
+    double s = 0.0;
+    for (int c = lane; c < C; c += kWave) s += part[c];
+    return wave_sum(s);
+}
+
+// Barrier, then one wave combines the chunk sums of `nd` reductions.
+__device__ inline void reduce_parts(Shared& s, int nd, int C) {
+    __syncthreads();
+    if (threadIdx.x < kWave) {
+        for (int d = 0; d < nd; ++d) {
+            const double v = final_sum(s.h.part[d], C);
+            if (threadIdx.x == 0) s.red[d] = v;
+        }
+    }
+    __syncthreads();
+}
+
+// ------------------------------------------------------------ the cycle
+
+template <class T>
+DARCY_INLINE LvP<T> global_level(const Hier& h, int l);
+template <>
+DARCY_INLINE LvP<double> global_level<double>(const Hier& h, int l) {
+    return {h.n[l], h.ex[l], h.ny[l], h.id[l], static_cast<double*>(h.x[l]), static_cast<double*>(h.b[l])};
+}
+template <>
+DARCY_INLINE LvP<float> global_level<float>(const Hier& h, int l) {
+    return {h.n[l], h.ex32[l], h.ny32[l], h.id32[l], static_cast<float*>(h.x[l]), static_cast<float*>(h.b[l])};
+}
+
+// The cycle of multigrid.hpp:97-127 run by one workgroup, with the coarse
+// levels in the LDS pool and the smallest ones on a single wave.
+template <class T>
+struct Cycle {
+    const Hier& h;
+    const LdsTable& t;
+    T* pool;
+    float* tail;  // FP32 kernel: tail matrix by columns, then a residual vector
+    CycleOpts o;
+
+    DARCY_INLINE LvP<T> G(int l) const { return global_level<T>(h, l); }
+    DARCY_INLINE LvP<T> S(int l) const {
+        const int* f = t.off[l];
+        return {h.n[l], pool + f[0], pool + f[1], pool + f[2], pool + f[3], pool + f[4]};
+    }
+
+    // --- one wave, everything in LDS (levels >= t.wave): the recursive cycle
+    DARCY_INLINE void wave_cycle(int l0, bool fresh0, Exec e) const {
+        cycle_levels<T>(l0, h.L, fresh0, o, e, WaveSync{}, [&](int l) { return S(l); });
+    }
+
+    // --- FP32 cycle: the tail as its matrix.  Fresh: x = M b; repeated
+    // W-cycle visit: x += M (b - A x).  Lane i owns row i; the row sum runs
+    // over k in order.
+    DARCY_INLINE void dense_tail(int l, bool fresh, Exec e) const {
+        const LvP<T> A = S(l);
+        const int n = A.n, dim = n * n, i = e.tid;
+        const WaveSync sync;
+        float* res = tail + kTailMat;
+        const T* rhs = A.b;
+        if (!fresh) {
+            if (i < dim) {
+                const int r = i / n, c = i - r * n;
+                res[i] = A.b[i] - apply_at(n, A.ex, A.ny, r, c, [&](int q) { return A.x[q]; });
+            }
+            sync();
+            rhs = res;
+        }
+        if (i < dim) {
+            float s = 0.0f;
+            for (int k = 0; k < dim; ++k) s += tail[k * dim + i] * rhs[k];
+            A.x[i] = fresh ? s : A.x[i] + s;
+        }
+        sync();
+    }
+
+    // Global-memory levels stream one pass per visit when one sweep is
+    // asked for (the default); otherwise they run phase by phase.
+    DARCY_INLINE void down(int l, bool fresh, Exec e) const {
+        const BlockSync sync;
+        if (l >= t.first)
+            visit_down(S(l), S(l + 1).b, fresh, o, e, sync);
+        else if (l + 1 >= t.first)
+            o.pre == 1 ? down_stream(G(l), S(l + 1).b, fresh, e) : visit_down(G(l), S(l + 1).b, fresh, o, e, sync);
+        else
+            o.pre == 1 ? down_stream(G(l), G(l + 1).b, fresh, e) : visit_down(G(l), G(l + 1).b, fresh, o, e, sync);
+    }
+
+    DARCY_INLINE void up(int l, Exec e) const {
+        const BlockSync sync;
+        if (l >= t.first)
+            visit_up(S(l), S(l + 1).x, o, e, sync);
+        else if (l + 1 >= t.first)
+            o.post == 1 ? up_stream(G(l), static_cast<const T*>(S(l + 1).x), e)
+                        : visit_up(G(l), static_cast<const T*>(S(l + 1).x), o, e, sync);
+        else
+            o.post == 1 ? up_stream(G(l), static_cast<const T*>(G(l + 1).x), e)
+                        : visit_up(G(l), static_cast<const T*>(G(l + 1).x), o, e, sync);
+    }
+
+    // z = M r from level l0 (x_{l0} zero on entry when fresh0).
+    DARCY_INLINE void run(int l0, bool fresh0, Exec e) const {
+        const int L = h.L;
+        uint32_t done = 0;
+        int l = l0;
+        bool fresh = fresh0;
+        for (;;) {
+            while (l + 1 < L && l < t.wave) {
+                down(l, fresh, e);
+                done &= ~(1u << l);
+                ++l;
+                fresh = true;
+            }
+            if (l >= t.wave) {
+                if (threadIdx.x < kWave) {
+                    const Exec we{static_cast<int>(threadIdx.x), kWave};
+                    if constexpr (sizeof(T) == 4)
+                        dense_tail(l, fresh, we);
+                    else
+                        wave_cycle(l, fresh, we);
+                }
+                __syncthreads();
+            } else if (l >= t.first) {
+                coarsest_solve(S(l), fresh, e, BlockSync{});
+            } else {
+                coarsest_solve(G(l), fresh, e, BlockSync{});
+            }
+            for (;;) {
+                if (l == l0) return;
+                --l;
+                if (o.gamma == 2 && !((done >> l) & 1u)) {
+                    done |= 1u << l;
+                    ++l;
+                    fresh = false;
+                    break;
+                }
+                up(l, e);
+            }
+        }
+    }
+};
+
+// Tail matrix of the FP32 cycle.  Column k is the FP64 recursive cycle from
+// level tl -- the reference's arithmetic, bit-exact with the oracle -- applied
+// to the unit vector e_k, rounded to FP32 and stored by columns.  The four
+// waves take columns in turn; `scratch` (the LDS pool, before it receives the
+// FP32 operators) holds the FP64 operators of the tail levels and each wave's
+// x and b.
+DARCY_INLINE void build_tail_matrix(const Hier& h, int tl, const CycleOpts& o, void* scratch, float* tail) {
+    double* dp = static_cast<double*>(scratch);
+    const int L = h.L, lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    auto op_off = [&](int l) {
+        int off = 0;
+        for (int q = tl; q < l; ++q) off += 2 * h.n[q] * (h.n[q] + 1) + h.n[q] * h.n[q];
+        return off;
+    };
+    const int ops_end = op_off(L);
+    for (int l = tl; l < L; ++l) {
+        const int n = h.n[l], nf = n * (n + 1), base = op_off(l);
+        for (int q = threadIdx.x; q < nf; q += kBlock) {
+            dp[base + q] = h.ex[l][q];
+            dp[base + nf + q] = h.ny[l][q];
+        }
+        for (int q = threadIdx.x; q < n * n; q += kBlock) dp[base + 2 * nf + q] = h.id[l][q];
+    }
+    __syncthreads();
+    double* vec = dp + ops_end + wave * 170;
+    auto view = [&](int l) {
+        int off = 0;
+        for (int q = tl; q < l; ++q) off += 2 * h.n[q] * h.n[q];
+        const int n = h.n[l], base = op_off(l);
+        return LvP<double>{n, dp + base, dp + base + n * (n + 1), dp + base + 2 * n * (n + 1), vec + off,
+                           vec + off + n * n};
+    };
+    const int dim = h.n[tl] * h.n[tl];
+    const WaveSync sync;
+    for (int k = wave; k < dim; k += kWaves) {
+        const LvP<double> top = view(tl);
+        for (int q = lane; q < dim; q += kWave) top.b[q] = q == k ? 1.0 : 0.0;
+        sync();
+        cycle_levels<double>(tl, L, true, o, Exec{lane, kWave}, sync, view);
+        for (int q = lane; q < dim; q += kWave) tail[k * dim + q] = static_cast<float>(top.x[q]);
+        sync();
+    }
+    __syncthreads();
+}
+
+// --------------------------------------------------------------- a problem
+
+template <class T>
+DARCY_INLINE void precondition(const Cycle<T>& cyc, const Hier& h, const SolveParams& P, int N, Exec e) {
+    if (P.precond == kPcMultigrid || P.mode == kModeApplyPreconditioner) {
+        cyc.run(0, true, e);
+    } else if (P.precond == kPcJacobi) {  // solver.hpp:127-132
+        for (int k = e.tid; k < N; k += e.nthr) h.z[k] = h.r[k] * h.id[0][k];
+        __syncthreads();
+    } else {  // solver.hpp:124-126
+        for (int k = e.tid; k < N; k += e.nthr) h.z[k] = h.r[k];
+        __syncthreads();
+    }
+}
+
+}  // namespace darcy_hip

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <numeric>
 #include <string>
@@ -16,7 +17,20 @@
 
 using namespace darcy_hip;
 
+using CycleOptsC = darcy_hip::CycleOpts;
 extern "C" hipError_t darcy_hip_launch_batch(const BatchArgs* args, int grid, int mixed, hipStream_t stream);
+extern "C" hipError_t darcy_grid_fine(const double* a, const Hier* hd, unsigned long long* bad, int blocks,
+                                      hipStream_t s);
+extern "C" hipError_t darcy_grid_galerkin(const Hier* hd, int l, int blocks, hipStream_t s);
+extern "C" hipError_t darcy_grid_fp32(const Hier* hd, int l, int blocks, hipStream_t s);
+extern "C" hipError_t darcy_grid_tail_matrix(const Hier* hd, CycleOptsC o, float* out, hipStream_t s);
+extern "C" hipError_t darcy_grid_presmooth(int mixed, const Hier* hd, int l, int kind, int blocks, hipStream_t s);
+extern "C" hipError_t darcy_grid_restrict(int mixed, const Hier* hd, int l, int blocks, hipStream_t s);
+extern "C" hipError_t darcy_grid_postsmooth(int mixed, const Hier* hd, int l, int kind, int blocks, hipStream_t s);
+extern "C" hipError_t darcy_grid_tail(int mixed, const Hier* hd, int l0, int fresh, CycleOptsC o, const float* tail,
+                                      hipStream_t s);
+extern "C" hipError_t darcy_grid_cg(int which, const GridCg* g, const double* pold, double* pnew, int blocks,
+                                    hipStream_t s);
 extern "C" hipError_t darcy_hip_batch_occupancy(int mixed, int* blocks_per_cu);
 extern "C" hipError_t darcy_hip_launch_copy(const void* src, void* dst, size_t bytes, int grid, hipStream_t stream);
 
@@ -62,7 +76,11 @@ struct darcy_ctx {
     double last_ms = 0.0;
     std::string error;
     std::mutex mu;
-    DevBuf slots, probs, stats, order, counter, staging;
+    DevBuf slots, probs, stats, order, counter, staging, grid;
+    // Single multigrid solves with n > grid_min_n run on the grid path, with
+    // levels of side <= grid_tail_side in one workgroup (tuned values).
+    int grid_min_n = 64;
+    int grid_tail_side = 64;
 };
 
 namespace {
@@ -290,6 +308,195 @@ int run(darcy_ctx* ctx, const Job& job) {
     return DARCY_OK;
 }
 
+// ------------------------------------------------------------ grid path
+
+int blocks_for(const darcy_ctx* ctx, int64_t work, int per_block) {
+    const int64_t b = (work + per_block - 1) / per_block;
+    return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(b, static_cast<int64_t>(ctx->cus) * 8)));
+}
+
+// One multigrid solve on the grid path (solver.hpp:83-174 with the cycle of
+// multigrid.hpp:97-127 recursed on the host over the large levels).
+int run_grid(darcy_ctx* ctx, const Job& job) {
+    const darcy_opts& o = *job.opts;
+    const bool mixed = o.precision == DARCY_MIXED;
+    const int n = static_cast<int>(job.n[0]);
+    const int64_t N = static_cast<int64_t>(n) * n;
+    const size_t cells = static_cast<size_t>(N) * 8;
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    hipStream_t s = ctx->stream;
+
+    // Workspace: slot, device Hier, state, bad-index word, tail matrix,
+    // history, staging of host inputs/outputs.
+    Hier h;
+    const size_t slot_bytes = plan_slot(nullptr, n, mixed, &h);
+    const size_t hist_n = static_cast<size_t>(o.max_iter) + 1;
+    const bool hist = o.record_history && job.history;
+    size_t off = 0;
+    auto take = [&](size_t b) {
+        const size_t at = off;
+        off += align_up(b);
+        return at;
+    };
+    const size_t o_slot = take(slot_bytes), o_h = take(sizeof(Hier)), o_st = take(sizeof(GridState)),
+                 o_bad = take(8), o_tail = take(64 * 64 * 4), o_hist = take(hist ? hist_n * 8 : 8);
+    const bool host = !job.device_ptrs;
+    const size_t o_a = host ? take(cells) : 0, o_rhs = host && job.rhs && job.rhs[0] ? take(cells) : 0,
+                 o_u0 = host && job.in && job.in[0] ? take(cells) : 0, o_u = host ? take(cells) : 0;
+    HIP_TRY(ctx->grid.reserve(off), "workspace allocation");
+    char* base = ctx->grid.as<char>();
+    plan_slot(base + o_slot, n, mixed, &h);
+    Hier* hd = reinterpret_cast<Hier*>(base + o_h);
+    GridState* st = reinterpret_cast<GridState*>(base + o_st);
+    auto* bad = reinterpret_cast<unsigned long long*>(base + o_bad);
+    float* tail = reinterpret_cast<float*>(base + o_tail);
+    double* history = hist ? (host ? reinterpret_cast<double*>(base + o_hist) : job.history) : nullptr;
+
+    const double* a = job.a[0];
+    const double* rhs = job.rhs ? job.rhs[0] : nullptr;
+    const double* u0 = job.in ? job.in[0] : nullptr;
+    double* u = job.out[0];
+    if (host) {
+        HIP_TRY(hipMemcpyAsync(base + o_a, a, cells, hipMemcpyHostToDevice, s), "upload");
+        a = reinterpret_cast<const double*>(base + o_a);
+        if (rhs) {
+            HIP_TRY(hipMemcpyAsync(base + o_rhs, rhs, cells, hipMemcpyHostToDevice, s), "upload");
+            rhs = reinterpret_cast<const double*>(base + o_rhs);
+        }
+        if (u0) {
+            HIP_TRY(hipMemcpyAsync(base + o_u0, u0, cells, hipMemcpyHostToDevice, s), "upload");
+            u0 = reinterpret_cast<const double*>(base + o_u0);
+        }
+        u = reinterpret_cast<double*>(base + o_u);
+    }
+    const unsigned long long none = ~0ull;
+    HIP_TRY(hipMemcpyAsync(hd, &h, sizeof(Hier), hipMemcpyHostToDevice, s), "hierarchy table");
+    HIP_TRY(hipMemcpyAsync(bad, &none, 8, hipMemcpyHostToDevice, s), "bad word");
+    HIP_TRY(hipMemsetAsync(st, 0, sizeof(GridState), s), "state reset");
+
+    HIP_TRY(hipEventRecord(ctx->ev0, s), "event record");
+    // Hierarchy (level.hpp:79-97, :171-211).
+    HIP_TRY(darcy_grid_fine(a, hd, bad, blocks_for(ctx, N, 1024), s), "fine level");
+    for (int l = 0; l + 1 < h.L; ++l) {
+        const int64_t nc = static_cast<int64_t>(h.n[l + 1]) * h.n[l + 1];
+        HIP_TRY(darcy_grid_galerkin(hd, l, blocks_for(ctx, nc, 256), s), "coarsening");
+        if (mixed) HIP_TRY(darcy_grid_fp32(hd, l, blocks_for(ctx, static_cast<int64_t>(h.n[l]) * h.n[l], 1024), s),
+                           "fp32 copy");
+    }
+    if (mixed) HIP_TRY(darcy_grid_fp32(hd, h.L - 1, 1, s), "fp32 copy");
+    const int T = std::max(32, ctx->grid_tail_side);
+    int lT = 0;
+    while (lT < h.L - 1 && h.n[lT] > T) ++lT;
+    const CycleOptsC co{o.pre_sweeps, o.post_sweeps, o.cycle_index};
+    if (mixed && h.L > 1) HIP_TRY(darcy_grid_tail_matrix(hd, co, tail, s), "tail matrix");
+
+    GridCg g{hd, st, a, rhs, job.f ? job.f[0] : 1.0, u0, u, history, o.tol, o.max_iter, mixed ? 1 : 0};
+    const int cg_blocks = blocks_for(ctx, chunks_of(N), 4);
+    GridState hs{};
+    auto read_state = [&]() -> int {
+        HIP_TRY(hipMemcpyAsync(&hs, st, sizeof(GridState), hipMemcpyDeviceToHost, s), "state readback");
+        HIP_TRY(hipStreamSynchronize(s), "solve");
+        return DARCY_OK;
+    };
+
+    // The cycle: large levels phase by phase, the tail in one workgroup.
+    int rc = DARCY_OK;
+    std::function<void(int, bool)> cycle = [&](int l, bool fresh) {
+        if (rc) return;
+        if (l >= lT) {
+            const hipError_t e = darcy_grid_tail(mixed, hd, l, fresh ? 1 : 0, co, tail, s);
+            if (e != hipSuccess) rc = hip_fail(ctx, e, "tail cycle");
+            return;
+        }
+        const int64_t nl = static_cast<int64_t>(h.n[l]) * h.n[l];
+        const int bl = blocks_for(ctx, nl, 2048), bc = blocks_for(ctx, nl / 4 + 1, 256);
+        hipError_t e = hipSuccess;
+        for (int sw = 0; sw < o.pre_sweeps && e == hipSuccess; ++sw) {
+            if (sw == 0 && fresh) {
+                e = darcy_grid_presmooth(mixed, hd, l, 0, bl, s);
+            } else {
+                e = darcy_grid_presmooth(mixed, hd, l, 1, bl, s);
+                if (e == hipSuccess) e = darcy_grid_presmooth(mixed, hd, l, 2, bl, s);
+            }
+        }
+        if (e == hipSuccess) e = darcy_grid_restrict(mixed, hd, l, bc, s);
+        if (e != hipSuccess) {
+            rc = hip_fail(ctx, e, "cycle launch");
+            return;
+        }
+        cycle(l + 1, true);
+        if (o.cycle_index == 2) cycle(l + 1, false);
+        if (rc) return;
+        e = darcy_grid_postsmooth(mixed, hd, l, 0, bl, s);
+        if (e == hipSuccess) e = darcy_grid_postsmooth(mixed, hd, l, 1, bl, s);
+        for (int sw = 1; sw < o.post_sweeps && e == hipSuccess; ++sw) {
+            e = darcy_grid_postsmooth(mixed, hd, l, 2, bl, s);
+            if (e == hipSuccess) e = darcy_grid_postsmooth(mixed, hd, l, 1, bl, s);
+        }
+        if (e != hipSuccess) rc = hip_fail(ctx, e, "cycle launch");
+    };
+
+    HIP_TRY(darcy_grid_cg(0, &g, nullptr, nullptr, cg_blocks, s), "init");
+    if (const int r = read_state()) return r;
+    unsigned long long bad_h = 0;
+    HIP_TRY(hipMemcpy(&bad_h, bad, 8, hipMemcpyDeviceToHost), "bad word");
+    if (bad_h != ~0ull) {
+        if (job.stats) {
+            darcy_stats& d = job.stats[0];
+            d = darcy_stats{0, 0, 1, kInvalid, 0.0, static_cast<int64_t>(bad_h), 0, 0};
+        }
+        char msg[160];
+        std::snprintf(msg, sizeof msg,
+                      "permeability must be finite and strictly positive everywhere; offending value at flat index %llu",
+                      bad_h);
+        return fail(ctx, DARCY_INVALID_INPUT, msg);
+    }
+    const bool zero_rhs = hs.status == kConverged && hs.bnorm == 0.0;
+    double* pold = h.p;
+    double* pnew = h.u;
+    if (hs.status == kRunning) {
+        cycle(0, true);
+        if (rc) return rc;
+        HIP_TRY(darcy_grid_cg(1, &g, nullptr, nullptr, cg_blocks, s), "rz");
+        for (;;) {
+            HIP_TRY(darcy_grid_cg(2, &g, pold, pnew, cg_blocks, s), "apply");
+            HIP_TRY(darcy_grid_cg(3, &g, pnew, nullptr, cg_blocks, s), "update");
+            if (const int r = read_state()) return r;
+            if (hs.status != kRunning) break;
+            cycle(0, true);
+            if (rc) return rc;
+            HIP_TRY(darcy_grid_cg(1, &g, nullptr, nullptr, cg_blocks, s), "rz");
+            std::swap(pold, pnew);
+        }
+    }
+    if (zero_rhs) HIP_TRY(hipMemsetAsync(u, 0, cells, s), "zero solution");
+    HIP_TRY(hipEventRecord(ctx->ev1, s), "event record");
+    if (host) HIP_TRY(hipMemcpyAsync(job.out[0], u, cells, hipMemcpyDeviceToHost, s), "result readback");
+    HIP_TRY(hipStreamSynchronize(s), "solve");
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "event timing");
+    ctx->last_ms = ms;
+    // A breakdown reports the iterations completed before it (solver.hpp:149-155).
+    const int iters = hs.it;
+    const int32_t hist_len = hist && !zero_rhs ? std::min<int32_t>(iters + 1, job.history_cap) : 0;
+    if (hist && host && hist_len > 0)
+        HIP_TRY(hipMemcpy(job.history, history, static_cast<size_t>(hist_len) * 8, hipMemcpyDeviceToHost),
+                "history readback");
+    if (job.stats) {
+        darcy_stats& d = job.stats[0];
+        d.iterations = iters;
+        d.converged = hs.status == kConverged ? 1 : 0;
+        d.levels = zero_rhs ? 1 : h.L;
+        d.status = hs.status;
+        d.residual = zero_rhs ? 0.0 : hs.res;
+        d.bad_index = -1;
+        d.history_len = hist_len;
+        d.reserved = 0;
+    }
+    ctx->error.clear();
+    return DARCY_OK;
+}
+
 }  // namespace
 
 extern "C" void darcy_default_opts(darcy_opts* o) {
@@ -351,6 +558,7 @@ extern "C" int darcy_solve(darcy_ctx* ctx, const double* a, const double* rhs, d
     if (const int rc = check_opts(ctx, o)) return rc;
     Job job{kModeSolve, &o, 1, &n, &a, &rhs, &f, &u0, &u_out, stats, history, history ? history_cap : 0,
             o.device_pointers != 0};
+    if (o.preconditioner == DARCY_PC_MULTIGRID && n > ctx->grid_min_n) return run_grid(ctx, job);
     return run(ctx, job);
 }
 
@@ -476,5 +684,31 @@ extern "C" int darcy_bandwidth_probe(darcy_ctx* ctx, size_t bytes, int32_t repea
     (void)hipFree(dst);
     if (err != hipSuccess) return hip_fail(ctx, err, "bandwidth probe");
     *gbps = 2.0 * static_cast<double>(bytes) * repeats / (ms * 1e-3) / 1e9;
+    return DARCY_OK;
+}
+
+extern "C" int darcy_ctx_set_tuning(darcy_ctx* ctx, const char* key, int32_t value) {
+    if (!ctx || !key) return DARCY_INVALID_INPUT;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    const std::string k = key;
+    if (k == "grid_min_n" && value >= 1) {
+        ctx->grid_min_n = value;
+    } else if (k == "grid_tail_side" && value >= 32) {
+        ctx->grid_tail_side = value;
+    } else {
+        return fail(ctx, DARCY_INVALID_INPUT, "unknown tuning key or value out of range: " + k);
+    }
+    return DARCY_OK;
+}
+
+extern "C" int darcy_ctx_get_tuning(const darcy_ctx* ctx, const char* key, int32_t* value) {
+    if (!ctx || !key || !value) return DARCY_INVALID_INPUT;
+    const std::string k = key;
+    if (k == "grid_min_n")
+        *value = ctx->grid_min_n;
+    else if (k == "grid_tail_side")
+        *value = ctx->grid_tail_side;
+    else
+        return DARCY_INVALID_INPUT;
     return DARCY_OK;
 }

@@ -246,11 +246,6 @@ DARCY_INLINE void coarsest_solve(const LvP<T>& L, bool fresh, Exec e, S sync) {
     }
 }
 
-struct CycleOpts {
-    int pre;
-    int post;
-    int gamma;  // 1 = V-cycle, 2 = W-cycle
-};
 
 // Going down through level l: pre-smoothing (red, black) then the restricted
 // residual into the next level's b (multigrid.hpp:107-115).

@@ -1,0 +1,376 @@
+// Grid path for large single solves (1024^2, 4096^2): each phase of a level
+// larger than the tail side T is one launch over the whole GPU; levels of
+// side <= T run inside one workgroup (the block cycle of block.hpp, with the
+// LDS levels and the dense FP32 tail); the CG vector operations are flat
+// launches with the canonical chunked reductions, finished by the last
+// workgroup to arrive, which also advances the device-resident CG scalars.
+//
+// The per-cell arithmetic, the cycle and the reduction order are those of
+// the persistent batch kernel, so a solve on either path gives the same
+// bits.  The host drives the W/V recursion over the large levels
+// (multigrid.hpp:97-127) and the PCG loop (solver.hpp:139-172).
+#include "block.hpp"
+
+namespace darcy_hip {
+
+DARCY_INLINE Exec grid_exec() {
+    return Exec{static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x), static_cast<int>(gridDim.x * blockDim.x)};
+}
+
+// ------------------------------------------------------------------ setup
+
+__global__ __launch_bounds__(kBlock) void kg_fine(const double* a, const Hier* hd, unsigned long long* bad) {
+    const Exec e = grid_exec();
+    const int n = hd->n[0];
+    const long long b = first_invalid(a, n * n, e);
+    if (b != INT64_MAX) atomicMin(bad, static_cast<unsigned long long>(b));
+    build_fine_faces(a, n, hd->ex[0], hd->ny[0], e);
+    build_fine_invdiag(a, n, hd->id[0], e);
+}
+
+__global__ __launch_bounds__(kBlock) void kg_galerkin(const Hier* hd, int l) {
+    const Exec e = grid_exec();
+    galerkin_faces(hd->n[l], hd->ex[l], hd->ny[l], hd->ex[l + 1], hd->ny[l + 1], e);
+    galerkin_invdiag(hd->n[l], hd->ex[l], hd->ny[l], hd->id[l + 1], e);
+}
+
+__global__ __launch_bounds__(kBlock) void kg_fp32(const Hier* hd, int l) {
+    const Exec e = grid_exec();
+    to_fp32(hd->n[l], hd->ex[l], hd->ny[l], hd->id[l], hd->ex32[l], hd->ny32[l], hd->id32[l], e);
+}
+
+// One workgroup: the FP32 tail matrix into global memory.
+__global__ __launch_bounds__(kBlock) void kg_tail_matrix(const Hier* hd, CycleOpts o, float* tail_out) {
+    __shared__ Shared s;
+    __shared__ __align__(16) float pool[pool_elems<float>()];
+    if (threadIdx.x == 0) {
+        s.h = *hd;
+        plan_lds<float, LdsSide<float>::value>(s.h, s.t);
+    }
+    __syncthreads();
+    if (s.t.wave < s.h.L) build_tail_matrix(s.h, s.t.wave, o, pool, tail_out);
+}
+
+// ----------------------------------------------------------- cycle phases
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void kg_presmooth(const Hier* hd, int l, int kind) {
+    const LvP<T> L = global_level<T>(*hd, l);
+    const Exec e = grid_exec();
+    if (kind == 0)
+        presmooth_from_zero(L, e);  // red + black from x = 0
+    else
+        gs_colour(L, kind == 1 ? 0 : 1, e);  // 1: red, 2: black
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void kg_restrict(const Hier* hd, int l) {
+    const LvP<T> L = global_level<T>(*hd, l);
+    residual_restrict(L, static_cast<T*>(hd->b[l + 1]), grid_exec());
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void kg_postsmooth(const Hier* hd, int l, int kind) {
+    const LvP<T> L = global_level<T>(*hd, l);
+    const Exec e = grid_exec();
+    if (kind == 0)
+        black_with_prolong(L, static_cast<const T*>(hd->x[l + 1]), e);
+    else
+        gs_colour(L, kind == 1 ? 0 : 1, e);  // 1: red, 2: black
+}
+
+// The cycle from level l0 (side <= T) down, in one workgroup.
+template <class T>
+__global__ __launch_bounds__(kBlock) void kg_tail(const Hier* hd, int l0, int fresh, CycleOpts o,
+                                                  const float* tail_in) {
+    __shared__ Shared s;
+    __shared__ __align__(16) T pool[pool_elems<T>()];
+    __shared__ __align__(16) float tail[sizeof(T) == 4 ? kTailLds : 1];
+    const Exec e{static_cast<int>(threadIdx.x), kBlock};
+    if (threadIdx.x == 0) {
+        s.h = *hd;
+        plan_lds<T, LdsSide<T>::value>(s.h, s.t);
+    }
+    __syncthreads();
+    const Hier& h = s.h;
+    for (int l = s.t.first; l < h.L; ++l) {
+        const LvP<T> g = global_level<T>(h, l);
+        const int* f = s.t.off[l];
+        const int nf = g.n * (g.n + 1), nc = g.n * g.n;
+        for (int q = e.tid; q < nf; q += e.nthr) {
+            pool[f[0] + q] = g.ex[q];
+            pool[f[1] + q] = g.ny[q];
+        }
+        for (int q = e.tid; q < nc; q += e.nthr) pool[f[2] + q] = g.id[q];
+    }
+    if constexpr (sizeof(T) == 4) {
+        if (s.t.wave < h.L) {
+            const int dim = h.n[s.t.wave] * h.n[s.t.wave];
+            for (int q = e.tid; q < dim * dim; q += e.nthr) tail[q] = tail_in[q];
+        }
+    }
+    __syncthreads();
+    const Cycle<T> cyc{h, s.t, pool, tail, o};
+    cyc.run(l0, fresh != 0, e);
+}
+
+// ------------------------------------------------------------ CG kernels
+
+// Called by every workgroup after its chunk sums are stored: true in the
+// last workgroup to arrive, which then sees every chunk sum.
+DARCY_INLINE bool last_to_arrive(GridState* st) {
+    __shared__ bool last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(&st->arrive, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (last) __threadfence();
+    return last;
+}
+
+DARCY_INLINE int grid_wave() { return static_cast<int>(blockIdx.x * kWaves + threadIdx.x / kWave); }
+DARCY_INLINE int grid_waves() { return static_cast<int>(gridDim.x * kWaves); }
+
+// r = b - A u0 or b; u = u0 or 0; <b, b> and <r, r> (solver.hpp:92-108).
+__global__ __launch_bounds__(kBlock) void kg_init(GridCg g) {
+    const Hier& h = *g.hd;
+    const int n = h.n[0], N = n * n, C = chunks_of(N);
+    const double hh = 1.0 / static_cast<double>(n + 1), h2 = hh * hh;
+    const LvP<double> A0 = global_level<double>(h, 0);
+    float* b0f = g.mixed ? static_cast<float*>(h.b[0]) : nullptr;
+    struct Init {
+        double bk, rk, u0;
+    };
+    chunk_pass<2>(
+        N, grid_wave(), grid_waves(), h.part,
+        [&](int k) {
+            Init v;
+            v.bk = (g.rhs ? g.rhs[k] : g.f) * h2;
+            v.rk = v.bk;
+            v.u0 = 0.0;
+            if (g.u0) {
+                const int i = k / n, j = k - i * n;
+                v.rk = v.bk - apply_at(n, A0.ex, A0.ny, i, j, [&](int t) { return g.u0[t]; });
+                v.u0 = g.u0[k];
+            }
+            return v;
+        },
+        [&](int k, const Init& v, double* acc) {
+            acc[0] += v.bk * v.bk;
+            g.u[k] = v.u0;
+            h.r[k] = v.rk;
+            acc[1] += v.rk * v.rk;
+            if (b0f) b0f[k] = static_cast<float>(v.rk);
+        });
+    if (!last_to_arrive(g.st)) return;
+    if (threadIdx.x < kWave) {
+        const double bb = final_sum(h.part[0], C), rr = final_sum(h.part[1], C);
+        if (threadIdx.x == 0) {
+            GridState& s = *g.st;
+            s.bnorm = sqrt(bb);
+            s.res = g.u0 ? sqrt(rr) / s.bnorm : 1.0;
+            s.it = 0;
+            s.first = 1;
+            s.beta = 0.0;
+            s.status = kRunning;
+            if (s.bnorm == 0.0)
+                s.status = kConverged;
+            else if (g.u0 && s.res < g.tol)
+                s.status = kConverged;
+            if (g.history) g.history[0] = s.res;
+            s.arrive = 0;
+        }
+    }
+}
+
+DARCY_INLINE double zval(const Hier& h, int mixed, int k) {
+    return mixed ? static_cast<double>(static_cast<const float*>(h.x[0])[k]) : h.z[k];
+}
+
+// <r, z> (and <z, A p> in the flexible variant); sets beta and rz.
+__global__ __launch_bounds__(kBlock) void kg_rz(GridCg g) {
+    const Hier& h = *g.hd;
+    const int N = h.n[0] * h.n[0], C = chunks_of(N);
+    struct Rz {
+        double r, z, ap;
+    };
+    chunk_pass<2>(
+        N, grid_wave(), grid_waves(), h.part,
+        [&](int k) { return Rz{h.r[k], zval(h, g.mixed, k), g.mixed ? h.ap[k] : 0.0}; },
+        [&](int, const Rz& v, double* acc) {
+            acc[0] += v.r * v.z;
+            if (g.mixed) acc[1] += v.z * v.ap;
+        });
+    if (!last_to_arrive(g.st)) return;
+    if (threadIdx.x < kWave) {
+        const double rz = final_sum(h.part[0], C);
+        const double zap = g.mixed ? final_sum(h.part[1], C) : 0.0;
+        if (threadIdx.x == 0) {
+            GridState& s = *g.st;
+            if (!s.first) s.beta = g.mixed ? (-s.alpha * zap) / s.rz : rz / s.rz;
+            s.rz = rz;
+            s.arrive = 0;
+        }
+    }
+}
+
+// p = z + beta p (first: p = z), ap = A p, <p, ap>; sets alpha or breakdown.
+__global__ __launch_bounds__(kBlock) void kg_apply(GridCg g, const double* pold, double* pnew) {
+    const Hier& h = *g.hd;
+    const int n = h.n[0], N = n * n, C = chunks_of(N);
+    const LvP<double> A0 = global_level<double>(h, 0);
+    const bool first = g.st->first != 0;
+    const double beta = g.st->beta;
+    struct PAp {
+        double p, ap;
+    };
+    chunk_pass<1>(
+        N, grid_wave(), grid_waves(), h.part,
+        [&](int k) {
+            const int i = k / n, j = k - i * n;
+            auto pv = [&](int t) -> double {
+                return first ? zval(h, g.mixed, t) : zval(h, g.mixed, t) + beta * pold[t];
+            };
+            PAp v;
+            v.ap = apply_at(n, A0.ex, A0.ny, i, j, pv);
+            v.p = pv(k);
+            return v;
+        },
+        [&](int k, const PAp& v, double* acc) {
+            pnew[k] = v.p;
+            h.ap[k] = v.ap;
+            acc[0] += v.p * v.ap;
+        });
+    if (!last_to_arrive(g.st)) return;
+    if (threadIdx.x < kWave) {
+        const double pap = final_sum(h.part[0], C);
+        if (threadIdx.x == 0) {
+            GridState& s = *g.st;
+            s.pap = pap;
+            if (!(fabs(pap) > 0.0) || !isfinite(pap))  // solver.hpp:149-155
+                s.status = kBreakdown;
+            else
+                s.alpha = s.rz / pap;
+            s.first = 0;
+            s.arrive = 0;
+        }
+    }
+}
+
+// u += alpha p, r -= alpha ap, <r, r>; advances the iteration count.
+__global__ __launch_bounds__(kBlock) void kg_update(GridCg g, const double* p) {
+    const Hier& h = *g.hd;
+    const int N = h.n[0] * h.n[0], C = chunks_of(N);
+    const double alpha = g.st->alpha;
+    float* b0f = g.mixed ? static_cast<float*>(h.b[0]) : nullptr;
+    struct Upd {
+        double u, p, r, ap;
+    };
+    chunk_pass<1>(
+        N, grid_wave(), grid_waves(), h.part, [&](int k) { return Upd{g.u[k], p[k], h.r[k], h.ap[k]}; },
+        [&](int k, const Upd& v, double* acc) {
+            g.u[k] = v.u + alpha * v.p;
+            const double ri = v.r - alpha * v.ap;
+            h.r[k] = ri;
+            acc[0] += ri * ri;
+            if (b0f) b0f[k] = static_cast<float>(ri);
+        });
+    if (!last_to_arrive(g.st)) return;
+    if (threadIdx.x < kWave) {
+        const double rr = final_sum(h.part[0], C);
+        if (threadIdx.x == 0) {
+            GridState& s = *g.st;
+            s.res = sqrt(rr) / s.bnorm;
+            s.it += 1;
+            if (g.history) g.history[s.it] = s.res;
+            if (s.res < g.tol)
+                s.status = kConverged;
+            else if (s.it >= g.max_iter)
+                s.status = kMaxIter;
+            s.arrive = 0;
+        }
+    }
+}
+
+template __global__ void kg_presmooth<double>(const Hier*, int, int);
+template __global__ void kg_presmooth<float>(const Hier*, int, int);
+template __global__ void kg_restrict<double>(const Hier*, int);
+template __global__ void kg_restrict<float>(const Hier*, int);
+template __global__ void kg_postsmooth<double>(const Hier*, int, int);
+template __global__ void kg_postsmooth<float>(const Hier*, int, int);
+template __global__ void kg_tail<double>(const Hier*, int, int, CycleOpts, const float*);
+template __global__ void kg_tail<float>(const Hier*, int, int, CycleOpts, const float*);
+
+}  // namespace darcy_hip
+
+// ------------------------------------------------------------ launchers
+// Plain C entry points for the host code (capi.cpp); `blocks` is chosen by
+// the caller.
+using namespace darcy_hip;
+
+extern "C" hipError_t darcy_grid_fine(const double* a, const Hier* hd, unsigned long long* bad, int blocks,
+                                      hipStream_t s) {
+    hipLaunchKernelGGL(kg_fine, dim3(blocks), dim3(kBlock), 0, s, a, hd, bad);
+    return hipGetLastError();
+}
+extern "C" hipError_t darcy_grid_galerkin(const Hier* hd, int l, int blocks, hipStream_t s) {
+    hipLaunchKernelGGL(kg_galerkin, dim3(blocks), dim3(kBlock), 0, s, hd, l);
+    return hipGetLastError();
+}
+extern "C" hipError_t darcy_grid_fp32(const Hier* hd, int l, int blocks, hipStream_t s) {
+    hipLaunchKernelGGL(kg_fp32, dim3(blocks), dim3(kBlock), 0, s, hd, l);
+    return hipGetLastError();
+}
+extern "C" hipError_t darcy_grid_tail_matrix(const Hier* hd, CycleOpts o, float* out, hipStream_t s) {
+    hipLaunchKernelGGL(kg_tail_matrix, dim3(1), dim3(kBlock), 0, s, hd, o, out);
+    return hipGetLastError();
+}
+extern "C" hipError_t darcy_grid_presmooth(int mixed, const Hier* hd, int l, int kind, int blocks, hipStream_t s) {
+    if (mixed)
+        hipLaunchKernelGGL(kg_presmooth<float>, dim3(blocks), dim3(kBlock), 0, s, hd, l, kind);
+    else
+        hipLaunchKernelGGL(kg_presmooth<double>, dim3(blocks), dim3(kBlock), 0, s, hd, l, kind);
+    return hipGetLastError();
+}
+extern "C" hipError_t darcy_grid_restrict(int mixed, const Hier* hd, int l, int blocks, hipStream_t s) {
+    if (mixed)
+        hipLaunchKernelGGL(kg_restrict<float>, dim3(blocks), dim3(kBlock), 0, s, hd, l);
+    else
+        hipLaunchKernelGGL(kg_restrict<double>, dim3(blocks), dim3(kBlock), 0, s, hd, l);
+    return hipGetLastError();
+}
+extern "C" hipError_t darcy_grid_postsmooth(int mixed, const Hier* hd, int l, int kind, int blocks,
+                                            hipStream_t s) {
+    if (mixed)
+        hipLaunchKernelGGL(kg_postsmooth<float>, dim3(blocks), dim3(kBlock), 0, s, hd, l, kind);
+    else
+        hipLaunchKernelGGL(kg_postsmooth<double>, dim3(blocks), dim3(kBlock), 0, s, hd, l, kind);
+    return hipGetLastError();
+}
+extern "C" hipError_t darcy_grid_tail(int mixed, const Hier* hd, int l0, int fresh, CycleOpts o, const float* tail,
+                                      hipStream_t s) {
+    if (mixed)
+        hipLaunchKernelGGL(kg_tail<float>, dim3(1), dim3(kBlock), 0, s, hd, l0, fresh, o, tail);
+    else
+        hipLaunchKernelGGL(kg_tail<double>, dim3(1), dim3(kBlock), 0, s, hd, l0, fresh, o, tail);
+    return hipGetLastError();
+}
+extern "C" hipError_t darcy_grid_cg(int which, const GridCg* g, const double* pold, double* pnew, int blocks,
+                                    hipStream_t s) {
+    switch (which) {
+        case 0:
+            hipLaunchKernelGGL(kg_init, dim3(blocks), dim3(kBlock), 0, s, *g);
+            break;
+        case 1:
+            hipLaunchKernelGGL(kg_rz, dim3(blocks), dim3(kBlock), 0, s, *g);
+            break;
+        case 2:
+            hipLaunchKernelGGL(kg_apply, dim3(blocks), dim3(kBlock), 0, s, *g, pold, pnew);
+            break;
+        default:
+            hipLaunchKernelGGL(kg_update, dim3(blocks), dim3(kBlock), 0, s, *g, pold);
+            break;
+    }
+    return hipGetLastError();
+}
+extern "C" size_t darcy_grid_state_bytes() { return sizeof(GridState); }

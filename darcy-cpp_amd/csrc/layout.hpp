@@ -36,6 +36,13 @@ constexpr int kMaxLevels = 32;   // multigrid.hpp:46
 constexpr int kChunk = 256;      // cells per canonical reduction chunk
 constexpr size_t kAlign = 256;   // byte alignment of every array
 
+// Options of the multigrid cycle (multigrid.hpp:29-47).
+struct CycleOpts {
+    int pre;
+    int post;
+    int gamma;  // 1 = V-cycle, 2 = W-cycle
+};
+
 DARCY_HD inline size_t align_up(size_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
 
 // Level count for side n: coarsen with (n + 1) / 2 until 1x1 or 32 levels

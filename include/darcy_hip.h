@@ -86,6 +86,16 @@ void darcy_default_opts(darcy_opts* opts);
 int darcy_ctx_create(int device, darcy_ctx** out);
 void darcy_ctx_destroy(darcy_ctx* ctx);
 
+/* Launch-shape tuning of a context (no reference counterpart; set by the
+ * autotuner, or by hand):
+ *   "grid_min_n"      single multigrid solves with n above this use the
+ *                     multi-workgroup grid path (default 64)
+ *   "grid_tail_side"  on the grid path, levels with side <= this run in one
+ *                     workgroup (>= 32, default 64)
+ * Neither changes any result: only the launch structure. */
+int darcy_ctx_set_tuning(darcy_ctx* ctx, const char* key, int32_t value);
+int darcy_ctx_get_tuning(const darcy_ctx* ctx, const char* key, int32_t* value);
+
 /* Message of the last failed call on this context ("" if none). */
 const char* darcy_last_error(const darcy_ctx* ctx);
 
