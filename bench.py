@@ -98,8 +98,111 @@ def cpu_baseline(n: int, sample: int, cycle: int) -> dict:
     }
 
 
+def suite_large(args) -> int:
+    """Large single solves (1024^2 and 4096^2, two-phase 12/3 and 1e3/1e-3
+    media, seed 0): FP64 and FP32-cycle paths, event time, iterations and,
+    with --oracle, the relative L2 distance to the CPU oracle's solution."""
+    from darcy.capi import Context
+
+    oracle = None
+    if args.oracle:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+
+        oracle = pyoracle.Oracle()
+    ctx = Context(0)
+    for n in args.sizes:
+        for medium, hi, lo in (("two_phase_12_3", 12.0, 3.0), ("contrast_1e6", 1e3, 1e-3)):
+            a = two_phase_permeability(gaussian_random_field(n, seed=0), hi=hi, lo=lo)
+            ab, ub = ctx.alloc(a.nbytes), ctx.alloc(a.nbytes)
+            ctx.upload(ab, a)
+            ref = oracle.solve(a, max_iter=20000) if oracle is not None else None
+            for precision in ("fp64", "mixed"):
+                ctx.solve_device(n, ab, ub, precision=precision, max_iter=20000)  # warm-up
+                times = []
+                for _ in range(max(1, args.steps)):
+                    t0 = time.perf_counter()
+                    r = ctx.solve_device(n, ab, ub, precision=precision, max_iter=20000)
+                    times.append((time.perf_counter() - t0) * 1e3)
+                u = np.empty_like(a)
+                ctx.download(u, ub)
+                line = {"suite": "large", "n": n, "medium": medium, "precision": precision,
+                        "iterations": r["iterations"], "levels": r["levels"], "converged": r["converged"],
+                        "residual": r["residual"], "ms_wall_min": round(min(times), 3),
+                        "ms_event": round(ctx.last_kernel_ms(), 3)}
+                if ref is not None:
+                    line["oracle_iterations"] = ref.iterations
+                    line["rel_l2_vs_oracle"] = float(np.linalg.norm(u - ref.u) / np.linalg.norm(ref.u))
+                print(json.dumps(line), flush=True)
+            ctx.free(ab)
+            ctx.free(ub)
+    ctx.close()
+    return 0
+
+
+def uneven_sizes(count: int) -> np.ndarray:
+    """Grid sides of the uneven batch: default_rng(0).choice over the seven sizes."""
+    return np.random.default_rng(0).choice(np.array([64, 96, 128, 192, 256, 384, 512]), size=count)
+
+
+def suite_uneven(args) -> int:
+    """2000 problems, side drawn from {64,...,512} by seed 0, problem i on GRF
+    seed i, two-phase 12/3, solved in one persistent launch per step."""
+    from darcy.capi import Context
+
+    count = args.batch if args.batch != 1000 else 2000
+    sizes = uneven_sizes(count)
+    media = [two_phase_permeability(gaussian_random_field(int(n), seed=i)) for i, n in enumerate(sizes)]
+    ctx = Context(0)
+    total = int(sum(int(n) * int(n) for n in sizes))
+    ab, ub = ctx.alloc(total * 8), ctx.alloc(total * 8)
+    a_ptrs, u_ptrs, off = [], [], 0
+    for m in media:
+        ctx.upload(ab + off * 8, m)
+        a_ptrs.append(ab + off * 8)
+        u_ptrs.append(ub + off * 8)
+        off += m.size
+    out = {}
+    for precision in ("fp64", "mixed"):
+        ctx.solve_batch_device([int(n) for n in sizes], a_ptrs, u_ptrs, precision=precision)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            r = ctx.solve_batch_device([int(n) for n in sizes], a_ptrs, u_ptrs, precision=precision)
+        ms = (time.perf_counter() - t0) / args.steps * 1e3
+        it = r["iterations"]
+        out[precision] = {"solves_per_s": round(count / (ms * 1e-3), 1), "ms_per_step": round(ms, 2),
+                          "kernel_ms": round(ctx.last_kernel_ms(), 2), "all_converged": bool(r["converged"].all()),
+                          "iterations_by_size": {int(s): float(it[sizes == s].mean()) for s in np.unique(sizes)}}
+    ctx.free(ab)
+    ctx.free(ub)
+    ctx.close()
+    line = {"suite": "uneven", "count": count, "cells": total, "sizes": {int(s): int((sizes == s).sum())
+                                                                         for s in np.unique(sizes)}, **out}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+
+        solver = pyoracle.Reference() if pyoracle.reference_available() else pyoracle.Oracle()
+        threads = max(1, min(len(os.sched_getaffinity(0)), 16))
+        t_total, solved = 0.0, 0
+        for s in np.unique(sizes):  # the reference batch needs one n per call
+            idx = np.nonzero(sizes == s)[0][: args.cpu_sample // 7 + 1]
+            batch = np.stack([media[i] for i in idx])
+            t0 = time.perf_counter()
+            solver.solve_batch(batch, threads=threads)
+            t_total += time.perf_counter() - t0
+            solved += len(idx)
+        line["cpu_baseline_sample"] = {"solves_per_s_on_sample": round(solved / t_total, 1), "cores": threads,
+                                       "problems": solved, "note": "per-size-group calls, sample of each group"}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
+    ap.add_argument("--suite", choices=["metric", "large", "uneven"], default="metric")
+    ap.add_argument("--sizes", type=int, nargs="+", default=[1024, 4096])
+    ap.add_argument("--oracle", action="store_true", help="large suite: compare with the CPU oracle")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
@@ -118,6 +221,10 @@ def main() -> int:
 
     from darcy.capi import Context
 
+    if args.suite == "large":
+        return suite_large(args)
+    if args.suite == "uneven":
+        return suite_uneven(args)
     if args.copy_only:
         with Context(0) as c:
             print(json.dumps({"copy_MiB": args.copy_only, "GBs": c.bandwidth_probe(args.copy_only << 20, 5)}))

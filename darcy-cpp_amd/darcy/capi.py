@@ -67,6 +67,9 @@ def _load() -> C.CDLL:
     lib.darcy_last_kernel_ms.argtypes = [vp, dp]
     lib.darcy_device_info.argtypes = [vp, C.c_char_p, i32]
     lib.darcy_bandwidth_probe.argtypes = [vp, C.c_size_t, i32, dp]
+    lib.darcy_solve.argtypes = [vp, vp, vp, C.c_double, i64, C.POINTER(Opts), vp, vp, C.POINTER(Stats), dp, i32]
+    lib.darcy_ctx_set_tuning.argtypes = [vp, C.c_char_p, i32]
+    lib.darcy_ctx_get_tuning.argtypes = [vp, C.c_char_p, C.POINTER(i32)]
     return lib
 
 
@@ -133,6 +136,29 @@ class Context:
         return ms.value
 
     # ------------------------------------------------------------ solve
+    def set_tuning(self, key: str, value: int) -> None:
+        self._check(self.lib.darcy_ctx_set_tuning(self.ptr, key.encode(), value))
+
+    def get_tuning(self, key: str) -> int:
+        v = C.c_int32()
+        self._check(self.lib.darcy_ctx_get_tuning(self.ptr, key.encode(), C.byref(v)))
+        return v.value
+
+    def solve_device(self, n: int, a_ptr: int, u_ptr: int, *, u0_ptr: int | None = None, f: float = 1.0,
+                     tol: float = 1e-10, max_iter: int = 10000, cycle_index: int = 2,
+                     precision: str = "fp64") -> dict:
+        """One solve whose a / u (and u0) live in device memory."""
+        o = Opts()
+        self.lib.darcy_default_opts(C.byref(o))
+        o.tol, o.max_iter, o.cycle_index = tol, max_iter, cycle_index
+        o.precision, o.device_pointers = PRECISION[precision], 1
+        st = Stats()
+        self._check(self.lib.darcy_solve(self.ptr, C.c_void_p(a_ptr), None, C.c_double(f), C.c_int64(n),
+                                         C.byref(o), C.c_void_p(u0_ptr) if u0_ptr else None, C.c_void_p(u_ptr),
+                                         C.byref(st), None, 0))
+        return {"iterations": st.iterations, "converged": bool(st.converged), "residual": st.residual,
+                "levels": st.levels}
+
     def solve_batch_device(
         self,
         sizes: Sequence[int],
