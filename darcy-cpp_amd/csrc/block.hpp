@@ -87,10 +87,20 @@ DARCY_INLINE void chunk_pass(int N, int w, int nw, double* const* part, Load&& l
     }
 }
 
+// Lane l adds chunk sums l, l+64, l+128, ... in that order; the loads of
+// sixteen terms are issued before their adds.
 __device__ inline double final_sum(const double* part, int C) {
     const int lane = threadIdx.x & (kWave - 1);
     double s = 0.0;
-    for (int c = lane; c < C; c += kWave) s += part[c];
+    int c = lane;
+    for (; c + 15 * kWave < C; c += 16 * kWave) {
+        double v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) v[u] = part[c + u * kWave];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) s += v[u];
+    }
+    for (; c < C; c += kWave) s += part[c];
     return wave_sum(s);
 }
 

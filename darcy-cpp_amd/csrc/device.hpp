@@ -80,16 +80,26 @@ struct Exec {
 
 // (A x)_k with off-grid neighbours dropped (level.hpp:105-119).  `X` maps a
 // flat index to the value of x there, so callers can form x on the fly.
+//
+// The stencil is branch-free so that every load of a cell issues at once:
+// an off-grid neighbour is read at the cell itself and its product is
+// selected away for an exact zero (the reference skips the term; results
+// differ at most in the sign of a zero).  The product, not the coefficient,
+// is selected: the value read in place of a missing neighbour may be stale
+// or uninitialised.
 template <class T, class X>
 DARCY_INLINE T apply_at(int n, const T* ex, const T* ny, int i, int j, X&& x) {
     const int k = i * n + j, f = i * (n + 1) + j;
     const T e = ex[f + 1], w = ex[f], nn = ny[(i + 1) * n + j], s = ny[i * n + j];
     const T d = ((e + w) + nn) + s;
-    T v = d * x(k);
-    if (j + 1 < n) v -= e * x(k + 1);
-    if (j > 0) v -= w * x(k - 1);
-    if (i + 1 < n) v -= nn * x(k + n);
-    if (i > 0) v -= s * x(k - n);
+    const bool hE = j + 1 < n, hW = j > 0, hN = i + 1 < n, hS = i > 0;
+    const T xk = x(k), xE = x(hE ? k + 1 : k), xW = x(hW ? k - 1 : k), xN = x(hN ? k + n : k),
+            xS = x(hS ? k - n : k);
+    T v = d * xk;
+    v -= hE ? e * xE : T(0);
+    v -= hW ? w * xW : T(0);
+    v -= hN ? nn * xN : T(0);
+    v -= hS ? s * xS : T(0);
     return v;
 }
 
@@ -98,12 +108,17 @@ DARCY_INLINE T apply_at(int n, const T* ex, const T* ny, int i, int j, X&& x) {
 template <class T, class X>
 DARCY_INLINE T gs_at(const LvP<T>& L, int i, int j, T bk, X&& x) {
     const int n = L.n, f = i * (n + 1) + j;
+    const bool hE = j + 1 < n, hW = j > 0, hN = i + 1 < n, hS = i > 0;
+    const T e = L.ex[f + 1], w = L.ex[f], nn = L.ny[(i + 1) * n + j], s = L.ny[i * n + j];
+    const T xE = x(i, hE ? j + 1 : j), xW = x(i, hW ? j - 1 : j), xN = x(hN ? i + 1 : i, j),
+            xS = x(hS ? i - 1 : i, j);
+    const T inv = L.id[i * n + j];
     T v = bk;
-    if (j + 1 < n) v += L.ex[f + 1] * x(i, j + 1);
-    if (j > 0) v += L.ex[f] * x(i, j - 1);
-    if (i + 1 < n) v += L.ny[(i + 1) * n + j] * x(i + 1, j);
-    if (i > 0) v += L.ny[i * n + j] * x(i - 1, j);
-    return v * L.id[i * n + j];
+    v += hE ? e * xE : T(0);
+    v += hW ? w * xW : T(0);
+    v += hN ? nn * xN : T(0);
+    v += hS ? s * xS : T(0);
+    return v * inv;
 }
 
 // ------------------------------------------------------------------ phases
