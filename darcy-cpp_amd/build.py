@@ -28,7 +28,7 @@ INCLUDE = os.path.join(ROOT, "include")
 ARCH = os.environ.get("DARCY_OFFLOAD_ARCH", "gfx950")
 VERSION = "1.0.0"
 
-HIP_SOURCES = ["batch.hip", "grid.hip", "capi.cpp"]
+HIP_SOURCES = ["batch.hip", "grid.hip", "conv.hip", "capi.cpp", "net.cpp"]
 HOST_SOURCES = ["host/api.cpp", "bindings.cpp"]
 
 HIPCC_FLAGS = [

@@ -14,6 +14,7 @@
 
 #include "../../include/darcy_hip.h"
 #include "abi_types.hpp"
+#include "ctx.hpp"
 
 using namespace darcy_hip;
 
@@ -33,55 +34,6 @@ extern "C" hipError_t darcy_grid_cg(int which, const GridCg* g, const double* po
                                     hipStream_t s);
 extern "C" hipError_t darcy_hip_batch_occupancy(int mixed, int* blocks_per_cu);
 extern "C" hipError_t darcy_hip_launch_copy(const void* src, void* dst, size_t bytes, int grid, hipStream_t stream);
-
-namespace {
-
-// A growable device allocation owned by the context.
-struct DevBuf {
-    void* ptr = nullptr;
-    size_t bytes = 0;
-    hipError_t reserve(size_t want) {
-        if (want <= bytes) return hipSuccess;
-        if (ptr) (void)hipFree(ptr);
-        ptr = nullptr;
-        bytes = 0;
-        const hipError_t e = hipMalloc(&ptr, want);
-        if (e == hipSuccess) bytes = want;
-        return e;
-    }
-    void release() {
-        if (ptr) (void)hipFree(ptr);
-        ptr = nullptr;
-        bytes = 0;
-    }
-    template <class T>
-    T* as() const {
-        return static_cast<T*>(ptr);
-    }
-};
-
-struct Failure {
-    int code;
-    std::string msg;
-};
-
-}  // namespace
-
-struct darcy_ctx {
-    int device = 0;
-    int cus = 0;
-    std::string arch;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    double last_ms = 0.0;
-    std::string error;
-    std::mutex mu;
-    DevBuf slots, probs, stats, order, counter, staging, grid;
-    // Single multigrid solves with n > grid_min_n run on the grid path, with
-    // levels of side <= grid_tail_side in one workgroup (tuned values).
-    int grid_min_n = 64;
-    int grid_tail_side = 64;
-};
 
 namespace {
 

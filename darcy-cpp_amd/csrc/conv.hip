@@ -1,0 +1,627 @@
+// Convolution kernels of the learned warm start: a BF16 encoder-decoder
+// with FP32 accumulation, on the matrix cores of gfx950.
+//
+// Every 3x3 convolution with a channel count that is a multiple of 8 runs as
+// an implicit GEMM on v_mfma_f32_16x16x32_bf16: a workgroup (4 waves) owns a
+// 64 x 64 tile of C, each wave a 32 x 32 sub-tile of 2 x 2 MFMA tiles, and the
+// K loop steps by 32 with the next step's operands loaded while the current
+// step's MFMAs run.  The operands are never materialised: `gather` functors
+// read the 8 consecutive K values a lane needs straight from the NHWC
+// activations (one 16-byte load when the 8 values are contiguous).
+//
+//   forward        Z[pixel][co]      = sum_(tap,ci) X[pixel@tap][ci] * W[co][tap][ci] + bias[co]
+//   backward data  dX[pixel][ci]     = sum_(tap,co) dZ[pixel@-tap][co] * WT[ci][tap][co]
+//   backward wt.   dW[co][(tap,ci)]  = sum_pixel    dZT[co][pixel] * X[pixel@tap][ci]
+//                  db[co]            = sum_pixel    dZT[co][pixel]            (a column of ones)
+//
+// The weight gradient splits K (pixels) over blockIdx.z and sums the FP32
+// partials in split order, so gradients do not depend on scheduling.  The
+// one-channel head is too narrow for the matrix cores and runs in FP32.
+// Activations are NHWC bf16; weights W[co][tap][ci] bf16 from FP32 masters.
+// No reference counterpart: the reference solve always starts from u = 0.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+namespace darcy_nn {
+
+typedef __bf16 bf16;
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+constexpr int kT = 256;  // threads per workgroup
+constexpr int kTile = 64;
+
+__device__ __forceinline__ float bf2f(bf16 v) { return static_cast<float>(v); }
+__device__ __forceinline__ bf16 f2bf(float v) { return static_cast<bf16>(v); }
+
+// GELU, tanh form: 0.5 z (1 + tanh(u)), u = c (z + 0.044715 z^3), c = sqrt(2/pi),
+// evaluated as z * s with s = 1 / (1 + exp(-2u)) (= (1 + tanh u) / 2): no
+// cancellation in 1 + tanh u for negative z.  Derivative:
+// s + 2 z s (1 - s) c (1 + 3 * 0.044715 z^2).
+__device__ __forceinline__ float gelu_s(float z) {
+    const float c = 0.7978845608028654f;
+    return 1.0f / (1.0f + expf(-2.0f * c * (z + 0.044715f * z * z * z)));
+}
+__device__ __forceinline__ float gelu(float z) { return z * gelu_s(z); }
+__device__ __forceinline__ float gelu_grad(float z) {
+    const float c = 0.7978845608028654f;
+    const float s = gelu_s(z);
+    return s + 2.0f * z * s * (1.0f - s) * c * (1.0f + 3.0f * 0.044715f * z * z);
+}
+
+// Activation tensor geometry (NHWC).
+struct Geo {
+    int b, h, w, c;
+};
+
+__device__ __forceinline__ size_t at(const Geo& g, int b, int y, int x) {
+    return ((static_cast<size_t>(b) * g.h + y) * g.w + x) * g.c;
+}
+
+// ------------------------------------------------------------ gathers
+// load8(row, k) returns the operand values (row, k .. k+7); zero outside.
+
+// im2col of the forward conv: row = output pixel, k = tap * Cin + ci.
+struct FwdA {
+    const bf16* x;
+    Geo in;
+    int oh, ow, stride, M;
+    __device__ v8bf load8(int m, int k) const {
+        v8bf r = {};
+        if (m >= M) return r;
+        const int ox = m % ow, t = m / ow, oy = t % oh, b = t / oh;
+        const int tap = k / in.c, ci = k - tap * in.c;
+        if (tap >= 9) return r;
+        const int iy = oy * stride + tap / 3 - 1, ix = ox * stride + tap % 3 - 1;
+        if (iy < 0 || iy >= in.h || ix < 0 || ix >= in.w) return r;
+        return *reinterpret_cast<const v8bf*>(x + at(in, b, iy, ix) + ci);
+    }
+};
+
+// Row-major [N][K] matrix used as B (B[k][n] = w[n][k]): weights in the
+// forward and backward-data GEMMs.
+struct RowsB {
+    const bf16* w;
+    int K, N;
+    __device__ v8bf load8(int k, int n) const {
+        v8bf r = {};
+        if (n >= N || k >= K) return r;
+        return *reinterpret_cast<const v8bf*>(w + static_cast<size_t>(n) * K + k);
+    }
+};
+
+// Backward data: row = input pixel (y, x), k = tap * Cout + co.  The output
+// pixel that reads (y, x) through tap (ky, kx) is ((y+1-ky)/s, (x+1-kx)/s)
+// when the division is exact and the pixel lies on the output grid.
+struct BwdDataA {
+    const bf16* dz;
+    Geo out;  // geometry of the conv output (and of dz)
+    int ih, iw, stride, M;
+    __device__ v8bf load8(int m, int k) const {
+        v8bf r = {};
+        if (m >= M) return r;
+        const int x = m % iw, t = m / iw, y = t % ih, b = t / ih;
+        const int tap = k / out.c, co = k - tap * out.c;
+        if (tap >= 9) return r;
+        int oy = y + 1 - tap / 3, ox = x + 1 - tap % 3;
+        if (stride == 2) {
+            if ((oy & 1) || (ox & 1)) return r;
+            oy >>= 1;
+            ox >>= 1;
+        }
+        if (oy < 0 || oy >= out.h || ox < 0 || ox >= out.w) return r;
+        return *reinterpret_cast<const v8bf*>(dz + at(out, b, oy, ox) + co);
+    }
+};
+
+// Row-major [M][K] matrix used as A: dZ^T in the weight gradient.
+struct RowsA {
+    const bf16* a;
+    int K, M;
+    __device__ v8bf load8(int m, int k) const {
+        v8bf r = {};
+        if (m >= M || k >= K) return r;
+        return *reinterpret_cast<const v8bf*>(a + static_cast<size_t>(m) * K + k);
+    }
+};
+
+// Weight gradient B: B[pixel][n] = X at the input pixel under tap n / Cin for
+// n < 9 Cin, and 1 in column 9 Cin (the bias).  P is a multiple of 8.
+struct BwdWtB {
+    const bf16* x;
+    Geo in;
+    int oh, ow, stride, P, NW;  // NW = 9 * Cin
+    __device__ v8bf load8(int p0, int n) const {
+        v8bf r = {};
+        if (n > NW || p0 >= P) return r;
+        if (n == NW) {
+#pragma unroll
+            for (int t = 0; t < 8; ++t) r[t] = f2bf(1.0f);
+            return r;
+        }
+        const int tap = n / in.c, ci = n - tap * in.c, ky = tap / 3 - 1, kx = tap % 3 - 1;
+        if ((ow & 7) == 0) {  // the 8 pixels share one output row
+            const int ox0 = p0 % ow, q = p0 / ow, oy = q % oh, b = q / oh;
+            const int iy = oy * stride + ky;
+            if (iy < 0 || iy >= in.h) return r;
+            const bf16* row = x + at(in, b, iy, 0) + ci;
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const int ix = (ox0 + t) * stride + kx;
+                if (ix >= 0 && ix < in.w) r[t] = row[static_cast<size_t>(ix) * in.c];
+            }
+            return r;
+        }
+        for (int t = 0; t < 8; ++t) {
+            const int p = p0 + t, ox = p % ow, q = p / ow, oy = q % oh, b = q / oh;
+            const int iy = oy * stride + ky, ix = ox * stride + kx;
+            if (iy >= 0 && iy < in.h && ix >= 0 && ix < in.w) r[t] = x[at(in, b, iy, ix) + ci];
+        }
+        return r;
+    }
+};
+
+// ------------------------------------------------------------ epilogues
+
+// Forward: z = acc + bias; stores z (if wanted) and gelu(z) (if wanted).
+struct EpiFwd {
+    bf16* z;
+    bf16* act;
+    const float* bias;
+    int M, N;
+    __device__ void operator()(int m, int n, float acc) const {
+        if (m >= M || n >= N) return;
+        const float v = acc + bias[n];
+        const size_t i = static_cast<size_t>(m) * N + n;
+        if (z) z[i] = f2bf(v);
+        if (act) act[i] = f2bf(gelu(v));
+    }
+};
+
+// Backward data: dX = acc.
+struct EpiStore {
+    bf16* out;
+    int M, N;
+    __device__ void operator()(int m, int n, float acc) const {
+        if (m >= M || n >= N) return;
+        out[static_cast<size_t>(m) * N + n] = f2bf(acc);
+    }
+};
+
+// Weight gradient: FP32 partial of split blockIdx.z.
+struct EpiPartial {
+    float* part;
+    int M, N;
+    __device__ void operator()(int m, int n, float acc) const {
+        if (m >= M || n >= N) return;
+        part[(static_cast<size_t>(blockIdx.z) * M + m) * N + n] = acc;
+    }
+};
+
+// ------------------------------------------------------------ the GEMM
+// v_mfma_f32_16x16x32_bf16 operand layout: lane l holds
+//   A[m = l % 16][k = 8 (l / 16) + t] and B[k = 8 (l / 16) + t][n = l % 16], t = 0..7,
+// and receives D[m = 4 (l / 16) + t][n = l % 16], t = 0..3.
+template <class GA, class GB, class Epi>
+__global__ __launch_bounds__(kT) void k_gemm(int kchunk, int K, GA ga, GB gb, Epi epi) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int m0 = blockIdx.x * kTile + (wave & 1) * 32, n0 = blockIdx.y * kTile + (wave >> 1) * 32;
+    const int r = lane & 15, kq = (lane >> 4) * 8;
+    const int kbeg = blockIdx.z * kchunk, kend = min(K, kbeg + kchunk);
+    v4f acc[2][2] = {};
+    if (kbeg < kend) {
+        v8bf a0 = ga.load8(m0 + r, kbeg + kq), a1 = ga.load8(m0 + 16 + r, kbeg + kq);
+        v8bf b0 = gb.load8(kbeg + kq, n0 + r), b1 = gb.load8(kbeg + kq, n0 + 16 + r);
+        for (int k0 = kbeg; k0 < kend; k0 += 32) {
+            // Operands of the next step (past kend they are not used).
+            const int kn = k0 + 32 < kend ? k0 + 32 + kq : kend;
+            const v8bf na0 = ga.load8(m0 + r, kn), na1 = ga.load8(m0 + 16 + r, kn);
+            const v8bf nb0 = gb.load8(kn, n0 + r), nb1 = gb.load8(kn, n0 + 16 + r);
+            acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, acc[1][1], 0, 0, 0);
+            a0 = na0;
+            a1 = na1;
+            b0 = nb0;
+            b1 = nb1;
+        }
+    }
+    const int rq = (lane >> 4) * 4;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) epi(m0 + 16 * i + rq + t, n0 + 16 * j + r, acc[i][j][t]);
+}
+
+// Sum of the split-K partials in split order: column NW of each row is the
+// bias gradient, the others the weight gradient [M][NW].
+__global__ __launch_bounds__(kT) void k_reduce_splits(const float* part, int splits, int M, int NW, float* dw,
+                                                      float* db) {
+    const int N = NW + 1;
+    const size_t count = static_cast<size_t>(M) * N;
+    for (size_t i = blockIdx.x * static_cast<size_t>(kT) + threadIdx.x; i < count; i += static_cast<size_t>(gridDim.x) * kT) {
+        float s = 0.0f;
+        for (int z = 0; z < splits; ++z) s += part[static_cast<size_t>(z) * count + i];
+        const int m = static_cast<int>(i / N), n = static_cast<int>(i % N);
+        if (n < NW)
+            dw[static_cast<size_t>(m) * NW + n] = s;
+        else
+            db[m] = s;
+    }
+}
+
+// ------------------------------------------------- the head (Cin -> 1, FP32)
+
+__global__ __launch_bounds__(kT) void k_head_fwd(const bf16* x, Geo in, const float* w, const float* bias, float* y) {
+    const size_t P = static_cast<size_t>(in.b) * in.h * in.w;
+    for (size_t p = blockIdx.x * static_cast<size_t>(kT) + threadIdx.x; p < P; p += static_cast<size_t>(gridDim.x) * kT) {
+        const int ox = static_cast<int>(p % in.w), t = static_cast<int>(p / in.w), oy = t % in.h, b = t / in.h;
+        float s = bias[0];
+        for (int tap = 0; tap < 9; ++tap) {
+            const int iy = oy + tap / 3 - 1, ix = ox + tap % 3 - 1;
+            if (iy < 0 || iy >= in.h || ix < 0 || ix >= in.w) continue;
+            const bf16* xi = x + at(in, b, iy, ix);
+            const float* wt = w + tap * in.c;
+            for (int ci = 0; ci < in.c; ci += 8) {
+                const v8bf v = *reinterpret_cast<const v8bf*>(xi + ci);
+#pragma unroll
+                for (int t = 0; t < 8; ++t) s += wt[ci + t] * bf2f(v[t]);
+            }
+        }
+        y[p] = s;
+    }
+}
+
+// dX[pixel][ci] = sum_tap w[tap][ci] dy[pixel - tap].
+__global__ __launch_bounds__(kT) void k_head_bwd_data(const float* dy, Geo in, const float* w, bf16* dx) {
+    const size_t P = static_cast<size_t>(in.b) * in.h * in.w;
+    for (size_t q = blockIdx.x * static_cast<size_t>(kT) + threadIdx.x; q < P * in.c; q += static_cast<size_t>(gridDim.x) * kT) {
+        const int ci = static_cast<int>(q % in.c);
+        const size_t p = q / in.c;
+        const int x = static_cast<int>(p % in.w), t = static_cast<int>(p / in.w), y = t % in.h, b = t / in.h;
+        float s = 0.0f;
+        for (int tap = 0; tap < 9; ++tap) {
+            const int oy = y + 1 - tap / 3, ox = x + 1 - tap % 3;
+            if (oy < 0 || oy >= in.h || ox < 0 || ox >= in.w) continue;
+            s += w[tap * in.c + ci] * dy[(static_cast<size_t>(b) * in.h + oy) * in.w + ox];
+        }
+        dx[q] = f2bf(s);
+    }
+}
+
+// Head weight gradient, stage 1: workgroup g sums pixels [g per, (g+1) per)
+// for all 9 Cin + 1 weights (thread i owns weights i, i + 256, ...).
+__global__ __launch_bounds__(kT) void k_head_dw_part(const bf16* x, Geo in, const float* dy, int per, float* part) {
+    const int nw = 9 * in.c;
+    const size_t P = static_cast<size_t>(in.b) * in.h * in.w;
+    const size_t p0 = static_cast<size_t>(blockIdx.x) * per;
+    for (int widx = threadIdx.x; widx <= nw; widx += kT) {
+        float s = 0.0f;
+        const int ci = widx % in.c, tap = widx / in.c;
+        for (size_t p = p0; p < p0 + per && p < P; ++p) {
+            if (widx == nw) {
+                s += dy[p];
+                continue;
+            }
+            const int ox = static_cast<int>(p % in.w), t = static_cast<int>(p / in.w), oy = t % in.h, b = t / in.h;
+            const int iy = oy + tap / 3 - 1, ix = ox + tap % 3 - 1;
+            if (iy < 0 || iy >= in.h || ix < 0 || ix >= in.w) continue;
+            s += dy[p] * bf2f(x[at(in, b, iy, ix) + ci]);
+        }
+        part[static_cast<size_t>(blockIdx.x) * (nw + 1) + widx] = s;
+    }
+}
+
+// ------------------------------------------------------------ elementwise
+
+// Input features, NHWC bf16 with 8 channels: (ln a - mu) / sd, 1, and the
+// position sin(pi (i + 1/2) / n) sin(pi (j + 1/2) / n); channels 3..7 are 0.
+// mu, sd map the two phases ln 3 and ln 12 to -1 and +1.
+__global__ __launch_bounds__(kT) void k_features(const double* const* a, int n, int batch, bf16* x) {
+    const size_t per = static_cast<size_t>(n) * n;
+    const double mu = 0.5 * (log(3.0) + log(12.0)), sd = 0.5 * (log(12.0) - log(3.0));
+    const double pi = 3.14159265358979323846;
+    for (size_t q = blockIdx.x * static_cast<size_t>(kT) + threadIdx.x; q < per * batch; q += static_cast<size_t>(gridDim.x) * kT) {
+        const size_t b = q / per, k = q % per;
+        const int i = static_cast<int>(k / n), j = static_cast<int>(k % n);
+        const double pos = sin(pi * (i + 0.5) / n) * sin(pi * (j + 0.5) / n);
+        v8bf v = {};
+        v[0] = f2bf(static_cast<float>((log(a[b][k]) - mu) / sd));
+        v[1] = f2bf(1.0f);
+        v[2] = f2bf(static_cast<float>(pos));
+        *reinterpret_cast<v8bf*>(x + 8 * q) = v;
+    }
+}
+
+// Decoder merge: z = up2(c) + skip, act = gelu(z) (nearest upsampling).
+__global__ __launch_bounds__(kT) void k_up_add_gelu(const bf16* c, Geo cg, const bf16* skip, bf16* z, bf16* act) {
+    const int H = cg.h * 2, W = cg.w * 2;
+    const size_t count = static_cast<size_t>(cg.b) * H * W * cg.c;
+    for (size_t q = blockIdx.x * static_cast<size_t>(kT) + threadIdx.x; q < count; q += static_cast<size_t>(gridDim.x) * kT) {
+        const int ch = static_cast<int>(q % cg.c);
+        const size_t p = q / cg.c;
+        const int x = static_cast<int>(p % W), t = static_cast<int>(p / W), y = t % H, b = t / H;
+        const float v = bf2f(c[at(cg, b, y / 2, x / 2) + ch]) + bf2f(skip[q]);
+        if (z) z[q] = f2bf(v);
+        act[q] = f2bf(gelu(v));
+    }
+}
+
+// d <- d * gelu'(z), in place.
+__global__ __launch_bounds__(kT) void k_gelu_bwd(bf16* d, const bf16* z, size_t count) {
+    for (size_t q = blockIdx.x * static_cast<size_t>(kT) + threadIdx.x; q < count; q += static_cast<size_t>(gridDim.x) * kT)
+        d[q] = f2bf(bf2f(d[q]) * gelu_grad(bf2f(z[q])));
+}
+
+// Gradient of the upsampled branch: dc = sum of dz over each 2x2 block.
+__global__ __launch_bounds__(kT) void k_down_sum(const bf16* dz, Geo cg, bf16* dc) {
+    const size_t count = static_cast<size_t>(cg.b) * cg.h * cg.w * cg.c;
+    const Geo fine = {cg.b, cg.h * 2, cg.w * 2, cg.c};
+    for (size_t q = blockIdx.x * static_cast<size_t>(kT) + threadIdx.x; q < count; q += static_cast<size_t>(gridDim.x) * kT) {
+        const int ch = static_cast<int>(q % cg.c);
+        const size_t p = q / cg.c;
+        const int x = static_cast<int>(p % cg.w), t = static_cast<int>(p / cg.w), y = t % cg.h, b = t / cg.h;
+        float s = 0.0f;
+        for (int dy = 0; dy < 2; ++dy)
+            for (int dx = 0; dx < 2; ++dx) s += bf2f(dz[at(fine, b, 2 * y + dy, 2 * x + dx) + ch]);
+        dc[q] = f2bf(s);
+    }
+}
+
+// dst <- dst + src.
+__global__ __launch_bounds__(kT) void k_add(bf16* dst, const bf16* src, size_t count) {
+    for (size_t q = blockIdx.x * static_cast<size_t>(kT) + threadIdx.x; q < count; q += static_cast<size_t>(gridDim.x) * kT)
+        dst[q] = f2bf(bf2f(dst[q]) + bf2f(src[q]));
+}
+
+// [P][C] -> [C][P].
+__global__ __launch_bounds__(kT) void k_transpose(const bf16* in, int P, int C, bf16* out) {
+    __shared__ bf16 tile[32][33];
+    const int p0 = blockIdx.x * 32, c0 = blockIdx.y * 32, tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int y = ty; y < 32; y += 8)
+        if (p0 + y < P && c0 + tx < C) tile[y][tx] = in[static_cast<size_t>(p0 + y) * C + c0 + tx];
+    __syncthreads();
+    for (int y = ty; y < 32; y += 8)
+        if (c0 + y < C && p0 + tx < P) out[static_cast<size_t>(c0 + y) * P + p0 + tx] = tile[tx][y];
+}
+
+// u0 = f * scale * pos * y, FP64 per problem.
+__global__ __launch_bounds__(kT) void k_output(const float* y, int n, int batch, const double* f, double scale,
+                                               double* const* u0) {
+    const size_t per = static_cast<size_t>(n) * n;
+    const double pi = 3.14159265358979323846;
+    for (size_t q = blockIdx.x * static_cast<size_t>(kT) + threadIdx.x; q < per * batch; q += static_cast<size_t>(gridDim.x) * kT) {
+        const size_t b = q / per, k = q % per;
+        const int i = static_cast<int>(k / n), j = static_cast<int>(k % n);
+        const float pos = static_cast<float>(sin(pi * (i + 0.5) / n) * sin(pi * (j + 0.5) / n));
+        u0[b][k] = f[b] * scale * static_cast<double>(pos * y[q]);
+    }
+}
+
+// MSE against t = u / scale:  L = mean((pos y - t)^2),  dy = 2 (pos y - t) pos / count.
+// Each workgroup writes its partial sum of squares (divided by count).
+__global__ __launch_bounds__(kT) void k_mse(const float* y, const double* const* u, int n, int batch, double scale,
+                                            float* dy, float* partial) {
+    __shared__ float red[kT];
+    const size_t per = static_cast<size_t>(n) * n, count = per * batch;
+    const double pi = 3.14159265358979323846;
+    float s = 0.0f;
+    for (size_t q = blockIdx.x * static_cast<size_t>(kT) + threadIdx.x; q < count; q += static_cast<size_t>(gridDim.x) * kT) {
+        const size_t b = q / per, k = q % per;
+        const int i = static_cast<int>(k / n), j = static_cast<int>(k % n);
+        const float pos = static_cast<float>(sin(pi * (i + 0.5) / n) * sin(pi * (j + 0.5) / n));
+        const float d = pos * y[q] - static_cast<float>(u[b][k] / scale);
+        dy[q] = 2.0f * d * pos / static_cast<float>(count);
+        s += d * d;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = kT / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0] / static_cast<float>(count);
+}
+
+// Adam on the FP32 masters; refreshes the bf16 copy (wb may be null).
+__global__ __launch_bounds__(kT) void k_adam(float* w, const float* g, float* m, float* v, bf16* wb, size_t count,
+                                             float lr, float b1, float b2, float eps, float c1, float c2) {
+    for (size_t i = blockIdx.x * static_cast<size_t>(kT) + threadIdx.x; i < count; i += static_cast<size_t>(gridDim.x) * kT) {
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.0f - b1) * gi, vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+        w[i] = wi;
+        if (wb) wb[i] = f2bf(wi);
+    }
+}
+
+__global__ __launch_bounds__(kT) void k_to_bf16(const float* in, bf16* out, size_t count) {
+    for (size_t i = blockIdx.x * static_cast<size_t>(kT) + threadIdx.x; i < count; i += static_cast<size_t>(gridDim.x) * kT)
+        out[i] = f2bf(in[i]);
+}
+
+// WT[ci][tap][co] = W[co][tap][ci].
+__global__ __launch_bounds__(kT) void k_weight_t(const bf16* w, int cout, int cin, bf16* wt) {
+    const size_t count = static_cast<size_t>(cout) * 9 * cin;
+    for (size_t i = blockIdx.x * static_cast<size_t>(kT) + threadIdx.x; i < count; i += static_cast<size_t>(gridDim.x) * kT) {
+        const int ci = static_cast<int>(i % cin), tap = static_cast<int>((i / cin) % 9), co = static_cast<int>(i / (9 * cin));
+        wt[(static_cast<size_t>(ci) * 9 + tap) * cout + co] = w[i];
+    }
+}
+
+// Lane-layout check of the MFMA: C (16 x 16) = A (16 x 32) * B (32 x 16), row-major.
+__global__ void k_mfma_probe(const bf16* A, const bf16* B, float* C) {
+    const int lane = threadIdx.x, r = lane & 15, kq = (lane >> 4) * 8;
+    v8bf a, b;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        a[t] = A[r * 32 + kq + t];
+        b[t] = B[(kq + t) * 16 + r];
+    }
+    v4f acc = {};
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) C[((lane >> 4) * 4 + t) * 16 + r] = acc[t];
+}
+
+inline unsigned ew_grid(size_t count) {
+    const size_t g = (count + kT - 1) / kT;
+    return static_cast<unsigned>(g < 8192 ? (g ? g : 1) : 8192);
+}
+
+}  // namespace darcy_nn
+
+using namespace darcy_nn;
+
+// ------------------------------------------------------------ launchers
+// All pointers are device pointers; shapes were checked by the caller.
+extern "C" {
+
+hipError_t darcy_nn_mfma_probe(const void* A, const void* B, float* C, hipStream_t s) {
+    k_mfma_probe<<<1, 64, 0, s>>>(static_cast<const bf16*>(A), static_cast<const bf16*>(B), C);
+    return hipGetLastError();
+}
+
+// z / act: [b][h/stride][w/stride][cout] (either may be null).
+hipError_t darcy_nn_conv_fwd(const void* x, int b, int h, int w, int cin, const void* wgt, const float* bias, int cout,
+                             int stride, void* z, void* act, hipStream_t s) {
+    const int oh = h / stride, ow = w / stride, M = b * oh * ow;
+    FwdA ga{static_cast<const bf16*>(x), Geo{b, h, w, cin}, oh, ow, stride, M};
+    RowsB gb{static_cast<const bf16*>(wgt), 9 * cin, cout};
+    EpiFwd epi{static_cast<bf16*>(z), static_cast<bf16*>(act), bias, M, cout};
+    const dim3 grid((M + kTile - 1) / kTile, (cout + kTile - 1) / kTile, 1);
+    k_gemm<<<grid, kT, 0, s>>>(9 * cin, 9 * cin, ga, gb, epi);
+    return hipGetLastError();
+}
+
+// dx: [b][h][w][cin]; dz: [b][h/stride][w/stride][cout]; wt: [cin][9][cout].
+hipError_t darcy_nn_conv_bwd_data(const void* dz, int b, int h, int w, int cin, const void* wt, int cout, int stride,
+                                  void* dx, hipStream_t s) {
+    const int M = b * h * w;
+    BwdDataA ga{static_cast<const bf16*>(dz), Geo{b, h / stride, w / stride, cout}, h, w, stride, M};
+    RowsB gb{static_cast<const bf16*>(wt), 9 * cout, cin};
+    EpiStore epi{static_cast<bf16*>(dx), M, cin};
+    const dim3 grid((M + kTile - 1) / kTile, (cin + kTile - 1) / kTile, 1);
+    k_gemm<<<grid, kT, 0, s>>>(9 * cout, 9 * cout, ga, gb, epi);
+    return hipGetLastError();
+}
+
+// Weight and bias gradient.  dzt: [cout][P] (P = b * oh * ow); part:
+// splits * cout * (9 cin + 1) floats; dw: [cout][9][cin]; db: [cout].
+hipError_t darcy_nn_conv_bwd_weight(const void* x, int b, int h, int w, int cin, const void* dzt, int cout, int stride,
+                                    int splits, float* part, float* dw, float* db, hipStream_t s) {
+    const int oh = h / stride, ow = w / stride, P = b * oh * ow, NW = 9 * cin;
+    RowsA ga{static_cast<const bf16*>(dzt), P, cout};
+    BwdWtB gb{static_cast<const bf16*>(x), Geo{b, h, w, cin}, oh, ow, stride, P, NW};
+    EpiPartial epi{part, cout, NW + 1};
+    int kchunk = (P + splits - 1) / splits;
+    kchunk = (kchunk + 31) / 32 * 32;
+    const dim3 grid((cout + kTile - 1) / kTile, (NW + 1 + kTile - 1) / kTile, splits);
+    k_gemm<<<grid, kT, 0, s>>>(kchunk, P, ga, gb, epi);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    k_reduce_splits<<<ew_grid(static_cast<size_t>(cout) * (NW + 1)), kT, 0, s>>>(part, splits, cout, NW, dw, db);
+    return hipGetLastError();
+}
+
+hipError_t darcy_nn_head_fwd(const void* x, int b, int h, int w, int cin, const float* wgt, const float* bias, float* y,
+                             hipStream_t s) {
+    k_head_fwd<<<ew_grid(static_cast<size_t>(b) * h * w), kT, 0, s>>>(static_cast<const bf16*>(x), Geo{b, h, w, cin}, wgt,
+                                                                       bias, y);
+    return hipGetLastError();
+}
+
+hipError_t darcy_nn_head_bwd_data(const float* dy, int b, int h, int w, int cin, const float* wgt, void* dx,
+                                  hipStream_t s) {
+    k_head_bwd_data<<<ew_grid(static_cast<size_t>(b) * h * w * cin), kT, 0, s>>>(dy, Geo{b, h, w, cin}, wgt,
+                                                                                  static_cast<bf16*>(dx));
+    return hipGetLastError();
+}
+
+// part: groups * (9 cin + 1) floats with groups = ceil(b h w / per); summed
+// in group order into dw [9 cin] and db [1].
+hipError_t darcy_nn_head_bwd_weight(const void* x, int b, int h, int w, int cin, const float* dy, int per, float* part,
+                                    float* dw, float* db, hipStream_t s) {
+    const size_t P = static_cast<size_t>(b) * h * w;
+    const int groups = static_cast<int>((P + per - 1) / per);
+    k_head_dw_part<<<groups, kT, 0, s>>>(static_cast<const bf16*>(x), Geo{b, h, w, cin}, dy, per, part);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    k_reduce_splits<<<ew_grid(9 * cin + 1), kT, 0, s>>>(part, groups, 1, 9 * cin, dw, db);
+    return hipGetLastError();
+}
+
+hipError_t darcy_nn_features(const double* const* a, int n, int batch, void* x, hipStream_t s) {
+    k_features<<<ew_grid(static_cast<size_t>(n) * n * batch), kT, 0, s>>>(a, n, batch, static_cast<bf16*>(x));
+    return hipGetLastError();
+}
+
+// c: [b][h][w][ch] (coarse); skip, z, act: [b][2h][2w][ch]; z may be null.
+hipError_t darcy_nn_up_add_gelu(const void* c, int b, int h, int w, int ch, const void* skip, void* z, void* act,
+                                hipStream_t s) {
+    k_up_add_gelu<<<ew_grid(static_cast<size_t>(b) * h * w * ch * 4), kT, 0, s>>>(
+        static_cast<const bf16*>(c), Geo{b, h, w, ch}, static_cast<const bf16*>(skip), static_cast<bf16*>(z),
+        static_cast<bf16*>(act));
+    return hipGetLastError();
+}
+
+hipError_t darcy_nn_gelu_bwd(void* d, const void* z, size_t count, hipStream_t s) {
+    k_gelu_bwd<<<ew_grid(count), kT, 0, s>>>(static_cast<bf16*>(d), static_cast<const bf16*>(z), count);
+    return hipGetLastError();
+}
+
+// dc: [b][h][w][ch] (coarse); dz: [b][2h][2w][ch].
+hipError_t darcy_nn_down_sum(const void* dz, int b, int h, int w, int ch, void* dc, hipStream_t s) {
+    k_down_sum<<<ew_grid(static_cast<size_t>(b) * h * w * ch), kT, 0, s>>>(static_cast<const bf16*>(dz), Geo{b, h, w, ch},
+                                                                            static_cast<bf16*>(dc));
+    return hipGetLastError();
+}
+
+hipError_t darcy_nn_add(void* dst, const void* src, size_t count, hipStream_t s) {
+    k_add<<<ew_grid(count), kT, 0, s>>>(static_cast<bf16*>(dst), static_cast<const bf16*>(src), count);
+    return hipGetLastError();
+}
+
+hipError_t darcy_nn_transpose(const void* in, int P, int C, void* out, hipStream_t s) {
+    const dim3 grid((P + 31) / 32, (C + 31) / 32);
+    k_transpose<<<grid, kT, 0, s>>>(static_cast<const bf16*>(in), P, C, static_cast<bf16*>(out));
+    return hipGetLastError();
+}
+
+hipError_t darcy_nn_output(const float* y, int n, int batch, const double* f, double scale, double* const* u0,
+                           hipStream_t s) {
+    k_output<<<ew_grid(static_cast<size_t>(n) * n * batch), kT, 0, s>>>(y, n, batch, f, scale, u0);
+    return hipGetLastError();
+}
+
+// partial: `blocks` floats.
+hipError_t darcy_nn_mse(const float* y, const double* const* u, int n, int batch, double scale, float* dy,
+                        float* partial, int blocks, hipStream_t s) {
+    k_mse<<<blocks, kT, 0, s>>>(y, u, n, batch, scale, dy, partial);
+    return hipGetLastError();
+}
+
+hipError_t darcy_nn_adam(float* w, const float* g, float* m, float* v, void* wb, size_t count, float lr, float b1,
+                         float b2, float eps, float c1, float c2, hipStream_t s) {
+    k_adam<<<ew_grid(count), kT, 0, s>>>(w, g, m, v, static_cast<bf16*>(wb), count, lr, b1, b2, eps, c1, c2);
+    return hipGetLastError();
+}
+
+hipError_t darcy_nn_to_bf16(const float* in, void* out, size_t count, hipStream_t s) {
+    k_to_bf16<<<ew_grid(count), kT, 0, s>>>(in, static_cast<bf16*>(out), count);
+    return hipGetLastError();
+}
+
+hipError_t darcy_nn_weight_t(const void* w, int cout, int cin, void* wt, hipStream_t s) {
+    k_weight_t<<<ew_grid(static_cast<size_t>(cout) * 9 * cin), kT, 0, s>>>(static_cast<const bf16*>(w), cout, cin,
+                                                                            static_cast<bf16*>(wt));
+    return hipGetLastError();
+}
+
+}  // extern "C"

@@ -1,0 +1,54 @@
+// The context behind the opaque darcy_ctx handle of the C ABI: one device,
+// one stream, timing events, the last error and the growable workspace.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <string>
+
+#include "../../include/darcy_hip.h"
+
+namespace darcy_hip {
+
+// A growable device allocation owned by the context.
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    hipError_t reserve(size_t want) {
+        if (want <= bytes) return hipSuccess;
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+        const hipError_t e = hipMalloc(&ptr, want);
+        if (e == hipSuccess) bytes = want;
+        return e;
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    template <class T>
+    T* as() const {
+        return static_cast<T*>(ptr);
+    }
+};
+
+}  // namespace darcy_hip
+
+struct darcy_ctx {
+    int device = 0;
+    int cus = 0;
+    std::string arch;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+    std::string error;
+    std::mutex mu;
+    darcy_hip::DevBuf slots, probs, stats, order, counter, staging, grid;
+    // Single multigrid solves with n > grid_min_n run on the grid path, with
+    // levels of side <= grid_tail_side in one workgroup (tuned values).
+    int grid_min_n = 64;
+    int grid_tail_side = 64;
+};

@@ -70,7 +70,27 @@ def _load() -> C.CDLL:
     lib.darcy_solve.argtypes = [vp, vp, vp, C.c_double, i64, C.POINTER(Opts), vp, vp, C.POINTER(Stats), dp, i32]
     lib.darcy_ctx_set_tuning.argtypes = [vp, C.c_char_p, i32]
     lib.darcy_ctx_get_tuning.argtypes = [vp, C.c_char_p, C.POINTER(i32)]
+    # Learned warm start.
+    fp, u16p = C.POINTER(C.c_float), C.POINTER(C.c_uint16)
+    lib.darcy_net_param_count.argtypes = []
+    lib.darcy_net_param_count.restype = i64
+    lib.darcy_net_create.argtypes = [vp, i64, i64, i32, C.POINTER(vp)]
+    lib.darcy_net_destroy.argtypes = [vp]
+    lib.darcy_net_destroy.restype = None
+    lib.darcy_net_set_params.argtypes = [vp, fp, i64]
+    lib.darcy_net_get_params.argtypes = [vp, fp, i64]
+    lib.darcy_net_predict.argtypes = [vp, i64, C.POINTER(vp), dp, C.POINTER(vp)]
+    lib.darcy_net_train_step.argtypes = [vp, i64, C.POINTER(vp), C.POINTER(vp), C.c_float, dp]
+    lib.darcy_net_gradient.argtypes = [vp, i64, C.POINTER(vp), C.POINTER(vp), dp, fp]
+    lib.darcy_nn_conv_forward.argtypes = [vp, i32, i32, i32, i32, i32, i32, u16p, u16p, fp, u16p, u16p]
+    lib.darcy_nn_conv_backward_data.argtypes = [vp, i32, i32, i32, i32, i32, i32, u16p, u16p, u16p]
+    lib.darcy_nn_conv_backward_weight.argtypes = [vp, i32, i32, i32, i32, i32, i32, i32, u16p, u16p, fp, fp]
+    lib.darcy_nn_mfma_check.argtypes = [vp, u16p, u16p, fp]
     return lib
+
+
+def _ptr(arr: np.ndarray, ctype):
+    return arr.ctypes.data_as(C.POINTER(ctype))
 
 
 class Context:
@@ -195,3 +215,112 @@ class Context:
             "residual": np.array([s.residual for s in st]),
             "levels": np.array([s.levels for s in st], dtype=np.int64),
         }
+
+    # ------------------------------------------------- conv layer checks
+    # bf16 tensors travel as uint16 bit patterns.
+    def conv_forward(self, x: np.ndarray, w: np.ndarray, bias: np.ndarray, stride: int,
+                     want_act: bool = True) -> tuple[np.ndarray, np.ndarray | None]:
+        b, h, wd, cin = x.shape
+        cout = w.shape[0]
+        x = np.ascontiguousarray(x, dtype=np.uint16)
+        w = np.ascontiguousarray(w, dtype=np.uint16)
+        bias = np.ascontiguousarray(bias, dtype=np.float32)
+        z = np.empty((b, h // stride, wd // stride, cout), dtype=np.uint16)
+        act = np.empty_like(z) if want_act else None
+        self._check(self.lib.darcy_nn_conv_forward(
+            self.ptr, b, h, wd, cin, cout, stride, _ptr(x, C.c_uint16), _ptr(w, C.c_uint16),
+            _ptr(bias, C.c_float), _ptr(z, C.c_uint16), _ptr(act, C.c_uint16) if want_act else None))
+        return z, act
+
+    def conv_backward_data(self, dz: np.ndarray, w: np.ndarray, in_shape: tuple[int, ...],
+                           stride: int) -> np.ndarray:
+        b, h, wd, cin = in_shape
+        cout = w.shape[0]
+        dz = np.ascontiguousarray(dz, dtype=np.uint16)
+        w = np.ascontiguousarray(w, dtype=np.uint16)
+        dx = np.empty((b, h, wd, cin), dtype=np.uint16)
+        self._check(self.lib.darcy_nn_conv_backward_data(
+            self.ptr, b, h, wd, cin, cout, stride, _ptr(dz, C.c_uint16), _ptr(w, C.c_uint16), _ptr(dx, C.c_uint16)))
+        return dx
+
+    def conv_backward_weight(self, x: np.ndarray, dz: np.ndarray, stride: int,
+                             splits: int = 1) -> tuple[np.ndarray, np.ndarray]:
+        b, h, wd, cin = x.shape
+        cout = dz.shape[3]
+        x = np.ascontiguousarray(x, dtype=np.uint16)
+        dz = np.ascontiguousarray(dz, dtype=np.uint16)
+        dw = np.empty((cout, 3, 3, cin), dtype=np.float32)
+        db = np.empty(cout, dtype=np.float32)
+        self._check(self.lib.darcy_nn_conv_backward_weight(
+            self.ptr, b, h, wd, cin, cout, stride, splits, _ptr(x, C.c_uint16), _ptr(dz, C.c_uint16),
+            _ptr(dw, C.c_float), _ptr(db, C.c_float)))
+        return dw, db
+
+    def mfma_check(self, a: np.ndarray, b: np.ndarray) -> np.ndarray:
+        a = np.ascontiguousarray(a, dtype=np.uint16)
+        b = np.ascontiguousarray(b, dtype=np.uint16)
+        assert a.shape == (16, 32) and b.shape == (32, 16)
+        c = np.empty((16, 16), dtype=np.float32)
+        self._check(self.lib.darcy_nn_mfma_check(self.ptr, _ptr(a, C.c_uint16), _ptr(b, C.c_uint16),
+                                                 _ptr(c, C.c_float)))
+        return c
+
+
+def net_param_count() -> int:
+    return int(_load().darcy_net_param_count())
+
+
+class Net:
+    """A darcy_net: the warm-start network on one context."""
+
+    def __init__(self, ctx: Context, n: int, max_batch: int, train: bool = False):
+        self.ctx, self.lib = ctx, ctx.lib
+        self.n, self.max_batch, self.train = n, max_batch, train
+        self.count = int(self.lib.darcy_net_param_count())
+        self.ptr = C.c_void_p()
+        ctx._check(self.lib.darcy_net_create(ctx.ptr, n, max_batch, int(train), C.byref(self.ptr)))
+
+    def close(self) -> None:
+        if self.ptr:
+            self.lib.darcy_net_destroy(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, params: np.ndarray) -> None:
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        self.ctx._check(self.lib.darcy_net_set_params(self.ptr, _ptr(p, C.c_float), p.size))
+
+    def get_params(self) -> np.ndarray:
+        p = np.empty(self.count, dtype=np.float32)
+        self.ctx._check(self.lib.darcy_net_get_params(self.ptr, _ptr(p, C.c_float), p.size))
+        return p
+
+    def predict_device(self, a_ptrs: Sequence[int], f: Sequence[float], u0_ptrs: Sequence[int]) -> None:
+        count = len(a_ptrs)
+        a = (C.c_void_p * count)(*a_ptrs)
+        u = (C.c_void_p * count)(*u0_ptrs)
+        fs = (C.c_double * count)(*f)
+        self.ctx._check(self.lib.darcy_net_predict(self.ptr, count, a, fs, u))
+
+    def train_step(self, a_ptrs: Sequence[int], u_ptrs: Sequence[int], lr: float, want_loss: bool = True) -> float:
+        count = len(a_ptrs)
+        a = (C.c_void_p * count)(*a_ptrs)
+        u = (C.c_void_p * count)(*u_ptrs)
+        loss = C.c_double(float("nan"))
+        self.ctx._check(self.lib.darcy_net_train_step(self.ptr, count, a, u, lr,
+                                                      C.byref(loss) if want_loss else None))
+        return loss.value
+
+    def gradient(self, a_ptrs: Sequence[int], u_ptrs: Sequence[int]) -> tuple[float, np.ndarray]:
+        count = len(a_ptrs)
+        a = (C.c_void_p * count)(*a_ptrs)
+        u = (C.c_void_p * count)(*u_ptrs)
+        loss = C.c_double()
+        g = np.empty(self.count, dtype=np.float32)
+        self.ctx._check(self.lib.darcy_net_gradient(self.ptr, count, a, u, C.byref(loss), _ptr(g, C.c_float)))
+        return loss.value, g

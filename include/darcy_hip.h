@@ -152,6 +152,70 @@ int darcy_bandwidth_probe(darcy_ctx* ctx, size_t bytes, int32_t repeats, double*
 /* "gfx950, 256 CUs, HIP 7.2" style description of the context's device. */
 int darcy_device_info(const darcy_ctx* ctx, char* buf, int32_t cap);
 
+/* ------------------------------------------------------------------------
+ * Learned warm start.  No counterpart in the reference, whose solve always
+ * starts from u = 0 (solver.hpp:99); its output is the u0 argument of
+ * darcy_solve / darcy_solve_batch, and CG still runs to tol from it.
+ *
+ * The network maps (ln a, the forcing, a position channel) on an n x n grid
+ * (n a multiple of 32) to u0 through a 4-level conv encoder-decoder with
+ * 32-64-128-256 channels, 3x3 kernels, GELU, bf16 weights and activations
+ * and FP32 accumulation:
+ *   enc: conv 8->32 (n), 32->64 /2, 64->128 /2, 128->256 /2 (each + GELU)
+ *   dec: conv 256->128, 128->64, 64->32, each followed by x2 nearest
+ *        upsampling, + the encoder activation of that size, GELU
+ *   head: conv 32->1 in FP32;  u0 = f * pos * y / 64,
+ *        pos = sin(pi (i + 1/2) / n) sin(pi (j + 1/2) / n).
+ * Parameters are one FP32 vector: for each conv in that order W[cout][3][3][cin]
+ * then bias[cout] (the first conv has 8 input channels, 3 of them used).
+ * ------------------------------------------------------------------------ */
+typedef struct darcy_net darcy_net;
+
+/* Length of the parameter vector. */
+int64_t darcy_net_param_count(void);
+
+/* A network for n x n problems, processing at most max_batch at a time;
+ * train != 0 also allocates what training needs.  Parameters start at 0. */
+int darcy_net_create(darcy_ctx* ctx, int64_t n, int64_t max_batch, int32_t train, darcy_net** out);
+void darcy_net_destroy(darcy_net* net);
+
+/* Host parameter vector in / out; set also resets the optimizer state. */
+int darcy_net_set_params(darcy_net* net, const float* params, int64_t count);
+int darcy_net_get_params(darcy_net* net, float* params, int64_t count);
+
+/* u0[i] (device, n*n doubles) from a[i] (device) and f[i] (host), for
+ * `count` problems (any count; processed max_batch at a time). */
+int darcy_net_predict(darcy_net* net, int64_t count, const double* const* a, const double* f, double* const* u0);
+
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) on the mean squared error of
+ * pos * y against u / scale over a batch of device-resident pairs (a, u)
+ * with f = 1.  *loss (may be null) receives the loss before the step. */
+int darcy_net_train_step(darcy_net* net, int64_t batch, const double* const* a, const double* const* u, float lr,
+                         double* loss);
+
+/* Loss and gradient (host vector of darcy_net_param_count floats) of the
+ * same objective, without an update. */
+int darcy_net_gradient(darcy_net* net, int64_t batch, const double* const* a, const double* const* u, double* loss,
+                       float* grad);
+
+/* Single-layer entry points on host arrays (bf16 as uint16 bit patterns,
+ * NHWC activations, W[cout][3][3][cin]), for checking the conv kernels:
+ * stride 1 or 2, cin and cout multiples of 8, h and w multiples of the
+ * stride, b * (h/s) * (w/s) a multiple of 8.
+ * z / act: [b][h/s][w/s][cout] (act may be null). */
+int darcy_nn_conv_forward(darcy_ctx* ctx, int32_t b, int32_t h, int32_t w, int32_t cin, int32_t cout, int32_t stride,
+                          const uint16_t* x, const uint16_t* wgt, const float* bias, uint16_t* z, uint16_t* act);
+/* dx: [b][h][w][cin] from dz: [b][h/s][w/s][cout]. */
+int darcy_nn_conv_backward_data(darcy_ctx* ctx, int32_t b, int32_t h, int32_t w, int32_t cin, int32_t cout,
+                                int32_t stride, const uint16_t* dz, const uint16_t* wgt, uint16_t* dx);
+/* dw: [cout][3][3][cin], db: [cout] (FP32), split over `splits` pixel ranges. */
+int darcy_nn_conv_backward_weight(darcy_ctx* ctx, int32_t b, int32_t h, int32_t w, int32_t cin, int32_t cout,
+                                  int32_t stride, int32_t splits, const uint16_t* x, const uint16_t* dz, float* dw,
+                                  float* db);
+/* C (16 x 16, FP32) = A (16 x 32) B (32 x 16), bf16 row-major, through one
+ * matrix-core instruction: checks the operand layout the kernels assume. */
+int darcy_nn_mfma_check(darcy_ctx* ctx, const uint16_t* a, const uint16_t* b, float* c);
+
 #ifdef __cplusplus
 }
 #endif

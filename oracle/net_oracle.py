@@ -1,0 +1,271 @@
+"""CPU oracle of the learned warm start network, in FP64 numpy.
+
+Test infrastructure: only tests, the smoke check and the benchmark's CPU leg
+use it, as the checker of the GPU network (libdarcy_hip: darcy_net_*,
+darcy_nn_*).  The reference has no learned component, so nothing here
+restates reference code; the network is the one the C header describes:
+
+    enc: conv 8->32 (n), 32->64 /2, 64->128 /2, 128->256 /2, each + GELU
+    dec: conv 256->128, 128->64, 64->32, each + nearest x2 upsampling
+         + the encoder activation of that size, then GELU
+    head: conv 32->1;  u0 = f * pos * y / 64
+
+Two modes share one code path:
+
+* ``emulate=False``: exact FP64 math.  Its gradient is pinned by central
+  finite differences of its own loss; its convolution by a direct
+  seven-loop sum and by the adjoint identity <conv x, dz> = <x, conv^T dz>.
+* ``emulate=True``: rounds to bf16 (round to nearest even) exactly where the
+  GPU stores bf16 -- input features, conv weights, every stored activation
+  and activation gradient -- so the GPU result differs from it only by FP32
+  accumulation order.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+# (cin, cout, stride, input side as a divisor of n), in parameter order.
+SPEC = [(8, 32, 1, 1), (32, 64, 2, 1), (64, 128, 2, 2), (128, 256, 2, 4),
+        (256, 128, 1, 8), (128, 64, 1, 4), (64, 32, 1, 2)]
+HEAD_CIN = 32
+SCALE = 1.0 / 64.0
+LN3, LN12 = np.log(3.0), np.log(12.0)
+
+
+def layout() -> tuple[list[tuple[int, int, tuple[int, ...]]], int]:
+    """[(w_offset, b_offset, w_shape)] for the 7 convs and the head, and the total."""
+    out, o = [], 0
+    for cin, cout, _, _ in SPEC + [(HEAD_CIN, 1, 1, 1)]:
+        shape = (cout, 3, 3, cin)
+        w = o
+        o += int(np.prod(shape))
+        out.append((w, o, shape))
+        o += cout
+    return out, o
+
+
+def unpack(params: np.ndarray) -> list[tuple[np.ndarray, np.ndarray]]:
+    lay, total = layout()
+    assert params.shape == (total,)
+    res = []
+    for w, b, shape in lay:
+        res.append((params[w:b].reshape(shape).astype(np.float64), params[b:b + shape[0]].astype(np.float64)))
+    return res
+
+
+def pack(pairs: list[tuple[np.ndarray, np.ndarray]]) -> np.ndarray:
+    return np.concatenate([np.concatenate([w.ravel(), b.ravel()]) for w, b in pairs])
+
+
+def bf16(x: np.ndarray) -> np.ndarray:
+    """Round to bfloat16 (nearest even) through float32; returns float64."""
+    u = np.asarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000).astype(np.uint32)
+    return r.view(np.float32).astype(np.float64)
+
+
+def bf16_bits(x: np.ndarray) -> np.ndarray:
+    """bf16 bit patterns (uint16) of x, rounded to nearest even."""
+    return (bf16(x).astype(np.float32).view(np.uint32) >> 16).astype(np.uint16)
+
+
+def from_bits(b: np.ndarray) -> np.ndarray:
+    return (b.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+def _gelu_s(z):
+    """(1 + tanh u) / 2 = 1 / (1 + exp(-2u)), u = sqrt(2/pi) (z + 0.044715 z^3)."""
+    c = np.sqrt(2.0 / np.pi)
+    with np.errstate(over="ignore"):
+        return 1.0 / (1.0 + np.exp(-2.0 * c * (z + 0.044715 * z ** 3)))
+
+
+def gelu(z):
+    """GELU, tanh form 0.5 z (1 + tanh u), written as z s to avoid cancellation."""
+    return z * _gelu_s(z)
+
+
+def gelu_grad(z):
+    c = np.sqrt(2.0 / np.pi)
+    s = _gelu_s(z)
+    return s + 2.0 * z * s * (1.0 - s) * c * (1.0 + 3.0 * 0.044715 * z * z)
+
+
+def position(n: int) -> np.ndarray:
+    """sin(pi (i + 1/2) / n) sin(pi (j + 1/2) / n), rounded to float32 as the GPU does."""
+    s = np.sin(np.pi * (np.arange(n) + 0.5) / n)
+    return np.outer(s, s).astype(np.float32).astype(np.float64)
+
+
+# ------------------------------------------------------------ convolutions
+
+def _patches(x: np.ndarray, stride: int) -> np.ndarray:
+    b, h, w, c = x.shape
+    oh, ow = h // stride, w // stride
+    xp = np.pad(x, ((0, 0), (1, 1), (1, 1), (0, 0)))
+    p = np.empty((b, oh, ow, 3, 3, c))
+    for ky in range(3):
+        for kx in range(3):
+            p[:, :, :, ky, kx, :] = xp[:, ky:ky + stride * oh:stride, kx:kx + stride * ow:stride, :]
+    return p
+
+
+def conv_forward(x: np.ndarray, w: np.ndarray, bias: np.ndarray, stride: int) -> np.ndarray:
+    """z[b, oy, ox, co] = bias[co] + sum w[co, ky, kx, ci] x[b, s oy + ky - 1, s ox + kx - 1, ci] (zero outside)."""
+    cout = w.shape[0]
+    p = _patches(x, stride)
+    z = p.reshape(-1, p.shape[3] * p.shape[4] * p.shape[5]) @ w.reshape(cout, -1).T + bias
+    return z.reshape(p.shape[0], p.shape[1], p.shape[2], cout)
+
+
+def conv_backward_data(dz: np.ndarray, w: np.ndarray, in_shape: tuple[int, ...], stride: int) -> np.ndarray:
+    b, h, wd, c = in_shape
+    oh, ow = dz.shape[1], dz.shape[2]
+    dp = (dz.reshape(-1, dz.shape[3]) @ w.reshape(w.shape[0], -1)).reshape(b, oh, ow, 3, 3, c)
+    dxp = np.zeros((b, h + 2, wd + 2, c))
+    for ky in range(3):
+        for kx in range(3):
+            dxp[:, ky:ky + stride * oh:stride, kx:kx + stride * ow:stride, :] += dp[:, :, :, ky, kx, :]
+    return dxp[:, 1:-1, 1:-1, :]
+
+
+def conv_backward_weight(x: np.ndarray, dz: np.ndarray, stride: int) -> tuple[np.ndarray, np.ndarray]:
+    p = _patches(x, stride)
+    cout = dz.shape[3]
+    dw = dz.reshape(-1, cout).T @ p.reshape(-1, p.shape[3] * p.shape[4] * p.shape[5])
+    return dw.reshape(cout, 3, 3, x.shape[3]), dz.reshape(-1, cout).sum(axis=0)
+
+
+def conv_direct(x: np.ndarray, w: np.ndarray, bias: np.ndarray, stride: int) -> np.ndarray:
+    """The definition, one output at a time (tiny shapes only)."""
+    b, h, wd, cin = x.shape
+    cout = w.shape[0]
+    oh, ow = h // stride, wd // stride
+    z = np.zeros((b, oh, ow, cout))
+    for bb in range(b):
+        for oy in range(oh):
+            for ox in range(ow):
+                for co in range(cout):
+                    s = bias[co]
+                    for ky in range(3):
+                        for kx in range(3):
+                            iy, ix = stride * oy + ky - 1, stride * ox + kx - 1
+                            if 0 <= iy < h and 0 <= ix < wd:
+                                for ci in range(cin):
+                                    s += w[co, ky, kx, ci] * x[bb, iy, ix, ci]
+                    z[bb, oy, ox, co] = s
+    return z
+
+
+def upsample2(c: np.ndarray) -> np.ndarray:
+    return np.repeat(np.repeat(c, 2, axis=1), 2, axis=2)
+
+
+def down_sum(d: np.ndarray) -> np.ndarray:
+    b, h, w, c = d.shape
+    return d.reshape(b, h // 2, 2, w // 2, 2, c).sum(axis=(2, 4))
+
+
+# ------------------------------------------------------------ the network
+
+def features(a: np.ndarray) -> np.ndarray:
+    """(B, n, n) permeability -> (B, n, n, 8) features before rounding."""
+    bsz, n, _ = a.shape
+    x = np.zeros((bsz, n, n, 8))
+    mu, sd = 0.5 * (LN3 + LN12), 0.5 * (LN12 - LN3)
+    x[..., 0] = ((np.log(a) - mu) / sd).astype(np.float32)
+    x[..., 1] = 1.0
+    x[..., 2] = position(n)
+    return x
+
+
+class Net:
+    """Forward, loss and backward of the warm-start network on a batch."""
+
+    def __init__(self, params: np.ndarray, emulate: bool = False):
+        self.emulate = emulate
+        self.layers = unpack(np.asarray(params, dtype=np.float64))
+        self.R = bf16 if emulate else (lambda v: v)
+        # Conv weights as the GPU uses them (bf16 copies); the head stays FP32.
+        self.w = [(self.R(w), b) for w, b in self.layers[:7]] + [self.layers[7]]
+
+    def forward(self, a: np.ndarray) -> dict:
+        R, s = self.R, {}
+        x = R(features(a))
+        s["x"] = x
+        for l in range(4):
+            w, b = self.w[l]
+            v = conv_forward(x, w, b, SPEC[l][2])
+            s[f"z{l}"], s[f"a{l}"] = R(v), R(gelu(v))
+            x = s[f"a{l}"]
+        for k in range(3):
+            l = 4 + k
+            w, b = self.w[l]
+            c = R(conv_forward(x, w, b, 1))
+            v = upsample2(c) + s[f"a{2 - k}"]
+            s[f"zm{k}"], s[f"am{k}"] = R(v), R(gelu(v))
+            x = s[f"am{k}"]
+        w, b = self.w[7]
+        s["y"] = conv_forward(x, w, b, 1)[..., 0]
+        return s
+
+    def loss(self, a: np.ndarray, u: np.ndarray) -> tuple[float, dict]:
+        s = self.forward(a)
+        pos = position(a.shape[1])
+        t = u / SCALE
+        if self.emulate:
+            t = t.astype(np.float32).astype(np.float64)
+        d = pos * s["y"] - t
+        s["dy"] = 2.0 * d * pos / d.size
+        return float(np.mean(d * d)), s
+
+    def gradient(self, a: np.ndarray, u: np.ndarray) -> tuple[float, np.ndarray]:
+        R = self.R
+        loss, s = self.loss(a, u)
+        grads: list = [None] * 8
+        dy = s["dy"][..., None]
+        w, _ = self.w[7]
+        grads[7] = conv_backward_weight(s["am2"], dy, 1)
+        dm = R(conv_backward_data(dy, w, s["am2"].shape, 1))
+        # Decoder, last merge first.
+        dmerge = [None, None, None]
+        conv_in = [s["a3"], s["am0"], s["am1"]]
+        for k in (2, 1, 0):
+            l = 4 + k
+            dz = R(dm * gelu_grad(s[f"zm{k}"]))
+            dmerge[k] = dz
+            dc = R(down_sum(dz))
+            grads[l] = conv_backward_weight(conv_in[k], dc, 1)
+            dm = R(conv_backward_data(dc, self.w[l][0], conv_in[k].shape, 1))
+        # Encoder: dm is now the gradient of a3.
+        d = R(dm * gelu_grad(s["z3"]))
+        for l in (3, 2, 1, 0):
+            x = s[f"a{l - 1}"] if l > 0 else s["x"]
+            grads[l] = conv_backward_weight(x, d, SPEC[l][2])
+            if l == 0:
+                break
+            dx = R(conv_backward_data(d, self.w[l][0], x.shape, SPEC[l][2]))
+            dx = R(dx + dmerge[3 - l])
+            d = R(dx * gelu_grad(s[f"z{l - 1}"]))
+        return loss, pack(grads)
+
+    def predict(self, a: np.ndarray, f: np.ndarray | float = 1.0) -> np.ndarray:
+        s = self.forward(a)
+        f = np.broadcast_to(np.asarray(f, dtype=np.float64), (a.shape[0],))
+        return f[:, None, None] * SCALE * (position(a.shape[1]) * s["y"])
+
+
+def init_params(seed: int) -> np.ndarray:
+    """He-normal conv weights for GELU, zero biases; unused input channels 3..7 zero."""
+    rng = np.random.default_rng(seed)
+    lay, total = layout()
+    p = np.zeros(total, dtype=np.float32)
+    for i, (w, b, shape) in enumerate(lay):
+        fan_in = 9 * (3 if i == 0 else shape[3])
+        std = np.sqrt(2.0 / fan_in) if i < 7 else 0.1 * np.sqrt(1.0 / fan_in)
+        wt = rng.normal(0.0, std, size=shape).astype(np.float32)
+        if i == 0:
+            wt[..., 3:] = 0.0
+        p[w:b] = wt.ravel()
+    return p

@@ -1,0 +1,181 @@
+"""Pins the CPU oracle of the warm-start network (oracle/net_oracle.py).
+
+The oracle is what the GPU network is checked against, so it is pinned here
+by things that do not restate it: torch's bf16 conversion, a direct
+seven-loop convolution, the adjoint identities of the convolution, central
+finite differences of its own FP64 loss, and torch autograd in FP64 on the
+same network.  A planted mistake in the oracle's backward pass must fail the
+finite-difference check.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import net_oracle as no
+from conftest import rel_l2, two_phase
+
+
+def _batch(n: int, seeds: list[int]) -> np.ndarray:
+    return np.stack([two_phase(n, seed=s) for s in seeds])
+
+
+def _target(a: np.ndarray) -> np.ndarray:
+    # A smooth positive field of the solution's size (the loss only needs a target).
+    n = a.shape[1]
+    pos = no.position(n)
+    return 0.012 * pos[None] * (6.0 / a) ** 0.3
+
+
+def test_layout_counts():
+    lay, total = no.layout()
+    assert len(lay) == 8
+    assert total == sum(int(np.prod(s)) + s[0] for _, _, s in lay)
+    assert total == 777441
+    assert lay[0][2] == (32, 3, 3, 8) and lay[7][2] == (1, 3, 3, 32)
+
+
+def test_bf16_rounding_matches_torch():
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(1)
+    x = np.concatenate([
+        rng.normal(size=20000) * 10.0 ** rng.integers(-8, 8, size=20000),
+        # exact ties between two bf16 values, both parities
+        (np.arange(1, 2000, dtype=np.uint32) << 16 | 0x8000).view(np.float32).astype(np.float64),
+        np.array([0.0, -0.0, 1.0, -1.0, 3.0e38, 1e-40]),
+    ]).astype(np.float32)
+    ours = no.bf16(x)
+    ref = torch.from_numpy(x).to(torch.bfloat16).to(torch.float64).numpy()
+    np.testing.assert_array_equal(ours, ref)
+    np.testing.assert_array_equal(no.from_bits(no.bf16_bits(x)), ref)
+
+
+@pytest.mark.parametrize("stride", [1, 2])
+def test_conv_matches_direct_sum(stride):
+    rng = np.random.default_rng(2 + stride)
+    x = rng.normal(size=(2, 6, 4, 3))
+    w = rng.normal(size=(5, 3, 3, 3))
+    b = rng.normal(size=5)
+    np.testing.assert_allclose(no.conv_forward(x, w, b, stride), no.conv_direct(x, w, b, stride), rtol=1e-12,
+                               atol=1e-12)
+
+
+@pytest.mark.parametrize("stride", [1, 2])
+def test_conv_adjoint_identities(stride):
+    rng = np.random.default_rng(7 + stride)
+    x = rng.normal(size=(2, 8, 8, 4))
+    w = rng.normal(size=(6, 3, 3, 4))
+    z = no.conv_forward(x, w, np.zeros(6), stride)
+    dz = rng.normal(size=z.shape)
+    lhs = float(np.sum(z * dz))
+    dx = no.conv_backward_data(dz, w, x.shape, stride)
+    dw, db = no.conv_backward_weight(x, dz, stride)
+    assert abs(lhs - float(np.sum(x * dx))) <= 1e-10 * abs(lhs) + 1e-10
+    assert abs(lhs - float(np.sum(w * dw))) <= 1e-10 * abs(lhs) + 1e-10
+    np.testing.assert_allclose(db, dz.sum(axis=(0, 1, 2)), rtol=1e-12)
+
+
+def _fd_errors(net_cls, params, a, u, picks):
+    net = net_cls(params)
+    _, g = net.gradient(a, u)
+    errs = []
+    for i in picks:
+        h = 1e-5 * max(1.0, abs(float(params[i])))
+        pp, pm = params.astype(np.float64).copy(), params.astype(np.float64).copy()
+        pp[i] += h
+        pm[i] -= h
+        fd = (net_cls(pp).loss(a, u)[0] - net_cls(pm).loss(a, u)[0]) / (2 * h)
+        errs.append(abs(fd - g[i]) / max(abs(fd), abs(g[i]), 1e-12))
+    return np.array(errs), g
+
+
+def _picks(params, rng, per_layer=3):
+    lay, _ = no.layout()
+    out = []
+    for w, b, shape in lay:
+        out.extend(rng.integers(w, b, size=per_layer).tolist())  # weights
+        out.append(b + int(rng.integers(0, shape[0])))  # a bias
+    return out
+
+
+def _fd_problem():
+    rng = np.random.default_rng(11)
+    params = no.init_params(3)
+    params[no.layout()[0][0][1]:no.layout()[0][0][1] + 32] = rng.normal(size=32) * 0.1  # nonzero biases
+    a = _batch(32, [5])
+    return rng, params, a, _target(a)
+
+
+def test_gradient_matches_finite_differences():
+    rng, params, a, u = _fd_problem()
+    # Only channels 0..2 of the first conv see nonzero input; pick weights there.
+    picks = [p for p in _picks(params, rng) if not (p < no.layout()[0][0][1] and p % 8 >= 3)]
+    errs, g = _fd_errors(no.Net, params, a, u, picks)
+    assert np.max(errs) < 1e-5, errs
+    assert np.count_nonzero(g) > 0.5 * g.size
+
+
+def test_planted_mistake_fails_finite_differences():
+    """A wrong adjoint of the upsampling (mean instead of sum) in the oracle must be caught."""
+
+    class Broken(no.Net):
+        def gradient(self, a, u):
+            saved = no.down_sum
+            try:
+                # Wrong adjoint of the nearest upsampling: mean instead of sum.
+                no.down_sum = lambda d: saved(d) / 4.0
+                return super().gradient(a, u)
+            finally:
+                no.down_sum = saved
+
+    rng, params, a, u = _fd_problem()
+    lay, _ = no.layout()
+    picks = list(range(lay[4][0], lay[4][0] + 5))  # decoder weights sit behind the upsampling
+    errs, _ = _fd_errors(Broken, params, a, u, picks)
+    assert np.max(errs) > 0.1
+
+
+def test_gradient_matches_torch_autograd():
+    torch = pytest.importorskip("torch")
+    F = torch.nn.functional
+    rng, params, a, u = _fd_problem()
+    a = _batch(32, [5, 6])
+    u = _target(a)
+    loss_np, g_np = no.Net(params).gradient(a, u)
+
+    lay, _ = no.layout()
+    ts = []
+    for w, b, shape in lay:
+        ts.append(torch.tensor(params[w:b].reshape(shape), dtype=torch.float64, requires_grad=True))
+        ts.append(torch.tensor(params[b:b + shape[0]], dtype=torch.float64, requires_grad=True))
+
+    def conv(x, i, stride):  # x: NCHW; weight [co][ky][kx][ci] -> [co][ci][ky][kx]
+        return F.conv2d(x, ts[2 * i].permute(0, 3, 1, 2), ts[2 * i + 1], stride=stride, padding=1)
+
+    x = torch.tensor(no.features(a), dtype=torch.float64).permute(0, 3, 1, 2)
+    acts = []
+    for l in range(4):
+        x = F.gelu(conv(x, l, no.SPEC[l][2]), approximate="tanh")
+        acts.append(x)
+    for k in range(3):
+        c = conv(x, 4 + k, 1)
+        x = F.gelu(F.interpolate(c, scale_factor=2, mode="nearest") + acts[2 - k], approximate="tanh")
+    y = conv(x, 7, 1)[:, 0]
+    pos = torch.tensor(no.position(32), dtype=torch.float64)
+    t = torch.tensor(u / no.SCALE, dtype=torch.float64)
+    loss = torch.mean((pos * y - t) ** 2)
+    loss.backward()
+    g_t = torch.cat([p.grad.reshape(-1) for p in ts]).numpy()
+    assert abs(loss.item() - loss_np) <= 1e-12 * abs(loss_np)
+    assert rel_l2(g_np, g_t) < 1e-10
+
+
+def test_emulated_rounding_stays_close_to_exact():
+    rng, params, a, u = _fd_problem()
+    l0, g0 = no.Net(params).gradient(a, u)
+    l1, g1 = no.Net(params, emulate=True).gradient(a, u)
+    assert abs(l1 - l0) < 2e-2 * l0
+    lay, _ = no.layout()
+    for w, b, shape in lay:
+        assert rel_l2(g1[w:b + shape[0]], g0[w:b + shape[0]]) < 5e-2
