@@ -1,16 +1,20 @@
 // Convolution kernels of the learned warm start: a BF16 encoder-decoder
 // with FP32 accumulation, on the matrix cores of gfx950.
 //
-// Every 3x3 convolution with a channel count that is a multiple of 8 runs as
-// an implicit GEMM on v_mfma_f32_16x16x32_bf16: a workgroup (4 waves) owns a
-// 64 x 64 tile of C, each wave a 32 x 32 sub-tile of 2 x 2 MFMA tiles, and the
-// K loop steps by 32 with the next step's operands loaded while the current
-// step's MFMAs run.  The operands are never materialised: `gather` functors
-// read the 8 consecutive K values a lane needs straight from the NHWC
-// activations (one 16-byte load when the 8 values are contiguous).
+// Forward convolutions with a power-of-two input channel count (and the
+// stride-1 backward data, which is the forward conv of dZ with the flipped,
+// transposed weights) run in k_conv, a direct implicit GEMM on
+// v_mfma_f32_16x16x32_bf16: a wave owns a 64-pixel x 64-channel tile, the K
+// loop walks (tap, ci) in steps of 32 with the next step's operands loaded
+// while the current step's MFMAs run, and the pixel coordinates are decoded
+// once per lane.  Every other shape (strided backward data, the weight
+// gradient, channel counts that are not powers of two) runs in k_gemm, a
+// generic 64 x 64 tile whose `gather` functors read the 8 consecutive K
+// values a lane needs straight from the NHWC tensors (one 16-byte load when
+// they are contiguous).  No operand is ever materialised.
 //
 //   forward        Z[pixel][co]      = sum_(tap,ci) X[pixel@tap][ci] * W[co][tap][ci] + bias[co]
-//   backward data  dX[pixel][ci]     = sum_(tap,co) dZ[pixel@-tap][co] * WT[ci][tap][co]
+//   backward data  dX[pixel][ci]     = sum_(tap,co) dZ[pixel@-tap][co] * W[co][tap][ci]
 //   backward wt.   dW[co][(tap,ci)]  = sum_pixel    dZT[co][pixel] * X[pixel@tap][ci]
 //                  db[co]            = sum_pixel    dZT[co][pixel]            (a column of ones)
 //
@@ -91,9 +95,11 @@ struct RowsB {
     }
 };
 
-// Backward data: row = input pixel (y, x), k = tap * Cout + co.  The output
-// pixel that reads (y, x) through tap (ky, kx) is ((y+1-ky)/s, (x+1-kx)/s)
-// when the division is exact and the pixel lies on the output grid.
+// Backward data against the flipped, transposed weights
+// WF[ci][t][co] = W[co][8 - t][ci]: row = input pixel (y, x), k = t * Cout + co.
+// The output pixel that reads (y, x) through the original tap 8 - t is
+// ((y + t/3 - 1) / s, (x + t%3 - 1) / s) when the division is exact and the
+// pixel lies on the output grid.
 struct BwdDataA {
     const bf16* dz;
     Geo out;  // geometry of the conv output (and of dz)
@@ -104,7 +110,7 @@ struct BwdDataA {
         const int x = m % iw, t = m / iw, y = t % ih, b = t / ih;
         const int tap = k / out.c, co = k - tap * out.c;
         if (tap >= 9) return r;
-        int oy = y + 1 - tap / 3, ox = x + 1 - tap % 3;
+        int oy = y + tap / 3 - 1, ox = x + tap % 3 - 1;
         if (stride == 2) {
             if ((oy & 1) || (ox & 1)) return r;
             oy >>= 1;
@@ -115,19 +121,23 @@ struct BwdDataA {
     }
 };
 
-// Row-major [M][K] matrix used as A: dZ^T in the weight gradient.
+// Row-major [M][K] matrix used as A: dZ^T in the weight gradient (K, the
+// pixel count, need not be a multiple of 8 on the coarsest levels).
 struct RowsA {
     const bf16* a;
     int K, M;
     __device__ v8bf load8(int m, int k) const {
         v8bf r = {};
         if (m >= M || k >= K) return r;
-        return *reinterpret_cast<const v8bf*>(a + static_cast<size_t>(m) * K + k);
+        const bf16* row = a + static_cast<size_t>(m) * K;
+        if ((K & 7) == 0) return *reinterpret_cast<const v8bf*>(row + k);
+        for (int t = 0; t < 8 && k + t < K; ++t) r[t] = row[k + t];
+        return r;
     }
 };
 
 // Weight gradient B: B[pixel][n] = X at the input pixel under tap n / Cin for
-// n < 9 Cin, and 1 in column 9 Cin (the bias).  P is a multiple of 8.
+// n < 9 Cin, and 1 in column 9 Cin (the bias).
 struct BwdWtB {
     const bf16* x;
     Geo in;
@@ -137,7 +147,7 @@ struct BwdWtB {
         if (n > NW || p0 >= P) return r;
         if (n == NW) {
 #pragma unroll
-            for (int t = 0; t < 8; ++t) r[t] = f2bf(1.0f);
+            for (int t = 0; t < 8; ++t) r[t] = f2bf(p0 + t < P ? 1.0f : 0.0f);
             return r;
         }
         const int tap = n / in.c, ci = n - tap * in.c, ky = tap / 3 - 1, kx = tap % 3 - 1;
@@ -155,6 +165,7 @@ struct BwdWtB {
         }
         for (int t = 0; t < 8; ++t) {
             const int p = p0 + t, ox = p % ow, q = p / ow, oy = q % oh, b = q / oh;
+            if (p >= P) break;
             const int iy = oy * stride + ky, ix = ox * stride + kx;
             if (iy >= 0 && iy < in.h && ix >= 0 && ix < in.w) r[t] = x[at(in, b, iy, ix) + ci];
         }
@@ -164,7 +175,7 @@ struct BwdWtB {
 
 // ------------------------------------------------------------ epilogues
 
-// Forward: z = acc + bias; stores z (if wanted) and gelu(z) (if wanted).
+// Forward: z = acc + bias (bias may be null); stores z and gelu(z) where wanted.
 struct EpiFwd {
     bf16* z;
     bf16* act;
@@ -172,7 +183,7 @@ struct EpiFwd {
     int M, N;
     __device__ void operator()(int m, int n, float acc) const {
         if (m >= M || n >= N) return;
-        const float v = acc + bias[n];
+        const float v = bias ? acc + bias[n] : acc;
         const size_t i = static_cast<size_t>(m) * N + n;
         if (z) z[i] = f2bf(v);
         if (act) act[i] = f2bf(gelu(v));
@@ -236,6 +247,117 @@ __global__ __launch_bounds__(kT) void k_gemm(int kchunk, int K, GA ga, GB gb, Ep
 #pragma unroll
             for (int t = 0; t < 4; ++t) epi(m0 + 16 * i + rq + t, n0 + 16 * j + r, acc[i][j][t]);
 }
+
+// ------------------------------------------------- the direct conv kernel
+// Forward 3x3 conv (and stride-1 backward data, which is a forward conv of dZ
+// with the flipped, transposed weights) for a power-of-two input channel
+// count >= 8.  Rows are output pixels, columns output channels, and
+// K = (tap, ci) in steps of 32.  Each lane decodes its 4 pixel rows once;
+// per K step it only adds the tap offset (tap / 3 as (tap * 11) >> 5).  A
+// wave owns a 64 x WN tile (4 x WN/16 MFMA tiles), a workgroup WAVES_N waves
+// across the columns and 4 / WAVES_N down the rows.  Workgroups are dealt to
+// the 8 XCDs round-robin; the tile order gives each XCD a contiguous band of
+// pixel rows so the taps' overlapping input rows stay in one L2.
+struct ConvArgs {
+    const bf16* x;  // NHWC input, in.c = 1 << lg_cin channels
+    const bf16* w;  // [cout][9][cin]
+    Geo in;
+    int oh, ow, stride, lg_cin, cout, M, tiles_m, tiles_n;
+};
+
+template <int WN, int WAVES_N, class Epi>
+__global__ __launch_bounds__(kT) void k_conv(ConvArgs c, Epi epi) {
+    constexpr int WAVES_M = 4 / WAVES_N, TN = WN / 16;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nb = gridDim.x;
+    int t = blockIdx.x;
+    if ((nb & 7) == 0) t = (t & 7) * (nb >> 3) + (t >> 3);
+    const int tm = t / c.tiles_n, tn = t - tm * c.tiles_n;
+    const int m_base = tm * (64 * WAVES_M) + (wave / WAVES_N) * 64;
+    const int n_base = tn * (WN * WAVES_N) + (wave % WAVES_N) * WN;
+    const int r = lane & 15, kq = (lane >> 4) * 8;
+    const int cin = 1 << c.lg_cin, K = 9 << c.lg_cin;
+
+    int iy0[4], ix0[4];
+    long long base[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int m = m_base + 16 * i + r;
+        const int ox = m % c.ow, q = m / c.ow, oy = q % c.oh, b = q / c.oh;
+        iy0[i] = m < c.M ? oy * c.stride - 1 : -1000000;
+        ix0[i] = ox * c.stride - 1;
+        base[i] = ((static_cast<long long>(b) * c.in.h + iy0[i]) * c.in.w + ix0[i]) * cin;
+    }
+    auto load_a = [&](int k, v8bf* a) {
+        const int tap = k >> c.lg_cin, ci = k & (cin - 1);
+        const int ky = (tap * 11) >> 5, kx = tap - 3 * ky;
+        const long long off = (static_cast<long long>(ky) * c.in.w + kx) * cin + ci;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int iy = iy0[i] + ky, ix = ix0[i] + kx;
+            v8bf v = {};
+            if (tap < 9 && static_cast<unsigned>(iy) < static_cast<unsigned>(c.in.h) &&
+                static_cast<unsigned>(ix) < static_cast<unsigned>(c.in.w))
+                v = *reinterpret_cast<const v8bf*>(c.x + base[i] + off);
+            a[i] = v;
+        }
+    };
+    auto load_b = [&](int k, v8bf* bv) {
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+            const int n = n_base + 16 * j + r;
+            v8bf v = {};
+            if (n < c.cout && k < K) v = *reinterpret_cast<const v8bf*>(c.w + static_cast<size_t>(n) * K + k);
+            bv[j] = v;
+        }
+    };
+
+    v4f acc[4][TN] = {};
+    v8bf a[4], bw[TN];
+    load_a(kq, a);
+    load_b(kq, bw);
+    for (int k0 = 0; k0 < K; k0 += 32) {
+        v8bf na[4], nbw[TN];
+        const int kn = k0 + 32 < K ? k0 + 32 + kq : K;  // past the end: loads zero, unused
+        load_a(kn, na);
+        load_b(kn, nbw);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], bw[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a[i] = na[i];
+#pragma unroll
+        for (int j = 0; j < TN; ++j) bw[j] = nbw[j];
+    }
+    const int rq = (lane >> 4) * 4;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) epi(m_base + 16 * i + rq + q, n_base + 16 * j + r, acc[i][j][q]);
+}
+
+template <class Epi>
+hipError_t launch_conv(const bf16* x, Geo in, int stride, const bf16* w, int cout, Epi epi, hipStream_t s) {
+    int lg = 0;
+    while ((1 << lg) < in.c) ++lg;
+    const int oh = in.h / stride, ow = in.w / stride, M = in.b * oh * ow;
+    ConvArgs c{x, w, in, oh, ow, stride, lg, cout, M, 0, 0};
+    auto go = [&](auto kernel, int wn, int waves_n) {
+        c.tiles_m = (M + 64 * (4 / waves_n) - 1) / (64 * (4 / waves_n));
+        c.tiles_n = (cout + wn * waves_n - 1) / (wn * waves_n);
+        kernel<<<c.tiles_m * c.tiles_n, kT, 0, s>>>(c, epi);
+        return hipGetLastError();
+    };
+    if (cout >= 128) return go(k_conv<64, 2, Epi>, 64, 2);
+    if (cout >= 64) return go(k_conv<64, 1, Epi>, 64, 1);
+    return go(k_conv<32, 1, Epi>, 32, 1);
+}
+
+inline bool pow2_at_least_8(int c) { return c >= 8 && (c & (c - 1)) == 0; }
 
 // Sum of the split-K partials in split order: column NW of each row is the
 // bias gradient, the others the weight gradient [M][NW].
@@ -318,36 +440,52 @@ __global__ __launch_bounds__(kT) void k_head_dw_part(const bf16* x, Geo in, cons
 
 // ------------------------------------------------------------ elementwise
 
-// Input features, NHWC bf16 with 8 channels: (ln a - mu) / sd, 1, and the
-// position sin(pi (i + 1/2) / n) sin(pi (j + 1/2) / n); channels 3..7 are 0.
-// mu, sd map the two phases ln 3 and ln 12 to -1 and +1.
-__global__ __launch_bounds__(kT) void k_features(const double* const* a, int n, int batch, bf16* x) {
+// pos[i n + j] = sin(pi (i + 1/2) / n) sin(pi (j + 1/2) / n), rounded to float:
+// the position channel of the input and the factor of the output.
+__global__ __launch_bounds__(kT) void k_position(int n, float* pos) {
+    const double pi = 3.14159265358979323846;
+    for (int q = blockIdx.x * kT + threadIdx.x; q < n * n; q += gridDim.x * kT) {
+        const int i = q / n, j = q % n;
+        pos[q] = static_cast<float>(sin(pi * (i + 0.5) / n) * sin(pi * (j + 0.5) / n));
+    }
+}
+
+// Input features, NHWC bf16 with 8 channels: (ln a - mu) / sd, 1 (the
+// forcing, which enters the output linearly) and the position; channels
+// 3..7 are 0.  mu, sd map the two phases ln 3 and ln 12 to -1 and +1.
+__global__ __launch_bounds__(kT) void k_features(const double* const* a, const float* pos, int n, int batch, bf16* x) {
     const size_t per = static_cast<size_t>(n) * n;
     const double mu = 0.5 * (log(3.0) + log(12.0)), sd = 0.5 * (log(12.0) - log(3.0));
-    const double pi = 3.14159265358979323846;
     for (size_t q = blockIdx.x * static_cast<size_t>(kT) + threadIdx.x; q < per * batch; q += static_cast<size_t>(gridDim.x) * kT) {
         const size_t b = q / per, k = q % per;
-        const int i = static_cast<int>(k / n), j = static_cast<int>(k % n);
-        const double pos = sin(pi * (i + 0.5) / n) * sin(pi * (j + 0.5) / n);
         v8bf v = {};
         v[0] = f2bf(static_cast<float>((log(a[b][k]) - mu) / sd));
         v[1] = f2bf(1.0f);
-        v[2] = f2bf(static_cast<float>(pos));
+        v[2] = f2bf(pos[k]);
         *reinterpret_cast<v8bf*>(x + 8 * q) = v;
     }
 }
 
-// Decoder merge: z = up2(c) + skip, act = gelu(z) (nearest upsampling).
+// Decoder merge: z = up2(c) + skip, act = gelu(z) (nearest upsampling), 8
+// channels per thread.
 __global__ __launch_bounds__(kT) void k_up_add_gelu(const bf16* c, Geo cg, const bf16* skip, bf16* z, bf16* act) {
-    const int H = cg.h * 2, W = cg.w * 2;
-    const size_t count = static_cast<size_t>(cg.b) * H * W * cg.c;
+    const int H = cg.h * 2, W = cg.w * 2, c8 = cg.c / 8;
+    const size_t count = static_cast<size_t>(cg.b) * H * W * c8;
     for (size_t q = blockIdx.x * static_cast<size_t>(kT) + threadIdx.x; q < count; q += static_cast<size_t>(gridDim.x) * kT) {
-        const int ch = static_cast<int>(q % cg.c);
-        const size_t p = q / cg.c;
+        const int ch = static_cast<int>(q % c8) * 8;
+        const size_t p = q / c8;
         const int x = static_cast<int>(p % W), t = static_cast<int>(p / W), y = t % H, b = t / H;
-        const float v = bf2f(c[at(cg, b, y / 2, x / 2) + ch]) + bf2f(skip[q]);
-        if (z) z[q] = f2bf(v);
-        act[q] = f2bf(gelu(v));
+        const v8bf cv = *reinterpret_cast<const v8bf*>(c + at(cg, b, y / 2, x / 2) + ch);
+        const v8bf sv = *reinterpret_cast<const v8bf*>(skip + 8 * q);
+        v8bf zv, av;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const float v = bf2f(cv[e]) + bf2f(sv[e]);
+            zv[e] = f2bf(v);
+            av[e] = f2bf(gelu(v));
+        }
+        if (z) *reinterpret_cast<v8bf*>(z + 8 * q) = zv;
+        *reinterpret_cast<v8bf*>(act + 8 * q) = av;
     }
 }
 
@@ -390,33 +528,66 @@ __global__ __launch_bounds__(kT) void k_transpose(const bf16* in, int P, int C, 
 }
 
 // u0 = f * scale * pos * y, FP64 per problem.
-__global__ __launch_bounds__(kT) void k_output(const float* y, int n, int batch, const double* f, double scale,
-                                               double* const* u0) {
+__global__ __launch_bounds__(kT) void k_output(const float* y, const float* pos, int n, int batch, const double* f,
+                                               double scale, double* const* u0) {
     const size_t per = static_cast<size_t>(n) * n;
-    const double pi = 3.14159265358979323846;
     for (size_t q = blockIdx.x * static_cast<size_t>(kT) + threadIdx.x; q < per * batch; q += static_cast<size_t>(gridDim.x) * kT) {
         const size_t b = q / per, k = q % per;
-        const int i = static_cast<int>(k / n), j = static_cast<int>(k % n);
-        const float pos = static_cast<float>(sin(pi * (i + 0.5) / n) * sin(pi * (j + 0.5) / n));
-        u0[b][k] = f[b] * scale * static_cast<double>(pos * y[q]);
+        u0[b][k] = f[b] * scale * static_cast<double>(pos[k] * y[q]);
     }
 }
 
-// MSE against t = u / scale:  L = mean((pos y - t)^2),  dy = 2 (pos y - t) pos / count.
-// Each workgroup writes its partial sum of squares (divided by count).
-__global__ __launch_bounds__(kT) void k_mse(const float* y, const double* const* u, int n, int batch, double scale,
-                                            float* dy, float* partial) {
+// Training loss against t = u / scale with e = pos y - t:
+//   L = (sum_cells e^2 + lam e^T A e) / cells,
+// A the finite-volume operator of the problem's permeability (harmonic
+// interior faces, boundary faces carrying the cell's own a), so the second
+// term is the energy norm in which CG converges; dy = pos dL/de with
+// dL/de = 2 (e + lam A e) / cells.  Each workgroup writes its partial loss.
+__global__ __launch_bounds__(kT) void k_loss(const float* y, const double* const* u, const double* const* a,
+                                             const float* pos, int n, int batch, double scale, float lam, float* dy,
+                                             float* partial) {
     __shared__ float red[kT];
     const size_t per = static_cast<size_t>(n) * n, count = per * batch;
-    const double pi = 3.14159265358979323846;
+    const float inv_c = 1.0f / static_cast<float>(count);
+    auto err = [&](size_t b, size_t k, size_t q) { return pos[k] * y[q] - static_cast<float>(u[b][k] / scale); };
+    auto hm = [](float x, float z) { return 2.0f * x * z / (x + z); };
     float s = 0.0f;
     for (size_t q = blockIdx.x * static_cast<size_t>(kT) + threadIdx.x; q < count; q += static_cast<size_t>(gridDim.x) * kT) {
         const size_t b = q / per, k = q % per;
         const int i = static_cast<int>(k / n), j = static_cast<int>(k % n);
-        const float pos = static_cast<float>(sin(pi * (i + 0.5) / n) * sin(pi * (j + 0.5) / n));
-        const float d = pos * y[q] - static_cast<float>(u[b][k] / scale);
-        dy[q] = 2.0f * d * pos / static_cast<float>(count);
-        s += d * d;
+        const double* ab = a[b];
+        const float e = err(b, k, q), ac = static_cast<float>(ab[k]);
+        float ae = 0.0f, energy = 0.0f;  // (A e)_c, and the faces this cell owns
+        if (j + 1 < n) {
+            const float d = e - err(b, k + 1, q + 1), f = hm(ac, static_cast<float>(ab[k + 1]));
+            ae += f * d;
+            energy += f * d * d;
+        } else {
+            ae += ac * e;
+            energy += ac * e * e;
+        }
+        if (j > 0)
+            ae += hm(ac, static_cast<float>(ab[k - 1])) * (e - err(b, k - 1, q - 1));
+        else {
+            ae += ac * e;
+            energy += ac * e * e;
+        }
+        if (i + 1 < n) {
+            const float d = e - err(b, k + n, q + n), f = hm(ac, static_cast<float>(ab[k + n]));
+            ae += f * d;
+            energy += f * d * d;
+        } else {
+            ae += ac * e;
+            energy += ac * e * e;
+        }
+        if (i > 0)
+            ae += hm(ac, static_cast<float>(ab[k - n])) * (e - err(b, k - n, q - n));
+        else {
+            ae += ac * e;
+            energy += ac * e * e;
+        }
+        dy[q] = 2.0f * (e + lam * ae) * inv_c * pos[k];
+        s += (e * e + lam * energy) * inv_c;
     }
     red[threadIdx.x] = s;
     __syncthreads();
@@ -424,7 +595,7 @@ __global__ __launch_bounds__(kT) void k_mse(const float* y, const double* const*
         if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
         __syncthreads();
     }
-    if (threadIdx.x == 0) partial[blockIdx.x] = red[0] / static_cast<float>(count);
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
 // Adam on the FP32 masters; refreshes the bf16 copy (wb may be null).
@@ -446,12 +617,12 @@ __global__ __launch_bounds__(kT) void k_to_bf16(const float* in, bf16* out, size
         out[i] = f2bf(in[i]);
 }
 
-// WT[ci][tap][co] = W[co][tap][ci].
+// WF[ci][8 - tap][co] = W[co][tap][ci]: the weights of the backward-data conv.
 __global__ __launch_bounds__(kT) void k_weight_t(const bf16* w, int cout, int cin, bf16* wt) {
     const size_t count = static_cast<size_t>(cout) * 9 * cin;
     for (size_t i = blockIdx.x * static_cast<size_t>(kT) + threadIdx.x; i < count; i += static_cast<size_t>(gridDim.x) * kT) {
         const int ci = static_cast<int>(i % cin), tap = static_cast<int>((i / cin) % 9), co = static_cast<int>(i / (9 * cin));
-        wt[(static_cast<size_t>(ci) * 9 + tap) * cout + co] = w[i];
+        wt[(static_cast<size_t>(ci) * 9 + (8 - tap)) * cout + co] = w[i];
     }
 }
 
@@ -492,18 +663,28 @@ hipError_t darcy_nn_mfma_probe(const void* A, const void* B, float* C, hipStream
 hipError_t darcy_nn_conv_fwd(const void* x, int b, int h, int w, int cin, const void* wgt, const float* bias, int cout,
                              int stride, void* z, void* act, hipStream_t s) {
     const int oh = h / stride, ow = w / stride, M = b * oh * ow;
+    EpiFwd epi{static_cast<bf16*>(z), static_cast<bf16*>(act), bias, M, cout};
+    if (pow2_at_least_8(cin))
+        return launch_conv(static_cast<const bf16*>(x), Geo{b, h, w, cin}, stride, static_cast<const bf16*>(wgt), cout,
+                           epi, s);
     FwdA ga{static_cast<const bf16*>(x), Geo{b, h, w, cin}, oh, ow, stride, M};
     RowsB gb{static_cast<const bf16*>(wgt), 9 * cin, cout};
-    EpiFwd epi{static_cast<bf16*>(z), static_cast<bf16*>(act), bias, M, cout};
     const dim3 grid((M + kTile - 1) / kTile, (cout + kTile - 1) / kTile, 1);
     k_gemm<<<grid, kT, 0, s>>>(9 * cin, 9 * cin, ga, gb, epi);
     return hipGetLastError();
 }
 
-// dx: [b][h][w][cin]; dz: [b][h/stride][w/stride][cout]; wt: [cin][9][cout].
+// dx: [b][h][w][cin]; dz: [b][h/stride][w/stride][cout]; wt: WF[cin][9][cout]
+// (flipped taps, from darcy_nn_weight_t).  Stride 1 is the forward conv of
+// dz with WF.
 hipError_t darcy_nn_conv_bwd_data(const void* dz, int b, int h, int w, int cin, const void* wt, int cout, int stride,
                                   void* dx, hipStream_t s) {
     const int M = b * h * w;
+    if (stride == 1 && pow2_at_least_8(cout)) {
+        EpiFwd epi{static_cast<bf16*>(dx), nullptr, nullptr, M, cin};
+        return launch_conv(static_cast<const bf16*>(dz), Geo{b, h, w, cout}, 1, static_cast<const bf16*>(wt), cin, epi,
+                           s);
+    }
     BwdDataA ga{static_cast<const bf16*>(dz), Geo{b, h / stride, w / stride, cout}, h, w, stride, M};
     RowsB gb{static_cast<const bf16*>(wt), 9 * cout, cin};
     EpiStore epi{static_cast<bf16*>(dx), M, cin};
@@ -557,15 +738,20 @@ hipError_t darcy_nn_head_bwd_weight(const void* x, int b, int h, int w, int cin,
     return hipGetLastError();
 }
 
-hipError_t darcy_nn_features(const double* const* a, int n, int batch, void* x, hipStream_t s) {
-    k_features<<<ew_grid(static_cast<size_t>(n) * n * batch), kT, 0, s>>>(a, n, batch, static_cast<bf16*>(x));
+hipError_t darcy_nn_position(int n, float* pos, hipStream_t s) {
+    k_position<<<ew_grid(static_cast<size_t>(n) * n), kT, 0, s>>>(n, pos);
+    return hipGetLastError();
+}
+
+hipError_t darcy_nn_features(const double* const* a, const float* pos, int n, int batch, void* x, hipStream_t s) {
+    k_features<<<ew_grid(static_cast<size_t>(n) * n * batch), kT, 0, s>>>(a, pos, n, batch, static_cast<bf16*>(x));
     return hipGetLastError();
 }
 
 // c: [b][h][w][ch] (coarse); skip, z, act: [b][2h][2w][ch]; z may be null.
 hipError_t darcy_nn_up_add_gelu(const void* c, int b, int h, int w, int ch, const void* skip, void* z, void* act,
                                 hipStream_t s) {
-    k_up_add_gelu<<<ew_grid(static_cast<size_t>(b) * h * w * ch * 4), kT, 0, s>>>(
+    k_up_add_gelu<<<ew_grid(static_cast<size_t>(b) * h * w * ch / 2), kT, 0, s>>>(
         static_cast<const bf16*>(c), Geo{b, h, w, ch}, static_cast<const bf16*>(skip), static_cast<bf16*>(z),
         static_cast<bf16*>(act));
     return hipGetLastError();
@@ -594,16 +780,16 @@ hipError_t darcy_nn_transpose(const void* in, int P, int C, void* out, hipStream
     return hipGetLastError();
 }
 
-hipError_t darcy_nn_output(const float* y, int n, int batch, const double* f, double scale, double* const* u0,
-                           hipStream_t s) {
-    k_output<<<ew_grid(static_cast<size_t>(n) * n * batch), kT, 0, s>>>(y, n, batch, f, scale, u0);
+hipError_t darcy_nn_output(const float* y, const float* pos, int n, int batch, const double* f, double scale,
+                           double* const* u0, hipStream_t s) {
+    k_output<<<ew_grid(static_cast<size_t>(n) * n * batch), kT, 0, s>>>(y, pos, n, batch, f, scale, u0);
     return hipGetLastError();
 }
 
 // partial: `blocks` floats.
-hipError_t darcy_nn_mse(const float* y, const double* const* u, int n, int batch, double scale, float* dy,
-                        float* partial, int blocks, hipStream_t s) {
-    k_mse<<<blocks, kT, 0, s>>>(y, u, n, batch, scale, dy, partial);
+hipError_t darcy_nn_loss(const float* y, const double* const* u, const double* const* a, const float* pos, int n,
+                         int batch, double scale, float lam, float* dy, float* partial, int blocks, hipStream_t s) {
+    k_loss<<<blocks, kT, 0, s>>>(y, u, a, pos, n, batch, scale, lam, dy, partial);
     return hipGetLastError();
 }
 

@@ -4,6 +4,7 @@
 // entry points used to check the conv kernels.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -28,17 +29,18 @@ hipError_t darcy_nn_head_bwd_data(const float* dy, int b, int h, int w, int cin,
                                   hipStream_t s);
 hipError_t darcy_nn_head_bwd_weight(const void* x, int b, int h, int w, int cin, const float* dy, int per, float* part,
                                     float* dw, float* db, hipStream_t s);
-hipError_t darcy_nn_features(const double* const* a, int n, int batch, void* x, hipStream_t s);
+hipError_t darcy_nn_position(int n, float* pos, hipStream_t s);
+hipError_t darcy_nn_features(const double* const* a, const float* pos, int n, int batch, void* x, hipStream_t s);
 hipError_t darcy_nn_up_add_gelu(const void* c, int b, int h, int w, int ch, const void* skip, void* z, void* act,
                                 hipStream_t s);
 hipError_t darcy_nn_gelu_bwd(void* d, const void* z, size_t count, hipStream_t s);
 hipError_t darcy_nn_down_sum(const void* dz, int b, int h, int w, int ch, void* dc, hipStream_t s);
 hipError_t darcy_nn_add(void* dst, const void* src, size_t count, hipStream_t s);
 hipError_t darcy_nn_transpose(const void* in, int P, int C, void* out, hipStream_t s);
-hipError_t darcy_nn_output(const float* y, int n, int batch, const double* f, double scale, double* const* u0,
-                           hipStream_t s);
-hipError_t darcy_nn_mse(const float* y, const double* const* u, int n, int batch, double scale, float* dy,
-                        float* partial, int blocks, hipStream_t s);
+hipError_t darcy_nn_output(const float* y, const float* pos, int n, int batch, const double* f, double scale,
+                           double* const* u0, hipStream_t s);
+hipError_t darcy_nn_loss(const float* y, const double* const* u, const double* const* a, const float* pos, int n,
+                         int batch, double scale, float lam, float* dy, float* partial, int blocks, hipStream_t s);
 hipError_t darcy_nn_adam(float* w, const float* g, float* m, float* v, void* wb, size_t count, float lr, float b1,
                          float b2, float eps, float c1, float c2, hipStream_t s);
 hipError_t darcy_nn_to_bf16(const float* in, void* out, size_t count, hipStream_t s);
@@ -47,30 +49,42 @@ hipError_t darcy_nn_weight_t(const void* w, int cout, int cin, void* wt, hipStre
 
 namespace {
 
-// The convolutions in parameter order: input channels, output channels,
-// stride, and the side of the input as a divisor of n.
-struct ConvSpec {
-    int cin, cout, stride, div;
-};
-constexpr int kConvs = 7;
-constexpr ConvSpec kSpec[kConvs] = {
-    {8, 32, 1, 1},    {32, 64, 2, 1},  {64, 128, 2, 2}, {128, 256, 2, 4},  // encoder
-    {256, 128, 1, 8}, {128, 64, 1, 4}, {64, 32, 1, 2},                     // decoder
-};
+// Levels of the U-Net: level l has side n / 2^l and kChan[l] channels.  The
+// encoder conv D_l produces level l (stride 2 for l > 0, from level l - 1;
+// D_0 reads the 8-channel features).  The decoder conv U_i maps level i + 1
+// (kChan[i+1] channels) to kChan[i] channels at that size; its output is
+// upsampled x2 (nearest), added to the encoder activation of level i and
+// passed through GELU.  Levels 0..3 (32-64-128-256) are the encoder-decoder
+// proper; levels 4..6 (256 channels, n/16 .. n/64) give every cell a
+// receptive field covering the whole grid for 18% more FLOPs.
+constexpr int kLevels = 7;
+constexpr int kChan[kLevels] = {32, 64, 128, 256, 256, 256, 256};
+constexpr int kInC = 8;
+constexpr int kConvs = 2 * kLevels - 1;  // D_0 .. D_6, then U_5 .. U_0
 constexpr int kHeadCin = 32;
 constexpr double kScale = 1.0 / 64.0;  // u0 = f * pos * y * kScale
-constexpr int kMseBlocks = 1024;
+constexpr float kEnergyWeight = 1000.0f;  // lam of the energy term of the loss
+constexpr int kLossBlocks = 1024;
 constexpr int kHeadPer = 2048;  // pixels per head-gradient partial
+
+struct ConvSpec {
+    int cin, cout, stride, level_in;
+};
+constexpr int u_index(int i) { return 2 * kLevels - 2 - i; }  // conv index of U_i
+constexpr ConvSpec spec(int c) {
+    return c < kLevels ? ConvSpec{c ? kChan[c - 1] : kInC, kChan[c], c ? 2 : 1, c ? c - 1 : 0}
+                       : ConvSpec{kChan[2 * kLevels - 1 - c], kChan[2 * kLevels - 2 - c], 1, 2 * kLevels - 1 - c};
+}
 
 struct Layout {
     size_t w[kConvs + 1], b[kConvs + 1], total;
     Layout() {
         size_t o = 0;
-        for (int l = 0; l <= kConvs; ++l) {
-            const int cin = l < kConvs ? kSpec[l].cin : kHeadCin, cout = l < kConvs ? kSpec[l].cout : 1;
-            w[l] = o;
+        for (int c = 0; c <= kConvs; ++c) {
+            const int cin = c < kConvs ? spec(c).cin : kHeadCin, cout = c < kConvs ? spec(c).cout : 1;
+            w[c] = o;
             o += static_cast<size_t>(cout) * 9 * cin;
-            b[l] = o;
+            b[c] = o;
             o += cout;
         }
         total = o;
@@ -88,10 +102,10 @@ int hip_fail(darcy_ctx* c, hipError_t e, const char* what) {
     return fail(c, code, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-#define NN_TRY(expr, what)                                         \
-    do {                                                           \
-        const hipError_t e_ = (expr);                              \
-        if (e_ != hipSuccess) return hip_fail(ctx, e_, what);      \
+#define NN_TRY(expr, what)                                    \
+    do {                                                      \
+        const hipError_t e_ = (expr);                         \
+        if (e_ != hipSuccess) return hip_fail(ctx, e_, what); \
     } while (0)
 
 int splits_for(int cout, int cin, size_t P) {
@@ -109,52 +123,48 @@ struct darcy_net {
     int n = 0, max_batch = 0;
     bool train = false;
     long step = 0;
-    DevBuf params, wb, wt, grad, m, v;
-    DevBuf feat, y;
-    DevBuf z[kConvs], act[kConvs];   // conv outputs (decoder: the conv result c)
-    DevBuf zm[3], am[3];             // decoder merges: up2(c) + skip, and its GELU
-    DevBuf dy, d[7], dc[3], tmp, wpart, hpart, mse, tab_a, tab_u, tab_f;
+    DevBuf params, wb, wt, grad, m, v, pos, feat, y;
+    DevBuf act[kLevels], z[kLevels];               // encoder outputs (after / before GELU)
+    DevBuf cbuf[kLevels - 1];                      // decoder conv outputs U_i (level i + 1 size)
+    DevBuf zm[kLevels - 1], am[kLevels - 1];       // merges at level i (before / after GELU)
+    DevBuf dA[kLevels], dM[kLevels - 1], dC[kLevels - 1];
+    DevBuf dy, tmp, wpart, hpart, lossp, tab_a, tab_u, tab_f;
 
-    int side(int l) const { return n / kSpec[l].div; }
-    size_t px(int div, int b) const { return static_cast<size_t>(b) * (n / div) * (n / div); }
-    const uint16_t* wbf(int l) const { return wb.as<uint16_t>() + kLayout.w[l]; }
-    const uint16_t* wtr(int l) const { return wt.as<uint16_t>() + kLayout.w[l]; }
-    const float* bias(int l) const { return params.as<float>() + kLayout.b[l]; }
+    int side(int level) const { return n >> level; }
+    size_t px(int level, int b) const { return static_cast<size_t>(b) * side(level) * side(level); }
+    const uint16_t* wbf(int c) const { return wb.as<uint16_t>() + kLayout.w[c]; }
+    const uint16_t* wtr(int c) const { return wt.as<uint16_t>() + kLayout.w[c]; }
+    const float* bias(int c) const { return params.as<float>() + kLayout.b[c]; }
 
     int alloc();
     int refresh_weights();
     int upload_tables(const void* const* a, const void* const* u, const double* f, int64_t count);
     int forward(int b, int64_t offset);
     int backward(int b, int64_t offset, double* loss);
-    int wgrad(int l, const void* x, const void* dz, int b);
+    int wgrad(int c, const void* x, const void* dz, int b);
+    void release();
 };
 
 int darcy_net::alloc() {
-    const size_t B = static_cast<size_t>(max_batch);
+    const int B = max_batch;
     const size_t P = kLayout.total;
-    auto bf = [&](DevBuf& buf, size_t elems) { return buf.reserve(elems * 2); };
+    auto bf = [](DevBuf& buf, size_t elems) { return buf.reserve(elems * 2); };
     NN_TRY(params.reserve(P * 4), "net parameters");
     NN_TRY(wb.reserve(P * 2), "net parameters");
     NN_TRY(wt.reserve(P * 2), "net parameters");
     NN_TRY(hipMemsetAsync(params.ptr, 0, P * 4, ctx->stream), "net parameters");
-    NN_TRY(bf(feat, px(1, B) * 8), "net activations");
-    NN_TRY(y.reserve(px(1, B) * 4), "net activations");
-    // Encoder: outputs at n, n/2, n/4, n/8.  Decoder convs at n/8, n/4, n/2.
-    const int out_div[kConvs] = {1, 2, 4, 8, 8, 4, 2};
-    for (int l = 0; l < kConvs; ++l) {
-        const size_t e = px(out_div[l], B) * kSpec[l].cout;
-        if (l < 4) {
-            NN_TRY(bf(act[l], e), "net activations");
-            if (train) NN_TRY(bf(z[l], e), "net activations");
-        } else {
-            NN_TRY(bf(z[l], e), "net activations");
-        }
+    NN_TRY(pos.reserve(static_cast<size_t>(n) * n * 4), "net position");
+    NN_TRY(darcy_nn_position(n, pos.as<float>(), ctx->stream), "net position");
+    NN_TRY(bf(feat, px(0, B) * kInC), "net activations");
+    NN_TRY(y.reserve(px(0, B) * 4), "net activations");
+    for (int l = 0; l < kLevels; ++l) {
+        NN_TRY(bf(act[l], px(l, B) * kChan[l]), "net activations");
+        if (train) NN_TRY(bf(z[l], px(l, B) * kChan[l]), "net activations");
     }
-    const int merge_div[3] = {4, 2, 1}, merge_c[3] = {128, 64, 32};
-    for (int k = 0; k < 3; ++k) {
-        const size_t e = px(merge_div[k], B) * merge_c[k];
-        NN_TRY(bf(am[k], e), "net activations");
-        if (train) NN_TRY(bf(zm[k], e), "net activations");
+    for (int i = 0; i + 1 < kLevels; ++i) {
+        NN_TRY(bf(cbuf[i], px(i + 1, B) * kChan[i]), "net activations");
+        NN_TRY(bf(am[i], px(i, B) * kChan[i]), "net activations");
+        if (train) NN_TRY(bf(zm[i], px(i, B) * kChan[i]), "net activations");
     }
     if (train) {
         NN_TRY(grad.reserve(P * 4), "net gradients");
@@ -162,34 +172,51 @@ int darcy_net::alloc() {
         NN_TRY(v.reserve(P * 4), "optimizer state");
         NN_TRY(hipMemsetAsync(m.ptr, 0, P * 4, ctx->stream), "optimizer state");
         NN_TRY(hipMemsetAsync(v.ptr, 0, P * 4, ctx->stream), "optimizer state");
-        NN_TRY(dy.reserve(px(1, B) * 4), "net gradients");
-        // d[0..6]: gradients of the activations a0 a1 a2 a3 (encoder) and of
-        // the merges (n/4, 128), (n/2, 64), (n, 32).
-        const int d_div[7] = {1, 2, 4, 8, 4, 2, 1}, d_c[7] = {32, 64, 128, 256, 128, 64, 32};
-        for (int k = 0; k < 7; ++k) NN_TRY(bf(d[k], px(d_div[k], B) * d_c[k]), "net gradients");
-        const int dc_div[3] = {8, 4, 2}, dc_c[3] = {128, 64, 32};
-        for (int k = 0; k < 3; ++k) NN_TRY(bf(dc[k], px(dc_div[k], B) * dc_c[k]), "net gradients");
-        NN_TRY(bf(tmp, px(1, B) * 32), "net gradients");
-        size_t part = 0;
-        for (int l = 0; l < kConvs; ++l) {
-            const int od = out_div[l];
-            const int s = splits_for(kSpec[l].cout, kSpec[l].cin, px(od, B));
-            const size_t e = static_cast<size_t>(s) * kSpec[l].cout * (9 * kSpec[l].cin + 1);
-            if (e > part) part = e;
+        NN_TRY(dy.reserve(px(0, B) * 4), "net gradients");
+        for (int l = 0; l < kLevels; ++l) NN_TRY(bf(dA[l], px(l, B) * kChan[l]), "net gradients");
+        for (int i = 0; i + 1 < kLevels; ++i) {
+            NN_TRY(bf(dM[i], px(i, B) * kChan[i]), "net gradients");
+            NN_TRY(bf(dC[i], px(i + 1, B) * kChan[i]), "net gradients");
         }
+        size_t tmax = 0, part = 0;
+        for (int c = 0; c < kConvs; ++c) {
+            const ConvSpec s = spec(c);
+            const size_t out_px = px(s.level_in, B) / (s.stride * s.stride);
+            tmax = std::max(tmax, out_px * s.cout);
+            part = std::max(part, static_cast<size_t>(splits_for(s.cout, s.cin, out_px)) * s.cout * (9 * s.cin + 1));
+        }
+        NN_TRY(bf(tmp, tmax), "net gradients");
         NN_TRY(wpart.reserve(part * 4), "net gradients");
-        NN_TRY(hpart.reserve(((px(1, B) + kHeadPer - 1) / kHeadPer) * (9 * kHeadCin + 1) * 4), "net gradients");
-        NN_TRY(mse.reserve(kMseBlocks * 4), "net gradients");
+        NN_TRY(hpart.reserve(((px(0, B) + kHeadPer - 1) / kHeadPer) * (9 * kHeadCin + 1) * 4), "net gradients");
+        NN_TRY(lossp.reserve(kLossBlocks * 4), "net gradients");
     }
     return refresh_weights();
 }
 
-// bf16 copies of the weights and, for training, their transposes.
+void darcy_net::release() {
+    DevBuf* all[] = {&params, &wb, &wt, &grad, &m, &v, &pos, &feat, &y, &dy, &tmp, &wpart, &hpart, &lossp,
+                     &tab_a, &tab_u, &tab_f};
+    for (DevBuf* b : all) b->release();
+    for (int l = 0; l < kLevels; ++l) {
+        act[l].release();
+        z[l].release();
+        dA[l].release();
+    }
+    for (int i = 0; i + 1 < kLevels; ++i) {
+        cbuf[i].release();
+        zm[i].release();
+        am[i].release();
+        dM[i].release();
+        dC[i].release();
+    }
+}
+
+// bf16 copies of the weights and the flipped transposes used by backward data.
 int darcy_net::refresh_weights() {
     NN_TRY(darcy_nn_to_bf16(params.as<float>(), wb.ptr, kLayout.total, ctx->stream), "net weights");
-    for (int l = 1; l < kConvs; ++l)
-        NN_TRY(darcy_nn_weight_t(wb.as<uint16_t>() + kLayout.w[l], kSpec[l].cout, kSpec[l].cin,
-                                 wt.as<uint16_t>() + kLayout.w[l], ctx->stream),
+    for (int c = 1; c < kConvs; ++c)
+        NN_TRY(darcy_nn_weight_t(wb.as<uint16_t>() + kLayout.w[c], spec(c).cout, spec(c).cin,
+                                 wt.as<uint16_t>() + kLayout.w[c], ctx->stream),
                "net weights");
     return DARCY_OK;
 }
@@ -214,25 +241,24 @@ int darcy_net::upload_tables(const void* const* a, const void* const* u, const d
 // Forward pass of problems [offset, offset + b) of the uploaded tables.
 int darcy_net::forward(int b, int64_t offset) {
     hipStream_t s = ctx->stream;
-    const double* const* a = tab_a.as<const double*>() + offset;
-    NN_TRY(darcy_nn_features(a, n, b, feat.ptr, s), "net features");
+    NN_TRY(darcy_nn_features(tab_a.as<const double*>() + offset, pos.as<float>(), n, b, feat.ptr, s), "net features");
     const void* x = feat.ptr;
-    for (int l = 0; l < 4; ++l) {
-        const int h = side(l);
-        NN_TRY(darcy_nn_conv_fwd(x, b, h, h, kSpec[l].cin, wbf(l), bias(l), kSpec[l].cout, kSpec[l].stride,
-                                 train ? z[l].ptr : nullptr, act[l].ptr, s),
+    for (int l = 0; l < kLevels; ++l) {
+        const ConvSpec c = spec(l);
+        const int h = side(c.level_in);
+        NN_TRY(darcy_nn_conv_fwd(x, b, h, h, c.cin, wbf(l), bias(l), c.cout, c.stride, train ? z[l].ptr : nullptr,
+                                 act[l].ptr, s),
                "net encoder");
         x = act[l].ptr;
     }
-    const void* skip[3] = {act[2].ptr, act[1].ptr, act[0].ptr};
-    for (int k = 0; k < 3; ++k) {
-        const int l = 4 + k, h = side(l);
-        NN_TRY(darcy_nn_conv_fwd(x, b, h, h, kSpec[l].cin, wbf(l), bias(l), kSpec[l].cout, 1, z[l].ptr, nullptr, s),
+    for (int i = kLevels - 2; i >= 0; --i) {
+        const int c = u_index(i), h = side(i + 1);
+        NN_TRY(darcy_nn_conv_fwd(x, b, h, h, kChan[i + 1], wbf(c), bias(c), kChan[i], 1, cbuf[i].ptr, nullptr, s),
                "net decoder");
-        NN_TRY(darcy_nn_up_add_gelu(z[l].ptr, b, h, h, kSpec[l].cout, skip[k], train ? zm[k].ptr : nullptr, am[k].ptr,
+        NN_TRY(darcy_nn_up_add_gelu(cbuf[i].ptr, b, h, h, kChan[i], act[i].ptr, train ? zm[i].ptr : nullptr, am[i].ptr,
                                     s),
                "net decoder");
-        x = am[k].ptr;
+        x = am[i].ptr;
     }
     NN_TRY(darcy_nn_head_fwd(x, b, n, n, kHeadCin, params.as<float>() + kLayout.w[kConvs],
                              params.as<float>() + kLayout.b[kConvs], y.as<float>(), s),
@@ -240,15 +266,15 @@ int darcy_net::forward(int b, int64_t offset) {
     return DARCY_OK;
 }
 
-// Weight and bias gradient of conv l from its input x and output gradient dz.
-int darcy_net::wgrad(int l, const void* x, const void* dz, int b) {
-    const ConvSpec& c = kSpec[l];
-    const int h = side(l), oh = h / c.stride;
+// Weight and bias gradient of conv c from its input x and output gradient dz.
+int darcy_net::wgrad(int c, const void* x, const void* dz, int b) {
+    const ConvSpec sp = spec(c);
+    const int h = side(sp.level_in), oh = h / sp.stride;
     const size_t P = static_cast<size_t>(b) * oh * oh;
-    NN_TRY(darcy_nn_transpose(dz, static_cast<int>(P), c.cout, tmp.ptr, ctx->stream), "net backward");
-    const int splits = splits_for(c.cout, c.cin, P);
-    NN_TRY(darcy_nn_conv_bwd_weight(x, b, h, h, c.cin, tmp.ptr, c.cout, c.stride, splits, wpart.as<float>(),
-                                    grad.as<float>() + kLayout.w[l], grad.as<float>() + kLayout.b[l], ctx->stream),
+    NN_TRY(darcy_nn_transpose(dz, static_cast<int>(P), sp.cout, tmp.ptr, ctx->stream), "net backward");
+    const int splits = splits_for(sp.cout, sp.cin, P);
+    NN_TRY(darcy_nn_conv_bwd_weight(x, b, h, h, sp.cin, tmp.ptr, sp.cout, sp.stride, splits, wpart.as<float>(),
+                                    grad.as<float>() + kLayout.w[c], grad.as<float>() + kLayout.b[c], ctx->stream),
            "net backward");
     return DARCY_OK;
 }
@@ -256,46 +282,44 @@ int darcy_net::wgrad(int l, const void* x, const void* dz, int b) {
 // Loss and gradient for the batch whose forward pass just ran.
 int darcy_net::backward(int b, int64_t offset, double* loss) {
     hipStream_t s = ctx->stream;
-    const double* const* u = tab_u.as<const double*>() + offset;
-    NN_TRY(darcy_nn_mse(y.as<float>(), u, n, b, kScale, dy.as<float>(), mse.as<float>(), kMseBlocks, s), "net loss");
-    // Head.
+    NN_TRY(darcy_nn_loss(y.as<float>(), tab_u.as<const double*>() + offset, tab_a.as<const double*>() + offset,
+                         pos.as<float>(), n, b, kScale, kEnergyWeight, dy.as<float>(), lossp.as<float>(), kLossBlocks,
+                         s),
+           "net loss");
     const float* wh = params.as<float>() + kLayout.w[kConvs];
-    NN_TRY(darcy_nn_head_bwd_data(dy.as<float>(), b, n, n, kHeadCin, wh, d[6].ptr, s), "net backward");
-    NN_TRY(darcy_nn_head_bwd_weight(am[2].ptr, b, n, n, kHeadCin, dy.as<float>(), kHeadPer, hpart.as<float>(),
+    NN_TRY(darcy_nn_head_bwd_data(dy.as<float>(), b, n, n, kHeadCin, wh, dM[0].ptr, s), "net backward");
+    NN_TRY(darcy_nn_head_bwd_weight(am[0].ptr, b, n, n, kHeadCin, dy.as<float>(), kHeadPer, hpart.as<float>(),
                                     grad.as<float>() + kLayout.w[kConvs], grad.as<float>() + kLayout.b[kConvs], s),
            "net backward");
-    // Decoder: merge k (sizes n/4, n/2, n) follows conv 4 + k; walk back.
-    const void* conv_in[3] = {act[3].ptr, am[0].ptr, am[1].ptr};
-    DevBuf* dmerge[3] = {&d[4], &d[5], &d[6]};
-    DevBuf* dconv_in[3] = {&d[3], &d[4], &d[5]};
-    for (int k = 2; k >= 0; --k) {
-        const int l = 4 + k, h = side(l), ch = kSpec[l].cout;
-        const size_t cnt = static_cast<size_t>(b) * (2 * h) * (2 * h) * ch;
-        NN_TRY(darcy_nn_gelu_bwd(dmerge[k]->ptr, zm[k].ptr, cnt, s), "net backward");
-        NN_TRY(darcy_nn_down_sum(dmerge[k]->ptr, b, h, h, ch, dc[k].ptr, s), "net backward");
-        if (int r = wgrad(l, conv_in[k], dc[k].ptr, b)) return r;
-        NN_TRY(darcy_nn_conv_bwd_data(dc[k].ptr, b, h, h, kSpec[l].cin, wtr(l), ch, 1, dconv_in[k]->ptr, s),
+    // Decoder, finest merge first: dM[i] holds the gradient of merge i's
+    // activation; after the GELU step it is the gradient of the merge's sum,
+    // which is also the skip gradient of the encoder activation act[i].
+    for (int i = 0; i + 1 < kLevels; ++i) {
+        const int h = side(i + 1), ch = kChan[i], c = u_index(i);
+        NN_TRY(darcy_nn_gelu_bwd(dM[i].ptr, zm[i].ptr, px(i, b) * ch, s), "net backward");
+        NN_TRY(darcy_nn_down_sum(dM[i].ptr, b, h, h, ch, dC[i].ptr, s), "net backward");
+        const bool last = i + 2 == kLevels;
+        if (int r = wgrad(c, last ? act[kLevels - 1].ptr : am[i + 1].ptr, dC[i].ptr, b)) return r;
+        NN_TRY(darcy_nn_conv_bwd_data(dC[i].ptr, b, h, h, kChan[i + 1], wtr(c), ch, 1,
+                                      last ? dA[kLevels - 1].ptr : dM[i + 1].ptr, s),
                "net backward");
     }
-    // Encoder: conv l reads act[l-1] (or the features) and feeds act[l];
-    // act[0..2] also receive the skip gradients held by merges 2, 1, 0.
-    const size_t cnt3 = px(8, b) * 256;
-    NN_TRY(darcy_nn_gelu_bwd(d[3].ptr, z[3].ptr, cnt3, s), "net backward");
-    for (int l = 3; l >= 0; --l) {
-        const ConvSpec& c = kSpec[l];
-        const int h = side(l);
-        const void* x = l > 0 ? act[l - 1].ptr : feat.ptr;
-        if (int r = wgrad(l, x, d[l].ptr, b)) return r;
+    // Encoder, deepest first.
+    NN_TRY(darcy_nn_gelu_bwd(dA[kLevels - 1].ptr, z[kLevels - 1].ptr, px(kLevels - 1, b) * kChan[kLevels - 1], s),
+           "net backward");
+    for (int l = kLevels - 1; l >= 0; --l) {
+        if (int r = wgrad(l, l ? act[l - 1].ptr : feat.ptr, dA[l].ptr, b)) return r;
         if (l == 0) break;
-        const size_t cnt = static_cast<size_t>(b) * h * h * c.cin;
-        NN_TRY(darcy_nn_conv_bwd_data(d[l].ptr, b, h, h, c.cin, wtr(l), c.cout, c.stride, d[l - 1].ptr, s),
+        const int h = side(l - 1);
+        const size_t cnt = px(l - 1, b) * kChan[l - 1];
+        NN_TRY(darcy_nn_conv_bwd_data(dA[l].ptr, b, h, h, kChan[l - 1], wtr(l), kChan[l], 2, dA[l - 1].ptr, s),
                "net backward");
-        NN_TRY(darcy_nn_add(d[l - 1].ptr, d[7 - l].ptr, cnt, s), "net backward");  // skip of merge 3 - l
-        NN_TRY(darcy_nn_gelu_bwd(d[l - 1].ptr, z[l - 1].ptr, cnt, s), "net backward");
+        NN_TRY(darcy_nn_add(dA[l - 1].ptr, dM[l - 1].ptr, cnt, s), "net backward");
+        NN_TRY(darcy_nn_gelu_bwd(dA[l - 1].ptr, z[l - 1].ptr, cnt, s), "net backward");
     }
     if (loss) {
-        std::vector<float> part(kMseBlocks);
-        NN_TRY(hipMemcpyAsync(part.data(), mse.ptr, kMseBlocks * 4, hipMemcpyDeviceToHost, s), "net loss");
+        std::vector<float> part(kLossBlocks);
+        NN_TRY(hipMemcpyAsync(part.data(), lossp.ptr, kLossBlocks * 4, hipMemcpyDeviceToHost, s), "net loss");
         NN_TRY(hipStreamSynchronize(s), "net loss");
         double sum = 0.0;
         for (float p : part) sum += p;
@@ -308,10 +332,7 @@ int darcy_net::backward(int b, int64_t offset, double* loss) {
 
 namespace {
 
-int check_net(darcy_net* net) {
-    if (!net || !net->ctx) return DARCY_INVALID_INPUT;
-    return DARCY_OK;
-}
+int check_net(darcy_net* net) { return (!net || !net->ctx) ? DARCY_INVALID_INPUT : DARCY_OK; }
 
 hipError_t set_device(darcy_ctx* ctx) { return hipSetDevice(ctx->device); }
 
@@ -319,33 +340,13 @@ hipError_t set_device(darcy_ctx* ctx) { return hipSetDevice(ctx->device); }
 
 extern "C" int64_t darcy_net_param_count(void) { return static_cast<int64_t>(kLayout.total); }
 
-namespace {
-
-void release_all(darcy_net* net) {
-    DevBuf* all[] = {&net->params, &net->wb,  &net->wt,   &net->grad,  &net->m,     &net->v,   &net->feat,  &net->y,
-                     &net->dy,     &net->tmp, &net->wpart, &net->hpart, &net->mse, &net->tab_a, &net->tab_u, &net->tab_f};
-    for (DevBuf* b : all) b->release();
-    for (int l = 0; l < kConvs; ++l) {
-        net->z[l].release();
-        net->act[l].release();
-    }
-    for (int k = 0; k < 3; ++k) {
-        net->zm[k].release();
-        net->am[k].release();
-        net->dc[k].release();
-    }
-    for (int k = 0; k < 7; ++k) net->d[k].release();
-}
-
-}  // namespace
-
 extern "C" void darcy_net_destroy(darcy_net* net) {
     if (!net) return;
     if (net->ctx) {
         std::lock_guard<std::mutex> lock(net->ctx->mu);
         (void)hipSetDevice(net->ctx->device);
         (void)hipStreamSynchronize(net->ctx->stream);
-        release_all(net);
+        net->release();
     }
     delete net;
 }
@@ -354,9 +355,10 @@ extern "C" int darcy_net_create(darcy_ctx* ctx, int64_t n, int64_t max_batch, in
     if (!ctx || !out) return DARCY_INVALID_INPUT;
     *out = nullptr;
     std::lock_guard<std::mutex> lock(ctx->mu);
-    if (n < 32 || n % 32 != 0 || n > 4096)
-        return fail(ctx, DARCY_INVALID_INPUT, "the warm-start network needs n a multiple of 32 in [32, 4096]");
-    if (max_batch < 1 || static_cast<double>(max_batch) * n * n > 2.0e9 / 32)
+    const int64_t unit = int64_t(1) << (kLevels - 1);
+    if (n < unit || n % unit != 0 || n > 4096)
+        return fail(ctx, DARCY_INVALID_INPUT, "the warm-start network needs n a multiple of 64 in [64, 4096]");
+    if (max_batch < 1 || static_cast<double>(max_batch) * n * n > 6.25e7)
         return fail(ctx, DARCY_INVALID_INPUT, "max_batch must be at least 1 and max_batch * n * n at most 6.25e7");
     NN_TRY(set_device(ctx), "hipSetDevice");
     darcy_net* net = new darcy_net;
@@ -367,7 +369,7 @@ extern "C" int darcy_net_create(darcy_ctx* ctx, int64_t n, int64_t max_batch, in
     const int rc = net->alloc();
     if (rc != DARCY_OK) {
         (void)hipStreamSynchronize(ctx->stream);
-        release_all(net);
+        net->release();
         delete net;
         return rc;
     }
@@ -416,15 +418,15 @@ extern "C" int darcy_net_predict(darcy_net* net, int64_t count, const double* co
         return fail(ctx, DARCY_INVALID_INPUT, "predict needs a, f and u0 tables");
     if (count == 0) return DARCY_OK;
     NN_TRY(set_device(ctx), "hipSetDevice");
-    NN_TRY(hipEventRecord(ctx->ev0, ctx->stream), "hipEventRecord");
     if (int r = net->upload_tables(reinterpret_cast<const void* const*>(a), reinterpret_cast<const void* const*>(u0),
                                    f, count))
         return r;
+    NN_TRY(hipEventRecord(ctx->ev0, ctx->stream), "hipEventRecord");
     for (int64_t off = 0; off < count; off += net->max_batch) {
         const int b = static_cast<int>(count - off < net->max_batch ? count - off : net->max_batch);
         if (int r = net->forward(b, off)) return r;
-        NN_TRY(darcy_nn_output(net->y.as<float>(), net->n, b, net->tab_f.as<double>() + off, kScale,
-                               net->tab_u.as<double*>() + off, ctx->stream),
+        NN_TRY(darcy_nn_output(net->y.as<float>(), net->pos.as<float>(), net->n, b, net->tab_f.as<double>() + off,
+                               kScale, net->tab_u.as<double*>() + off, ctx->stream),
                "net output");
     }
     NN_TRY(hipEventRecord(ctx->ev1, ctx->stream), "hipEventRecord");
@@ -465,9 +467,9 @@ int train_or_grad(darcy_net* net, int64_t batch, const double* const* a, const d
         NN_TRY(darcy_nn_adam(net->params.as<float>(), net->grad.as<float>(), net->m.as<float>(), net->v.as<float>(),
                              net->wb.ptr, kLayout.total, *lr, b1, b2, 1e-8f, c1, c2, ctx->stream),
                "net update");
-        for (int l = 1; l < kConvs; ++l)
-            NN_TRY(darcy_nn_weight_t(net->wb.as<uint16_t>() + kLayout.w[l], kSpec[l].cout, kSpec[l].cin,
-                                     net->wt.as<uint16_t>() + kLayout.w[l], ctx->stream),
+        for (int c = 1; c < kConvs; ++c)
+            NN_TRY(darcy_nn_weight_t(net->wb.as<uint16_t>() + kLayout.w[c], spec(c).cout, spec(c).cin,
+                                     net->wt.as<uint16_t>() + kLayout.w[c], ctx->stream),
                    "net update");
     }
     return DARCY_OK;

@@ -158,16 +158,19 @@ int darcy_device_info(const darcy_ctx* ctx, char* buf, int32_t cap);
  * darcy_solve / darcy_solve_batch, and CG still runs to tol from it.
  *
  * The network maps (ln a, the forcing, a position channel) on an n x n grid
- * (n a multiple of 32) to u0 through a 4-level conv encoder-decoder with
- * 32-64-128-256 channels, 3x3 kernels, GELU, bf16 weights and activations
- * and FP32 accumulation:
- *   enc: conv 8->32 (n), 32->64 /2, 64->128 /2, 128->256 /2 (each + GELU)
- *   dec: conv 256->128, 128->64, 64->32, each followed by x2 nearest
- *        upsampling, + the encoder activation of that size, GELU
+ * (n a multiple of 64) to u0 through a U-Net encoder-decoder with 3x3
+ * kernels, GELU, bf16 weights and activations and FP32 accumulation.  Level
+ * l has side n / 2^l and channels 32, 64, 128, 256 (levels 0-3, the
+ * encoder-decoder proper), then 256, 256, 256 (levels 4-6, a context stack
+ * that gives every cell a receptive field spanning the grid):
+ *   enc: D_0: conv 8->32 at n; D_l: stride-2 conv from level l-1 to level l
+ *   dec: U_i (i = 5..0): conv at level i+1 to the channels of level i, x2
+ *        nearest upsampling, + the encoder activation of level i, GELU
  *   head: conv 32->1 in FP32;  u0 = f * pos * y / 64,
  *        pos = sin(pi (i + 1/2) / n) sin(pi (j + 1/2) / n).
- * Parameters are one FP32 vector: for each conv in that order W[cout][3][3][cin]
- * then bias[cout] (the first conv has 8 input channels, 3 of them used).
+ * Parameters are one FP32 vector: for each conv in the order D_0..D_6,
+ * U_5..U_0, head: W[cout][3][3][cin] then bias[cout] (D_0 has 8 input
+ * channels, 3 of them used).
  * ------------------------------------------------------------------------ */
 typedef struct darcy_net darcy_net;
 
@@ -187,9 +190,11 @@ int darcy_net_get_params(darcy_net* net, float* params, int64_t count);
  * `count` problems (any count; processed max_batch at a time). */
 int darcy_net_predict(darcy_net* net, int64_t count, const double* const* a, const double* f, double* const* u0);
 
-/* One Adam step (beta 0.9 / 0.999, eps 1e-8) on the mean squared error of
- * pos * y against u / scale over a batch of device-resident pairs (a, u)
- * with f = 1.  *loss (may be null) receives the loss before the step. */
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) over a batch of
+ * device-resident pairs (a, u) with f = 1, on the loss
+ *   (sum_cells e^2 + 1000 e^T A e) / cells,  e = pos y - 64 u,
+ * A the finite-volume operator of a (the energy norm CG converges in).
+ * *loss (may be null) receives the loss before the step. */
 int darcy_net_train_step(darcy_net* net, int64_t batch, const double* const* a, const double* const* u, float lr,
                          double* loss);
 

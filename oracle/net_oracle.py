@@ -3,12 +3,16 @@
 Test infrastructure: only tests, the smoke check and the benchmark's CPU leg
 use it, as the checker of the GPU network (libdarcy_hip: darcy_net_*,
 darcy_nn_*).  The reference has no learned component, so nothing here
-restates reference code; the network is the one the C header describes:
+restates reference code; the network is the one the C header describes: a
+U-Net whose level l has side n / 2^l and CHAN[l] channels,
 
-    enc: conv 8->32 (n), 32->64 /2, 64->128 /2, 128->256 /2, each + GELU
-    dec: conv 256->128, 128->64, 64->32, each + nearest x2 upsampling
-         + the encoder activation of that size, then GELU
+    enc: D_0 conv 8->32 at n, D_l stride-2 conv from level l-1 to level l
+    dec: U_i (i = L-2..0) conv at level i+1 to CHAN[i] channels, nearest x2
+         upsampling, + the encoder activation of level i, GELU
     head: conv 32->1;  u0 = f * pos * y / 64
+
+and the training loss (sum e^2 + 1000 e^T A e) / cells, e = pos y - 64 u, A the
+finite-volume operator of the permeability.
 
 Two modes share one code path:
 
@@ -25,18 +29,26 @@ from __future__ import annotations
 
 import numpy as np
 
-# (cin, cout, stride, input side as a divisor of n), in parameter order.
-SPEC = [(8, 32, 1, 1), (32, 64, 2, 1), (64, 128, 2, 2), (128, 256, 2, 4),
-        (256, 128, 1, 8), (128, 64, 1, 4), (64, 32, 1, 2)]
+CHAN = [32, 64, 128, 256, 256, 256, 256]
+LEVELS = len(CHAN)
+IN_C = 8
 HEAD_CIN = 32
 SCALE = 1.0 / 64.0
+ENERGY_WEIGHT = 1000.0
 LN3, LN12 = np.log(3.0), np.log(12.0)
 
 
+def conv_specs() -> list[tuple[int, int, int, int]]:
+    """(cin, cout, stride, input level) of D_0..D_{L-1}, U_{L-2}..U_0, in parameter order."""
+    out = [(IN_C if l == 0 else CHAN[l - 1], CHAN[l], 1 if l == 0 else 2, max(l - 1, 0)) for l in range(LEVELS)]
+    out += [(CHAN[i + 1], CHAN[i], 1, i + 1) for i in range(LEVELS - 2, -1, -1)]
+    return out
+
+
 def layout() -> tuple[list[tuple[int, int, tuple[int, ...]]], int]:
-    """[(w_offset, b_offset, w_shape)] for the 7 convs and the head, and the total."""
+    """[(w_offset, b_offset, w_shape)] for the convs and the head, and the total."""
     out, o = [], 0
-    for cin, cout, _, _ in SPEC + [(HEAD_CIN, 1, 1, 1)]:
+    for cin, cout, _, _ in conv_specs() + [(HEAD_CIN, 1, 1, 0)]:
         shape = (cout, 3, 3, cin)
         w = o
         o += int(np.prod(shape))
@@ -158,6 +170,24 @@ def conv_direct(x: np.ndarray, w: np.ndarray, bias: np.ndarray, stride: int) -> 
     return z
 
 
+def operator(a: np.ndarray, x: np.ndarray) -> np.ndarray:
+    """A x for a batch (B, n, n): harmonic interior faces, boundary faces
+    carrying the cell's own a (the solver's operator, without the h^2 of b)."""
+    hm = lambda p, q: 2.0 * p * q / (p + q)  # noqa: E731
+    out = np.zeros_like(x)
+    fe, fn = hm(a[:, :, :-1], a[:, :, 1:]), hm(a[:, :-1, :], a[:, 1:, :])
+    dx, dy = x[:, :, :-1] - x[:, :, 1:], x[:, :-1, :] - x[:, 1:, :]
+    out[:, :, :-1] += fe * dx
+    out[:, :, 1:] -= fe * dx
+    out[:, :-1, :] += fn * dy
+    out[:, 1:, :] -= fn * dy
+    out[:, :, 0] += a[:, :, 0] * x[:, :, 0]
+    out[:, :, -1] += a[:, :, -1] * x[:, :, -1]
+    out[:, 0, :] += a[:, 0, :] * x[:, 0, :]
+    out[:, -1, :] += a[:, -1, :] * x[:, -1, :]
+    return out
+
+
 def upsample2(c: np.ndarray) -> np.ndarray:
     return np.repeat(np.repeat(c, 2, axis=1), 2, axis=2)
 
@@ -187,26 +217,30 @@ class Net:
         self.emulate = emulate
         self.layers = unpack(np.asarray(params, dtype=np.float64))
         self.R = bf16 if emulate else (lambda v: v)
+        nconv = 2 * LEVELS - 1
         # Conv weights as the GPU uses them (bf16 copies); the head stays FP32.
-        self.w = [(self.R(w), b) for w, b in self.layers[:7]] + [self.layers[7]]
+        self.w = [(self.R(w), b) for w, b in self.layers[:nconv]] + [self.layers[nconv]]
+
+    @staticmethod
+    def u_index(i: int) -> int:
+        return 2 * LEVELS - 2 - i
 
     def forward(self, a: np.ndarray) -> dict:
         R, s = self.R, {}
         x = R(features(a))
         s["x"] = x
-        for l in range(4):
+        for l in range(LEVELS):
             w, b = self.w[l]
-            v = conv_forward(x, w, b, SPEC[l][2])
+            v = conv_forward(x, w, b, 1 if l == 0 else 2)
             s[f"z{l}"], s[f"a{l}"] = R(v), R(gelu(v))
             x = s[f"a{l}"]
-        for k in range(3):
-            l = 4 + k
-            w, b = self.w[l]
+        for i in range(LEVELS - 2, -1, -1):
+            w, b = self.w[self.u_index(i)]
             c = R(conv_forward(x, w, b, 1))
-            v = upsample2(c) + s[f"a{2 - k}"]
-            s[f"zm{k}"], s[f"am{k}"] = R(v), R(gelu(v))
-            x = s[f"am{k}"]
-        w, b = self.w[7]
+            v = upsample2(c) + s[f"a{i}"]
+            s[f"zm{i}"], s[f"am{i}"] = R(v), R(gelu(v))
+            x = s[f"am{i}"]
+        w, b = self.w[-1]
         s["y"] = conv_forward(x, w, b, 1)[..., 0]
         return s
 
@@ -216,37 +250,38 @@ class Net:
         t = u / SCALE
         if self.emulate:
             t = t.astype(np.float32).astype(np.float64)
-        d = pos * s["y"] - t
-        s["dy"] = 2.0 * d * pos / d.size
-        return float(np.mean(d * d)), s
+        e = pos * s["y"] - t
+        ae = operator(a, e)
+        s["dy"] = 2.0 * (e + ENERGY_WEIGHT * ae) / e.size * pos
+        return float((np.sum(e * e) + ENERGY_WEIGHT * np.sum(e * ae)) / e.size), s
 
     def gradient(self, a: np.ndarray, u: np.ndarray) -> tuple[float, np.ndarray]:
         R = self.R
         loss, s = self.loss(a, u)
-        grads: list = [None] * 8
+        grads: list = [None] * (2 * LEVELS)
         dy = s["dy"][..., None]
-        w, _ = self.w[7]
-        grads[7] = conv_backward_weight(s["am2"], dy, 1)
-        dm = R(conv_backward_data(dy, w, s["am2"].shape, 1))
-        # Decoder, last merge first.
-        dmerge = [None, None, None]
-        conv_in = [s["a3"], s["am0"], s["am1"]]
-        for k in (2, 1, 0):
-            l = 4 + k
-            dz = R(dm * gelu_grad(s[f"zm{k}"]))
-            dmerge[k] = dz
+        w, _ = self.w[-1]
+        grads[-1] = conv_backward_weight(s["am0"], dy, 1)
+        dm = R(conv_backward_data(dy, w, s["am0"].shape, 1))
+        # Decoder, finest merge first; dzm[i] is also the skip gradient of a_i.
+        dzm = {}
+        for i in range(LEVELS - 1):
+            c = self.u_index(i)
+            dz = R(dm * gelu_grad(s[f"zm{i}"]))
+            dzm[i] = dz
             dc = R(down_sum(dz))
-            grads[l] = conv_backward_weight(conv_in[k], dc, 1)
-            dm = R(conv_backward_data(dc, self.w[l][0], conv_in[k].shape, 1))
-        # Encoder: dm is now the gradient of a3.
-        d = R(dm * gelu_grad(s["z3"]))
-        for l in (3, 2, 1, 0):
+            xin = s[f"a{LEVELS - 1}"] if i == LEVELS - 2 else s[f"am{i + 1}"]
+            grads[c] = conv_backward_weight(xin, dc, 1)
+            dm = R(conv_backward_data(dc, self.w[c][0], xin.shape, 1))
+        # Encoder, deepest first: dm is the gradient of a_{L-1}.
+        d = R(dm * gelu_grad(s[f"z{LEVELS - 1}"]))
+        for l in range(LEVELS - 1, -1, -1):
             x = s[f"a{l - 1}"] if l > 0 else s["x"]
-            grads[l] = conv_backward_weight(x, d, SPEC[l][2])
+            grads[l] = conv_backward_weight(x, d, 1 if l == 0 else 2)
             if l == 0:
                 break
-            dx = R(conv_backward_data(d, self.w[l][0], x.shape, SPEC[l][2]))
-            dx = R(dx + dmerge[3 - l])
+            dx = R(conv_backward_data(d, self.w[l][0], x.shape, 2))
+            dx = R(dx + dzm[l - 1])
             d = R(dx * gelu_grad(s[f"z{l - 1}"]))
         return loss, pack(grads)
 
@@ -263,7 +298,7 @@ def init_params(seed: int) -> np.ndarray:
     p = np.zeros(total, dtype=np.float32)
     for i, (w, b, shape) in enumerate(lay):
         fan_in = 9 * (3 if i == 0 else shape[3])
-        std = np.sqrt(2.0 / fan_in) if i < 7 else 0.1 * np.sqrt(1.0 / fan_in)
+        std = np.sqrt(2.0 / fan_in) if i < len(lay) - 1 else 0.1 * np.sqrt(1.0 / fan_in)
         wt = rng.normal(0.0, std, size=shape).astype(np.float32)
         if i == 0:
             wt[..., 3:] = 0.0

@@ -122,7 +122,7 @@ def _target(a):
 
 @pytest.fixture(scope="module")
 def net_case(ctx):
-    n, seeds = 32, [5, 6]
+    n, seeds = 64, [5, 6]
     a = np.stack([two_phase(n, seed=s) for s in seeds])
     u = _target(a)
     a_ptrs, u_ptrs = _device_batch(ctx, a), _device_batch(ctx, u)
@@ -136,16 +136,20 @@ def test_net_gradient_matches_oracle(ctx, net_case):
     params = _params()
     net = capi.Net(ctx, n, max_batch=4, train=True)
     net.set_params(params)
-    print("net gradient n=32 batch 2", flush=True)
+    print("net gradient n=64 batch 2", flush=True)
     loss, g = net.gradient(a_ptrs, u_ptrs)
     net.close()
     loss_e, g_e = no.Net(params, emulate=True).gradient(a, u)
     loss_x, g_x = no.Net(params).gradient(a, u)
-    assert abs(loss - loss_e) <= 1e-4 * loss_e
+    # On this case FP32 instead of FP64 accumulation in the emulated oracle
+    # alone moves the loss by 1e-4 (relative) and the per-layer gradients by
+    # up to 7e-3 (bf16 rounding flips); emulated and exact FP64 gradients
+    # differ by up to 1.3e-2.  Bounds: about 3x those.
+    assert abs(loss - loss_e) <= 5e-4 * loss_e
     lay, _ = no.layout()
     for i, (w, b, shape) in enumerate(lay):
         sl = slice(w, b + shape[0])
-        assert rel_l2(g[sl], g_e[sl]) < 5e-3, f"layer {i} vs emulated oracle"
+        assert rel_l2(g[sl], g_e[sl]) < 2e-2, f"layer {i} vs emulated oracle"
         assert rel_l2(g[sl], g_x[sl]) < 5e-2, f"layer {i} vs exact oracle"
 
 
@@ -156,7 +160,7 @@ def test_net_predict_matches_oracle(ctx, net_case):
     net.set_params(params)
     out = [ctx.alloc(n * n * 8) for _ in a_ptrs]
     f = [1.0, 2.5]
-    print("net predict n=32", flush=True)
+    print("net predict n=64", flush=True)
     net.predict_device(a_ptrs, f, out)
     got = np.stack([np.empty((n, n)) for _ in out])
     for i, p in enumerate(out):
@@ -167,10 +171,10 @@ def test_net_predict_matches_oracle(ctx, net_case):
     exact = no.Net(params).predict(a, np.array(f))
     # The random-init output is a small residue of large cancelling sums: on
     # this case FP32 instead of FP64 accumulation alone moves the emulated
-    # prediction by 9e-4 (bf16 rounding flips), and the emulated prediction
-    # differs from exact FP64 by 3.7e-3.  Indexing mistakes give O(1).
-    assert rel_l2(got, want) < 5e-3
-    assert rel_l2(got, exact) < 2e-2
+    # prediction by 5.6e-3 (bf16 rounding flips), and the emulated prediction
+    # differs from exact FP64 by 9.5e-3.  Indexing mistakes give O(1).
+    assert rel_l2(got, want) < 2e-2
+    assert rel_l2(got, exact) < 4e-2
     np.testing.assert_array_equal(net_param_roundtrip(ctx, n, params), params)
 
 
@@ -186,7 +190,7 @@ def test_adam_steps_reduce_loss(ctx, net_case):
     n, a, u, a_ptrs, u_ptrs = net_case
     net = capi.Net(ctx, n, max_batch=2, train=True)
     net.set_params(_params(5))
-    print("net train steps n=32", flush=True)
+    print("net train steps n=64", flush=True)
     losses = [net.train_step(a_ptrs, u_ptrs, 1e-3) for _ in range(30)]
     net.close()
     assert np.all(np.isfinite(losses))
@@ -194,9 +198,9 @@ def test_adam_steps_reduce_loss(ctx, net_case):
 
 
 def test_net_rejects_bad_shapes(ctx):
-    with pytest.raises(ValueError, match="multiple of 32"):
-        capi.Net(ctx, 48, 1)
-    net = capi.Net(ctx, 32, 2)
+    with pytest.raises(ValueError, match="multiple of 64"):
+        capi.Net(ctx, 96, 1)
+    net = capi.Net(ctx, 64, 2)
     with pytest.raises(ValueError, match="without training buffers"):
         net.train_step([0], [0], 1e-3)
     with pytest.raises(ValueError, match="wrong length"):

@@ -30,10 +30,12 @@ def _target(a: np.ndarray) -> np.ndarray:
 
 def test_layout_counts():
     lay, total = no.layout()
-    assert len(lay) == 8
+    assert len(lay) == 2 * no.LEVELS
     assert total == sum(int(np.prod(s)) + s[0] for _, _, s in lay)
-    assert total == 777441
-    assert lay[0][2] == (32, 3, 3, 8) and lay[7][2] == (1, 3, 3, 32)
+    assert total == 4317921
+    assert lay[0][2] == (32, 3, 3, 8) and lay[-1][2] == (1, 3, 3, 32)
+    # D_1 is 32 -> 64 at stride 2; U_0 maps 64 channels to 32
+    assert no.conv_specs()[1] == (32, 64, 2, 0) and no.conv_specs()[-1] == (64, 32, 1, 1)
 
 
 def test_bf16_rounding_matches_torch():
@@ -103,7 +105,7 @@ def _fd_problem():
     rng = np.random.default_rng(11)
     params = no.init_params(3)
     params[no.layout()[0][0][1]:no.layout()[0][0][1] + 32] = rng.normal(size=32) * 0.1  # nonzero biases
-    a = _batch(32, [5])
+    a = _batch(64, [5])
     return rng, params, a, _target(a)
 
 
@@ -131,7 +133,8 @@ def test_planted_mistake_fails_finite_differences():
 
     rng, params, a, u = _fd_problem()
     lay, _ = no.layout()
-    picks = list(range(lay[4][0], lay[4][0] + 5))  # decoder weights sit behind the upsampling
+    u0 = 2 * no.LEVELS - 2  # U_0: its weights sit behind the upsampling
+    picks = list(range(lay[u0][0], lay[u0][0] + 5))
     errs, _ = _fd_errors(Broken, params, a, u, picks)
     assert np.max(errs) > 0.1
 
@@ -140,7 +143,7 @@ def test_gradient_matches_torch_autograd():
     torch = pytest.importorskip("torch")
     F = torch.nn.functional
     rng, params, a, u = _fd_problem()
-    a = _batch(32, [5, 6])
+    a = _batch(64, [5, 6])
     u = _target(a)
     loss_np, g_np = no.Net(params).gradient(a, u)
 
@@ -155,20 +158,36 @@ def test_gradient_matches_torch_autograd():
 
     x = torch.tensor(no.features(a), dtype=torch.float64).permute(0, 3, 1, 2)
     acts = []
-    for l in range(4):
-        x = F.gelu(conv(x, l, no.SPEC[l][2]), approximate="tanh")
+    for l in range(no.LEVELS):
+        x = F.gelu(conv(x, l, 1 if l == 0 else 2), approximate="tanh")
         acts.append(x)
-    for k in range(3):
-        c = conv(x, 4 + k, 1)
-        x = F.gelu(F.interpolate(c, scale_factor=2, mode="nearest") + acts[2 - k], approximate="tanh")
-    y = conv(x, 7, 1)[:, 0]
-    pos = torch.tensor(no.position(32), dtype=torch.float64)
+    for i in range(no.LEVELS - 2, -1, -1):
+        c = conv(x, 2 * no.LEVELS - 2 - i, 1)
+        x = F.gelu(F.interpolate(c, scale_factor=2, mode="nearest") + acts[i], approximate="tanh")
+    y = conv(x, 2 * no.LEVELS - 1, 1)[:, 0]
+    pos = torch.tensor(no.position(64), dtype=torch.float64)
     t = torch.tensor(u / no.SCALE, dtype=torch.float64)
-    loss = torch.mean((pos * y - t) ** 2)
+    e = pos * y - t
+    at = torch.tensor(a, dtype=torch.float64)
+    fe = 2 * at[:, :, 1:] * at[:, :, :-1] / (at[:, :, 1:] + at[:, :, :-1])
+    fn = 2 * at[:, 1:, :] * at[:, :-1, :] / (at[:, 1:, :] + at[:, :-1, :])
+    energy = (torch.sum(fe * (e[:, :, 1:] - e[:, :, :-1]) ** 2) + torch.sum(fn * (e[:, 1:, :] - e[:, :-1, :]) ** 2)
+              + torch.sum(at[:, :, 0] * e[:, :, 0] ** 2) + torch.sum(at[:, :, -1] * e[:, :, -1] ** 2)
+              + torch.sum(at[:, 0, :] * e[:, 0, :] ** 2) + torch.sum(at[:, -1, :] * e[:, -1, :] ** 2))
+    loss = (torch.sum(e ** 2) + no.ENERGY_WEIGHT * energy) / e.numel()
     loss.backward()
     g_t = torch.cat([p.grad.reshape(-1) for p in ts]).numpy()
     assert abs(loss.item() - loss_np) <= 1e-12 * abs(loss_np)
     assert rel_l2(g_np, g_t) < 1e-10
+
+
+def test_loss_operator_is_the_solver_operator(oracle):
+    """The loss's energy term uses the finite-volume operator of the solve."""
+    rng = np.random.default_rng(21)
+    a = _batch(64, [7, 8])
+    x = rng.normal(size=a.shape)
+    for ai, xi, yi in zip(a, x, no.operator(a, x)):
+        np.testing.assert_allclose(yi, oracle.apply_operator(ai, xi), rtol=1e-12, atol=1e-12)
 
 
 def test_emulated_rounding_stays_close_to_exact():
