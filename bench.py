@@ -4,12 +4,14 @@
 Workload (the headline metric's configuration): 1000 independent 256 x 256
 two-phase media (GRF alpha=2, tau=3 thresholded to 12 / 3, held-out seeds
 20000-20999), f = 1, solved to a 1e-10 relative residual with the
-multigrid-preconditioned CG, W-cycle.  One "step" solves the whole batch in
-one persistent launch from HBM-resident inputs.
+multigrid-preconditioned CG, W-cycle.  One "step" solves the whole batch from
+HBM-resident permeabilities to HBM-resident solutions.
 
-Two GPU variants are measured on the same batch:
-  * "mixed": FP32 multigrid cycle inside an FP64 flexible CG  -> `value`
-  * "fp64":  plain FP64 MG-CG (the reference's arithmetic)      -> `fp64`
+Three GPU variants are measured on the same batch:
+  * "learned": the warm-start network predicts u0 (BF16 MFMA convolutions),
+    then the FP32-cycle / FP64-flexible-CG solve runs from it  -> `value`
+  * "mixed":   the same solve from u0 = 0                      -> `mixed`
+  * "fp64":    plain FP64 MG-CG from u0 = 0 (the reference's arithmetic) -> `fp64`
 and the reference CPU path (the reference's own solver headers built into
 oracle/_ref, OpenMP across samples) on a bounded sample of the same media
 -> `cpu_baseline`.
@@ -35,6 +37,10 @@ sys.path.insert(0, os.path.join(ROOT, "darcy-cpp_amd"))
 from darcy.fields import gaussian_random_field, two_phase_permeability  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 matrix peak
+# Network FLOPs per 256 x 256 sample: sum of 2 * pixels * 9 * cin * cout over
+# its convolutions (the first conv's 8 staged input channels counted).
+NET_FLOPS_PER_SAMPLE = 4.7563e9
 SEED0 = 20000
 
 # Algorithmic bytes per fine cell (counting rule: each array a kernel must
@@ -56,18 +62,32 @@ def algorithmic_bytes(iters: np.ndarray, n: int, precision: str, cycle: int) -> 
     return float(iters.sum()) * cells * B_ITER[(precision, cycle)] + len(iters) * cells * B_SETUP[precision]
 
 
-def run_gpu(ctx, a_ptrs, u_ptrs, n, precision, steps, warmup, cycle):
+def run_gpu(ctx, a_ptrs, u_ptrs, n, precision, steps, warmup, cycle, warm=None, u0_ptrs=None):
+    """Time `steps` batch solves; with `warm` (a WarmStart) each step first
+    predicts u0 into u0_ptrs.  Returns (result, wall ms/step, solve-kernel ms,
+    inference ms)."""
     sizes = [n] * len(a_ptrs)
+
+    def step():
+        inf = 0.0
+        if warm is not None:
+            warm.predict_device(a_ptrs, u0_ptrs)
+            inf = ctx.last_kernel_ms()
+        r = ctx.solve_batch_device(sizes, a_ptrs, u_ptrs, u0_ptrs=u0_ptrs if warm is not None else None,
+                                   precision=precision, cycle_index=cycle)
+        return r, ctx.last_kernel_ms(), inf
+
     res = None
     for _ in range(warmup):
-        res = ctx.solve_batch_device(sizes, a_ptrs, u_ptrs, precision=precision, cycle_index=cycle)
-    kernel_ms = []
+        res = step()[0]
+    kernel_ms, inf_ms = [], []
     t0 = time.perf_counter()
     for _ in range(steps):
-        res = ctx.solve_batch_device(sizes, a_ptrs, u_ptrs, precision=precision, cycle_index=cycle)
-        kernel_ms.append(ctx.last_kernel_ms())
+        res, k, i = step()
+        kernel_ms.append(k)
+        inf_ms.append(i)
     wall = time.perf_counter() - t0
-    return res, wall / steps * 1e3, float(np.mean(kernel_ms))
+    return res, wall / steps * 1e3, float(np.mean(kernel_ms)), float(np.mean(inf_ms))
 
 
 def cpu_baseline(n: int, sample: int, cycle: int) -> dict:
@@ -162,107 +182,39 @@ def suite_uneven(args) -> int:
         a_ptrs.append(ab + off * 8)
         u_ptrs.append(ub + off * 8)
         off += m.size
-    out = {}
-    for precision in ("fp64", "mixed"):
-        ctx.solve_batch_device([int(n) for n in sizes], a_ptrs, u_ptrs, precision=precision)
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            r = ctx.solve_batch_device([int(n) for n in sizes], a_ptrs, u_ptrs, precision=precision)
-        ms = (time.perf_counter() - t0) / args.steps * 1e3
-        it = r["iterations"]
-        out[precision] = {"solves_per_s": round(count / (ms * 1e-3), 1), "ms_per_step": round(ms, 2),
-                          "kernel_ms": round(ctx.last_kernel_ms(), 2), "all_converged": bool(r["converged"].all()),
-                          "iterations_by_size": {int(s): float(it[sizes == s].mean()) for s in np.unique(sizes)}}
-    ctx.free(ab)
-    ctx.free(ub)
-    ctx.close()
-    line = {"suite": "uneven", "count": count, "cells": total, "sizes": {int(s): int((sizes == s).sum())
-                                                                         for s in np.unique(sizes)}, **out}
-    if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import pyoracle
+    from darcy.learned import WarmStart, param_count
 
-        solver = pyoracle.Reference() if pyoracle.reference_available() else pyoracle.Oracle()
-        threads = max(1, min(len(os.sched_getaffinity(0)), 16))
-        t_total, solved = 0.0, 0
-        for s in np.unique(sizes):  # the reference batch needs one n per call
-            idx = np.nonzero(sizes == s)[0][: args.cpu_sample // 7 + 1]
-            batch = np.stack([media[i] for i in idx])
-            t0 = time.perf_counter()
-            solver.solve_batch(batch, threads=threads)
-            t_total += time.perf_counter() - t0
-            solved += len(idx)
-        line["cpu_baseline_sample"] = {"solves_per_s_on_sample": round(solved / t_total, 1), "cores": threads,
-                                       "problems": solved, "note": "per-size-group calls, sample of each group"}
-    print(json.dumps(line), flush=True)
-    return 0
-
-
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--suite", choices=["metric", "large", "uneven"], default="metric")
-    ap.add_argument("--sizes", type=int, nargs="+", default=[1024, 4096])
-    ap.add_argument("--oracle", action="store_true", help="large suite: compare with the CPU oracle")
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=1000)
-    ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--cycle", type=int, default=2)
-    ap.add_argument("--cpu-sample", type=int, default=320)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--copy-only", type=int, default=0,
-                    help="only run the streaming copy kernel on this many MiB (profiler calibration)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU: independent solves shard across GPUs with no communication, "
-              "so this build runs and reports N=1 only", file=sys.stderr)
-        return 2
-
-    from darcy.capi import Context
-
-    if args.suite == "large":
-        return suite_large(args)
-    if args.suite == "uneven":
-        return suite_uneven(args)
-    if args.copy_only:
-        with Context(0) as c:
-            print(json.dumps({"copy_MiB": args.copy_only, "GBs": c.bandwidth_probe(args.copy_only << 20, 5)}))
-        return 0
-
-    n, batch = args.n, args.batch
-    a = media(n, batch, SEED0)
-    ctx = Context(0)
-    copy_gbs = ctx.bandwidth_probe(1 << 30, 10)
-    cells = n * n
-    a_base = ctx.alloc(a.nbytes)
-    u_base = ctx.alloc(a.nbytes)
-    ctx.upload(a_base, a)
-    a_ptrs = [a_base + i * cells * 8 for i in range(batch)]
-    u_ptrs = [u_base + i * cells * 8 for i in range(batch)]
+    warm = WarmStart(ctx, n)  # fails loudly without the trained weights
+    u0_base = ctx.alloc(a.nbytes)
+    u0_ptrs = [u0_base + i * cells * 8 for i in range(batch)]
 
     out = {}
-    for precision in ("fp64", "mixed"):
-        res, ms_wall, ms_kernel = run_gpu(ctx, a_ptrs, u_ptrs, n, precision, args.steps, args.warmup, args.cycle)
+    for name, precision, ws in (("fp64", "fp64", None), ("mixed", "mixed", None), ("learned", "mixed", warm)):
+        res, ms_wall, ms_kernel, ms_inf = run_gpu(ctx, a_ptrs, u_ptrs, n, precision, args.steps, args.warmup,
+                                                  args.cycle, ws, u0_ptrs)
         if not res["converged"].all():
-            print(f"{precision}: {int((~res['converged']).sum())} problems did not converge", file=sys.stderr)
+            print(f"{name}: {int((~res['converged']).sum())} problems did not converge", file=sys.stderr)
             return 1
         nbytes = algorithmic_bytes(res["iterations"], n, precision, args.cycle)
-        out[precision] = {
+        out[name] = {
             "ms_per_step": ms_wall,
             "kernel_ms": ms_kernel,
+            "inference_ms": ms_inf,
             "solves_per_s": batch / (ms_wall * 1e-3),
             "mean_iterations": float(res["iterations"].mean()),
             "max_iterations": int(res["iterations"].max()),
             "alg_bytes": nbytes,
             "achieved_GBs": nbytes / (ms_kernel * 1e-3) / 1e9,
         }
+    warm.close()
+    ctx.free(u0_base)
     ctx.free(a_base)
     ctx.free(u_base)
     info = ctx.info()
     ctx.close()
 
-    best = out["mixed"]
+    best, mixed = out["learned"], out["mixed"]
+    net_flops = NET_FLOPS_PER_SAMPLE * batch
     line = {
         "metric": "solves/s to 1e-10 resid, 256² batch: learned+FP32-MG vs plain FP64 MG-CG",
         "value": round(best["solves_per_s"], 2),
@@ -274,27 +226,43 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 cycle / f64 CG",
-        "data": "synthetic: GRF two-phase media generated from seeds",
+        "dtype": "f32 cycle / f64 CG; bf16 network",
+        "data": "synthetic: GRF two-phase media generated from seeds; network trained on seeds 0-19999",
         "config": {
             "workload": f"{batch} x {n}x{n} two-phase (12/3) GRF media, seeds {SEED0}-{SEED0 + batch - 1}, f=1, "
-                        f"tol 1e-10, {'W' if args.cycle == 2 else 'V'}-cycle MG-CG, FP32 cycle in FP64 flexible CG, "
-                        f"u0 = 0 (learned warm start not in this line), one persistent launch per step",
+                        f"tol 1e-10, {'W' if args.cycle == 2 else 'V'}-cycle MG-CG; learned warm start (BF16 "
+                        f"encoder-decoder, {param_count()} parameters) then FP32 cycle in FP64 flexible CG; "
+                        f"network inference inside the timed region",
             "global_batch": batch,
             "grid": n,
             "parallelism": "single GPU",
         },
+        "mean_iterations": best["mean_iterations"],
+        "inference_ms": round(best["inference_ms"], 3),
+        "solve_ms": round(best["kernel_ms"], 3),
+        "speedup_vs_fp64": round(best["solves_per_s"] / out["fp64"]["solves_per_s"], 3),
         "fp64": {
             "value": round(out["fp64"]["solves_per_s"], 2),
             "ms_per_step": round(out["fp64"]["ms_per_step"], 3),
             "mean_iterations": out["fp64"]["mean_iterations"],
             "achieved_GBs": round(out["fp64"]["achieved_GBs"], 1),
         },
-        "speedup_vs_fp64": round(best["solves_per_s"] / out["fp64"]["solves_per_s"], 3),
-        "mean_iterations": best["mean_iterations"],
+        "mixed": {
+            "value": round(mixed["solves_per_s"], 2),
+            "ms_per_step": round(mixed["ms_per_step"], 3),
+            "mean_iterations": mixed["mean_iterations"],
+            "speedup_vs_fp64": round(mixed["solves_per_s"] / out["fp64"]["solves_per_s"], 3),
+        },
+        "network": {
+            "flops_per_sample": NET_FLOPS_PER_SAMPLE,
+            "inference_ms": round(best["inference_ms"], 3),
+            "TFLOPs": round(net_flops / (best["inference_ms"] * 1e-3) / 1e12, 1),
+            "frac_bf16_peak": round(net_flops / (best["inference_ms"] * 1e-3) / 1e12 / BF16_PEAK_TFLOPS, 4),
+            "iterations_saved": round(mixed["mean_iterations"] - best["mean_iterations"], 3),
+        },
         "roofline": {
             "bound": "hbm",
-            "kernel": "darcy_hip::k_batch<float> (persistent PCG, one workgroup per problem)",
+            "kernel": "darcy_hip::k_batch<float> (persistent PCG, one workgroup per problem), learned line",
             "achieved": round(best["achieved_GBs"], 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

@@ -265,6 +265,9 @@ PYBIND11_MODULE(_core, m) {
     m.def("set_num_threads", &set_num_threads, py::arg("threads"), "Set the host thread count.");
     m.def("device_info", &device_info, "Description of the GPU the solver runs on.");
     m.def("last_kernel_ms", &last_kernel_ms, "HIP-event time of the last solve's kernels (ms).");
+    m.def("set_tuning", &set_tuning, py::arg("key"), py::arg("value"),
+          "Set a launch-shape tuning value of the process context (results do not depend on it).");
+    m.def("get_tuning", &get_tuning, py::arg("key"), "Current launch-shape tuning value.");
     m.def("num_levels", [](index_t n) {
         int l = 1;  // multigrid.hpp:55-71 of the reference
         while (n > 1 && l < 32) {

@@ -388,6 +388,40 @@ int run_grid(darcy_ctx* ctx, const Job& job) {
         if (e != hipSuccess) rc = hip_fail(ctx, e, "cycle launch");
     };
 
+    // One preconditioner application and the rz kernel, launched eagerly or
+    // replayed from a graph captured on first use (the sequence depends only
+    // on the hierarchy, the options and the workspace address).
+    auto precondition = [&]() -> int {
+        if (!ctx->grid_graph || lT == 0) {
+            cycle(0, true);
+            if (rc) return rc;
+            HIP_TRY(darcy_grid_cg(1, &g, nullptr, nullptr, cg_blocks, s), "rz");
+            return DARCY_OK;
+        }
+        const int key[6] = {n, mixed ? 1 : 0, o.pre_sweeps, o.post_sweeps, o.cycle_index, lT};
+        const bool hit = ctx->cycle_exec && ctx->cycle_base == base && std::equal(key, key + 6, ctx->cycle_key);
+        if (!hit) {
+            if (ctx->cycle_exec) (void)hipGraphExecDestroy(ctx->cycle_exec);
+            if (ctx->cycle_graph) (void)hipGraphDestroy(ctx->cycle_graph);
+            ctx->cycle_exec = nullptr;
+            ctx->cycle_graph = nullptr;
+            HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "graph capture");
+            cycle(0, true);
+            const hipError_t e1 = rc ? hipErrorUnknown : darcy_grid_cg(1, &g, nullptr, nullptr, cg_blocks, s);
+            hipGraph_t graph = nullptr;
+            const hipError_t e2 = hipStreamEndCapture(s, &graph);
+            if (rc) return rc;
+            HIP_TRY(e1, "rz");
+            HIP_TRY(e2, "graph capture");
+            ctx->cycle_graph = graph;
+            HIP_TRY(hipGraphInstantiate(&ctx->cycle_exec, graph, nullptr, nullptr, 0), "graph instantiate");
+            ctx->cycle_base = base;
+            std::copy(key, key + 6, ctx->cycle_key);
+        }
+        HIP_TRY(hipGraphLaunch(ctx->cycle_exec, s), "graph launch");
+        return DARCY_OK;
+    };
+
     HIP_TRY(darcy_grid_cg(0, &g, nullptr, nullptr, cg_blocks, s), "init");
     if (const int r = read_state()) return r;
     unsigned long long bad_h = 0;
@@ -407,17 +441,13 @@ int run_grid(darcy_ctx* ctx, const Job& job) {
     double* pold = h.p;
     double* pnew = h.u;
     if (hs.status == kRunning) {
-        cycle(0, true);
-        if (rc) return rc;
-        HIP_TRY(darcy_grid_cg(1, &g, nullptr, nullptr, cg_blocks, s), "rz");
+        if (const int r = precondition()) return r;
         for (;;) {
             HIP_TRY(darcy_grid_cg(2, &g, pold, pnew, cg_blocks, s), "apply");
             HIP_TRY(darcy_grid_cg(3, &g, pnew, nullptr, cg_blocks, s), "update");
             if (const int r = read_state()) return r;
             if (hs.status != kRunning) break;
-            cycle(0, true);
-            if (rc) return rc;
-            HIP_TRY(darcy_grid_cg(1, &g, nullptr, nullptr, cg_blocks, s), "rz");
+            if (const int r = precondition()) return r;
             std::swap(pold, pnew);
         }
     }
@@ -489,7 +519,9 @@ extern "C" void darcy_ctx_destroy(darcy_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    for (DevBuf* b : {&c->slots, &c->probs, &c->stats, &c->order, &c->counter, &c->staging}) b->release();
+    if (c->cycle_exec) (void)hipGraphExecDestroy(c->cycle_exec);
+    if (c->cycle_graph) (void)hipGraphDestroy(c->cycle_graph);
+    for (DevBuf* b : {&c->slots, &c->probs, &c->stats, &c->order, &c->counter, &c->staging, &c->grid}) b->release();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -647,6 +679,8 @@ extern "C" int darcy_ctx_set_tuning(darcy_ctx* ctx, const char* key, int32_t val
         ctx->grid_min_n = value;
     } else if (k == "grid_tail_side" && value >= 32) {
         ctx->grid_tail_side = value;
+    } else if (k == "grid_graph" && (value == 0 || value == 1)) {
+        ctx->grid_graph = value;
     } else {
         return fail(ctx, DARCY_INVALID_INPUT, "unknown tuning key or value out of range: " + k);
     }
@@ -660,6 +694,8 @@ extern "C" int darcy_ctx_get_tuning(const darcy_ctx* ctx, const char* key, int32
         *value = ctx->grid_min_n;
     else if (k == "grid_tail_side")
         *value = ctx->grid_tail_side;
+    else if (k == "grid_graph")
+        *value = ctx->grid_graph;
     else
         return DARCY_INVALID_INPUT;
     return DARCY_OK;

@@ -175,18 +175,89 @@ struct BwdWtB {
 
 // ------------------------------------------------------------ epilogues
 
+// Epilogues take a row handle from row(m), computed once per output row, and
+// then put(row, n, acc) for each column the lane holds.
+struct RowOff {
+    size_t off;
+    bool ok;
+};
+
 // Forward: z = acc + bias (bias may be null); stores z and gelu(z) where wanted.
 struct EpiFwd {
     bf16* z;
     bf16* act;
     const float* bias;
     int M, N;
-    __device__ void operator()(int m, int n, float acc) const {
+    __device__ RowOff row(int m) const { return {static_cast<size_t>(m) * N, m < M}; }
+    __device__ void put(const RowOff& r, int n, float acc) const {
+        if (!r.ok || n >= N) return;
+        const bf16 zb = f2bf(bias ? acc + bias[n] : acc);
+        if (z) z[r.off + n] = zb;
+        if (act) act[r.off + n] = f2bf(gelu(bf2f(zb)));
+    }
+    __device__ void operator()(int m, int n, float acc) const { put(row(m), n, acc); }
+    // Two-phase form (k_conv): pre() is the value staged as bf16 z, put8()
+    // stores z and act = bf16(gelu(z)) for channels n .. n+7 of row m.
+    __device__ float pre(int n, float acc) const { return bias && n < N ? acc + bias[n] : acc; }
+    __device__ void put8(int m, int n, const v8bf& zv) const {
         if (m >= M || n >= N) return;
-        const float v = bias ? acc + bias[n] : acc;
         const size_t i = static_cast<size_t>(m) * N + n;
-        if (z) z[i] = f2bf(v);
-        if (act) act[i] = f2bf(gelu(v));
+        if (z) *reinterpret_cast<v8bf*>(z + i) = zv;
+        if (act) {
+            v8bf av;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) av[e] = f2bf(gelu(bf2f(zv[e])));
+            *reinterpret_cast<v8bf*>(act + i) = av;
+        }
+    }
+};
+
+// Decoder conv with its merge: c = bf16(acc + bias) at the coarse pixel
+// (oy, ox) is upsampled x2 (nearest) and added to the encoder activation
+// `skip`; stores z = c + skip (if wanted) and gelu(z) at the 4 fine pixels.
+// The same values as the conv followed by k_up_add_gelu, without the c tensor.
+struct EpiMerge {
+    const float* bias;
+    const bf16* skip;
+    bf16* z;
+    bf16* act;
+    int M, N, oh, ow;  // conv output (coarse) geometry
+    __device__ RowOff row(int m) const {
+        const int ox = m % ow, q = m / ow, oy = q % oh, b = q / oh;
+        const size_t W = 2 * static_cast<size_t>(ow);
+        return {((static_cast<size_t>(b) * 2 * oh + 2 * oy) * W + 2 * ox) * N, m < M};
+    }
+    __device__ float pre(int n, float acc) const { return n < N ? acc + bias[n] : acc; }
+    // c = the staged bf16 conv output of coarse row m, channels n .. n+7.
+    __device__ void put8(int m, int n, const v8bf& cv) const {
+        if (m >= M || n >= N) return;
+        const RowOff r = row(m);
+        const size_t wn = 2 * static_cast<size_t>(ow) * N;
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+            const size_t i = r.off + n + (d >> 1) * wn + (d & 1) * N;
+            const v8bf sv = *reinterpret_cast<const v8bf*>(skip + i);
+            v8bf zv, av;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                zv[e] = f2bf(bf2f(cv[e]) + bf2f(sv[e]));
+                av[e] = f2bf(gelu(bf2f(zv[e])));
+            }
+            if (z) *reinterpret_cast<v8bf*>(z + i) = zv;
+            *reinterpret_cast<v8bf*>(act + i) = av;
+        }
+    }
+    __device__ void put(const RowOff& r, int n, float acc) const {
+        if (!r.ok || n >= N) return;
+        const float vc = bf2f(f2bf(acc + bias[n]));
+        const size_t wn = 2 * static_cast<size_t>(ow) * N;
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+            const size_t i = r.off + n + (d >> 1) * wn + (d & 1) * N;
+            const bf16 zb = f2bf(vc + bf2f(skip[i]));
+            if (z) z[i] = zb;
+            act[i] = f2bf(gelu(bf2f(zb)));
+        }
     }
 };
 
@@ -252,12 +323,18 @@ __global__ __launch_bounds__(kT) void k_gemm(int kchunk, int K, GA ga, GB gb, Ep
 // Forward 3x3 conv (and stride-1 backward data, which is a forward conv of dZ
 // with the flipped, transposed weights) for a power-of-two input channel
 // count >= 8.  Rows are output pixels, columns output channels, and
-// K = (tap, ci) in steps of 32.  Each lane decodes its 4 pixel rows once;
-// per K step it only adds the tap offset (tap / 3 as (tap * 11) >> 5).  A
-// wave owns a 64 x WN tile (4 x WN/16 MFMA tiles), a workgroup WAVES_N waves
-// across the columns and 4 / WAVES_N down the rows.  Workgroups are dealt to
-// the 8 XCDs round-robin; the tile order gives each XCD a contiguous band of
-// pixel rows so the taps' overlapping input rows stay in one L2.
+// K = (tap, ci) in steps of 32.  A workgroup owns a BM x BN tile (waves
+// WAVES_M x WAVES_N, each 64 x WN = 4 x WN/16 MFMA tiles).  Each K step's A
+// (BM pixels x 32) and B (BN channels x 32) tiles go through LDS, double
+// buffered: the next step's 16-byte chunks are loaded into registers while the
+// current step's fragments are read from LDS and multiplied, then stored to
+// the other buffer; one barrier per step.  LDS holds each operand as
+// [chunk of 8 k][row] so both the stores and the fragment reads of 16
+// consecutive rows are conflict-free.  A thread owns one pixel row of the A
+// tile and decodes it once; per step it adds the tap offset
+// (tap / 3 as (tap * 11) >> 5).  Workgroups are dealt to the 8 XCDs
+// round-robin; the tile order gives each XCD a contiguous band of pixel rows,
+// so the taps' overlapping input rows stay in one L2.
 struct ConvArgs {
     const bf16* x;  // NHWC input, in.c = 1 << lg_cin channels
     const bf16* w;  // [cout][9][cin]
@@ -268,76 +345,109 @@ struct ConvArgs {
 template <int WN, int WAVES_N, class Epi>
 __global__ __launch_bounds__(kT) void k_conv(ConvArgs c, Epi epi) {
     constexpr int WAVES_M = 4 / WAVES_N, TN = WN / 16;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int BM = 64 * WAVES_M, BN = WN * WAVES_N;
+    constexpr int A_PER = BM * 4 / kT;                     // A chunks per thread per step
+    constexpr int B_PER = BN * 4 >= kT ? BN * 4 / kT : 1;  // B chunks per thread per step
+    // Staging buffers for the K loop; the epilogue reuses the same memory.
+    constexpr int kStage = (2 * 4 * BM + 2 * 4 * BN) * 16, kEpi = 4 * 64 * (WN + 8) * 2;
+    __shared__ __align__(16) char smem[kStage > kEpi ? kStage : kEpi];
+    v8bf(*lds_a)[4][BM] = reinterpret_cast<v8bf(*)[4][BM]>(smem);
+    v8bf(*lds_b)[4][BN] = reinterpret_cast<v8bf(*)[4][BN]>(smem + 2 * 4 * BM * 16);
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nb = gridDim.x;
     int t = blockIdx.x;
     if ((nb & 7) == 0) t = (t & 7) * (nb >> 3) + (t >> 3);
     const int tm = t / c.tiles_n, tn = t - tm * c.tiles_n;
-    const int m_base = tm * (64 * WAVES_M) + (wave / WAVES_N) * 64;
-    const int n_base = tn * (WN * WAVES_N) + (wave % WAVES_N) * WN;
-    const int r = lane & 15, kq = (lane >> 4) * 8;
-    const int cin = 1 << c.lg_cin, K = 9 << c.lg_cin;
+    const int m_wg = tm * BM, n_wg = tn * BN;
+    const int wm = wave / WAVES_N, wn = wave % WAVES_N;
+    const int r = lane & 15, g = lane >> 4;
+    const int cin = 1 << c.lg_cin, K = 9 << c.lg_cin, steps = (K + 31) / 32;
 
-    int iy0[4], ix0[4];
-    long long base[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int m = m_base + 16 * i + r;
+    // The pixel row this thread stages, decoded once.
+    const int row = tid % BM, kc0 = tid / BM;
+    int iy0, ix0;
+    long long base;
+    {
+        const int m = m_wg + row;
         const int ox = m % c.ow, q = m / c.ow, oy = q % c.oh, b = q / c.oh;
-        iy0[i] = m < c.M ? oy * c.stride - 1 : -1000000;
-        ix0[i] = ox * c.stride - 1;
-        base[i] = ((static_cast<long long>(b) * c.in.h + iy0[i]) * c.in.w + ix0[i]) * cin;
+        iy0 = m < c.M ? oy * c.stride - 1 : -1000000;
+        ix0 = ox * c.stride - 1;
+        base = ((static_cast<long long>(b) * c.in.h + iy0) * c.in.w + ix0) * cin;
     }
-    auto load_a = [&](int k, v8bf* a) {
-        const int tap = k >> c.lg_cin, ci = k & (cin - 1);
-        const int ky = (tap * 11) >> 5, kx = tap - 3 * ky;
-        const long long off = (static_cast<long long>(ky) * c.in.w + kx) * cin + ci;
+    v8bf ra[A_PER], rb[B_PER];
+    auto load = [&](int k0) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int iy = iy0[i] + ky, ix = ix0[i] + kx;
+        for (int s = 0; s < A_PER; ++s) {
+            const int k = k0 + 8 * (kc0 + s * (kT / BM));
+            const int tap = k >> c.lg_cin, ci = k & (cin - 1);
+            const int ky = (tap * 11) >> 5, kx = tap - 3 * ky;
+            const int iy = iy0 + ky, ix = ix0 + kx;
             v8bf v = {};
             if (tap < 9 && static_cast<unsigned>(iy) < static_cast<unsigned>(c.in.h) &&
                 static_cast<unsigned>(ix) < static_cast<unsigned>(c.in.w))
-                v = *reinterpret_cast<const v8bf*>(c.x + base[i] + off);
-            a[i] = v;
+                v = *reinterpret_cast<const v8bf*>(c.x + base + (static_cast<long long>(ky) * c.in.w + kx) * cin + ci);
+            ra[s] = v;
+        }
+#pragma unroll
+        for (int s = 0; s < B_PER; ++s) {
+            const int q = tid + s * kT, col = q % BN, kc = q / BN;
+            const int n = n_wg + col, k = k0 + 8 * kc;
+            v8bf v = {};
+            if (q < BN * 4 && n < c.cout && k < K)
+                v = *reinterpret_cast<const v8bf*>(c.w + static_cast<size_t>(n) * K + k);
+            rb[s] = v;
         }
     };
-    auto load_b = [&](int k, v8bf* bv) {
+    auto store = [&](int buf) {
 #pragma unroll
-        for (int j = 0; j < TN; ++j) {
-            const int n = n_base + 16 * j + r;
-            v8bf v = {};
-            if (n < c.cout && k < K) v = *reinterpret_cast<const v8bf*>(c.w + static_cast<size_t>(n) * K + k);
-            bv[j] = v;
+        for (int s = 0; s < A_PER; ++s) lds_a[buf][kc0 + s * (kT / BM)][row] = ra[s];
+#pragma unroll
+        for (int s = 0; s < B_PER; ++s) {
+            const int q = tid + s * kT;
+            if (q < BN * 4) lds_b[buf][q / BN][q % BN] = rb[s];
         }
     };
 
     v4f acc[4][TN] = {};
-    v8bf a[4], bw[TN];
-    load_a(kq, a);
-    load_b(kq, bw);
-    for (int k0 = 0; k0 < K; k0 += 32) {
-        v8bf na[4], nbw[TN];
-        const int kn = k0 + 32 < K ? k0 + 32 + kq : K;  // past the end: loads zero, unused
-        load_a(kn, na);
-        load_b(kn, nbw);
+    load(0);
+    store(0);
+    __syncthreads();
+    for (int step = 0; step < steps; ++step) {
+        const int cur = step & 1;
+        if (step + 1 < steps) load(32 * (step + 1));
+        v8bf fa[4], fb[TN];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) fa[i] = lds_a[cur][g][wm * 64 + 16 * i + r];
+#pragma unroll
+        for (int j = 0; j < TN; ++j) fb[j] = lds_b[cur][g][wn * WN + 16 * j + r];
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
             for (int j = 0; j < TN; ++j)
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], bw[j], acc[i][j], 0, 0, 0);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) a[i] = na[i];
-#pragma unroll
-        for (int j = 0; j < TN; ++j) bw[j] = nbw[j];
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+        if (step + 1 < steps) store(cur ^ 1);
+        __syncthreads();
     }
-    const int rq = (lane >> 4) * 4;
+    // Epilogue: each wave writes bf16(acc + bias) of its 64 x WN tile to LDS
+    // (row stride WN + 8), then stores 8 consecutive channels per lane.
+    const int m_base = m_wg + wm * 64, n_base = n_wg + wn * WN;
+    constexpr int RS = WN + 8;
+    bf16* tile = reinterpret_cast<bf16*>(smem) + wave * 64 * RS;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < TN; ++j)
+        for (int q = 0; q < 4; ++q)
 #pragma unroll
-            for (int q = 0; q < 4; ++q) epi(m_base + 16 * i + rq + q, n_base + 16 * j + r, acc[i][j][q]);
+            for (int j = 0; j < TN; ++j)
+                tile[(16 * i + 4 * g + q) * RS + 16 * j + r] = f2bf(epi.pre(n_base + 16 * j + r, acc[i][j][q]));
+    __syncthreads();
+    constexpr int C8 = WN / 8;
+#pragma unroll
+    for (int s = 0; s < 64 * C8 / 64; ++s) {
+        const int idx = s * 64 + lane, rr = idx / C8, c8 = idx % C8;
+        epi.put8(m_base + rr, n_base + 8 * c8, *reinterpret_cast<const v8bf*>(tile + rr * RS + 8 * c8));
+    }
 }
 
 template <class Epi>
@@ -396,6 +506,42 @@ __global__ __launch_bounds__(kT) void k_head_fwd(const bf16* x, Geo in, const fl
         }
         y[p] = s;
     }
+}
+
+// The head for 32 input channels: a workgroup computes a 16 x 16 pixel tile
+// from an 18 x 18 halo tile of x staged once in LDS (pixel stride padded to
+// 40 channels), instead of every thread re-reading its 9 neighbours.
+constexpr int kHeadTile = 16, kHeadC = 32, kHeadPad = 40;
+__global__ __launch_bounds__(kT) void k_head_fwd_tiled(const bf16* x, Geo in, const float* w, const float* bias,
+                                                       float* y) {
+    __shared__ __align__(16) bf16 tile[(kHeadTile + 2) * (kHeadTile + 2) * kHeadPad];
+    __shared__ float ws[9 * kHeadC + 1];
+    const int tx0 = blockIdx.x * kHeadTile, ty0 = blockIdx.y * kHeadTile, b = blockIdx.z;
+    for (int i = threadIdx.x; i < 9 * kHeadC; i += kT) ws[i] = w[i];
+    if (threadIdx.x == 0) ws[9 * kHeadC] = bias[0];
+    constexpr int side = kHeadTile + 2, chunks = side * side * (kHeadC / 8);
+    for (int q = threadIdx.x; q < chunks; q += kT) {
+        const int ch = (q & 3) * 8, p = q >> 2, py = p / side, px = p % side;
+        const int iy = ty0 + py - 1, ix = tx0 + px - 1;
+        v8bf v = {};
+        if (iy >= 0 && iy < in.h && ix >= 0 && ix < in.w) v = *reinterpret_cast<const v8bf*>(x + at(in, b, iy, ix) + ch);
+        *reinterpret_cast<v8bf*>(tile + p * kHeadPad + ch) = v;
+    }
+    __syncthreads();
+    const int ly = threadIdx.x / kHeadTile, lx = threadIdx.x % kHeadTile;
+    const int oy = ty0 + ly, ox = tx0 + lx;
+    float s = ws[9 * kHeadC];
+    for (int tap = 0; tap < 9; ++tap) {
+        const bf16* xp = tile + ((ly + tap / 3) * side + lx + tap % 3) * kHeadPad;
+        const float* wt = ws + tap * kHeadC;
+#pragma unroll
+        for (int ci = 0; ci < kHeadC; ci += 8) {
+            const v8bf v = *reinterpret_cast<const v8bf*>(xp + ci);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) s += wt[ci + e] * bf2f(v[e]);
+        }
+    }
+    if (oy < in.h && ox < in.w) y[(static_cast<size_t>(b) * in.h + oy) * in.w + ox] = s;
 }
 
 // dX[pixel][ci] = sum_tap w[tap][ci] dy[pixel - tap].
@@ -480,9 +626,8 @@ __global__ __launch_bounds__(kT) void k_up_add_gelu(const bf16* c, Geo cg, const
         v8bf zv, av;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-            const float v = bf2f(cv[e]) + bf2f(sv[e]);
-            zv[e] = f2bf(v);
-            av[e] = f2bf(gelu(v));
+            zv[e] = f2bf(bf2f(cv[e]) + bf2f(sv[e]));
+            av[e] = f2bf(gelu(bf2f(zv[e])));
         }
         if (z) *reinterpret_cast<v8bf*>(z + 8 * q) = zv;
         *reinterpret_cast<v8bf*>(act + 8 * q) = av;
@@ -674,6 +819,16 @@ hipError_t darcy_nn_conv_fwd(const void* x, int b, int h, int w, int cin, const 
     return hipGetLastError();
 }
 
+// Decoder conv (stride 1) fused with its merge: act / z (z may be null) are
+// [b][2h][2w][cout] = gelu / value of up2(conv(x) + bias) + skip.
+hipError_t darcy_nn_conv_fwd_merge(const void* x, int b, int h, int w, int cin, const void* wgt, const float* bias,
+                                   int cout, const void* skip, void* z, void* act, hipStream_t s) {
+    if (!pow2_at_least_8(cin)) return hipErrorInvalidValue;
+    EpiMerge epi{bias, static_cast<const bf16*>(skip), static_cast<bf16*>(z), static_cast<bf16*>(act), b * h * w, cout,
+                 h, w};
+    return launch_conv(static_cast<const bf16*>(x), Geo{b, h, w, cin}, 1, static_cast<const bf16*>(wgt), cout, epi, s);
+}
+
 // dx: [b][h][w][cin]; dz: [b][h/stride][w/stride][cout]; wt: WF[cin][9][cout]
 // (flipped taps, from darcy_nn_weight_t).  Stride 1 is the forward conv of
 // dz with WF.
@@ -713,6 +868,11 @@ hipError_t darcy_nn_conv_bwd_weight(const void* x, int b, int h, int w, int cin,
 
 hipError_t darcy_nn_head_fwd(const void* x, int b, int h, int w, int cin, const float* wgt, const float* bias, float* y,
                              hipStream_t s) {
+    if (cin == kHeadC) {
+        const dim3 grid((w + kHeadTile - 1) / kHeadTile, (h + kHeadTile - 1) / kHeadTile, b);
+        k_head_fwd_tiled<<<grid, kT, 0, s>>>(static_cast<const bf16*>(x), Geo{b, h, w, cin}, wgt, bias, y);
+        return hipGetLastError();
+    }
     k_head_fwd<<<ew_grid(static_cast<size_t>(b) * h * w), kT, 0, s>>>(static_cast<const bf16*>(x), Geo{b, h, w, cin}, wgt,
                                                                        bias, y);
     return hipGetLastError();

@@ -48,7 +48,16 @@ struct darcy_ctx {
     std::mutex mu;
     darcy_hip::DevBuf slots, probs, stats, order, counter, staging, grid;
     // Single multigrid solves with n > grid_min_n run on the grid path, with
-    // levels of side <= grid_tail_side in one workgroup (tuned values).
+    // levels of side <= grid_tail_side in one workgroup (tuned values);
+    // grid_graph != 0 replays the cycle from a captured HIP graph.
     int grid_min_n = 64;
     int grid_tail_side = 64;
+    int grid_graph = 1;
+    // The last captured cycle (+ the rz kernel) of the grid path and what it
+    // was captured for: the workspace address, n, precision, options and
+    // tail level.  Reused while all of them match.
+    hipGraphExec_t cycle_exec = nullptr;
+    hipGraph_t cycle_graph = nullptr;
+    const void* cycle_base = nullptr;
+    int cycle_key[6] = {0, 0, 0, 0, 0, 0};
 };

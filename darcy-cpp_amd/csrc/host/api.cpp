@@ -253,4 +253,16 @@ double last_kernel_ms() {
     return ms;
 }
 
+void set_tuning(const std::string& key, int value) {
+    darcy_ctx* ctx = context();
+    check(darcy_ctx_set_tuning(ctx, key.c_str(), value), ctx);
+}
+
+int get_tuning(const std::string& key) {
+    darcy_ctx* ctx = context();
+    int32_t v = 0;
+    if (darcy_ctx_get_tuning(ctx, key.c_str(), &v) != DARCY_OK) throw InvalidInput("unknown tuning key: " + key);
+    return v;
+}
+
 }  // namespace darcy

@@ -23,6 +23,8 @@ hipError_t darcy_nn_conv_bwd_data(const void* dz, int b, int h, int w, int cin, 
                                   void* dx, hipStream_t s);
 hipError_t darcy_nn_conv_bwd_weight(const void* x, int b, int h, int w, int cin, const void* dzt, int cout, int stride,
                                     int splits, float* part, float* dw, float* db, hipStream_t s);
+hipError_t darcy_nn_conv_fwd_merge(const void* x, int b, int h, int w, int cin, const void* wgt, const float* bias,
+                                   int cout, const void* skip, void* z, void* act, hipStream_t s);
 hipError_t darcy_nn_head_fwd(const void* x, int b, int h, int w, int cin, const float* wgt, const float* bias, float* y,
                              hipStream_t s);
 hipError_t darcy_nn_head_bwd_data(const float* dy, int b, int h, int w, int cin, const float* wgt, void* dx,
@@ -125,7 +127,6 @@ struct darcy_net {
     long step = 0;
     DevBuf params, wb, wt, grad, m, v, pos, feat, y;
     DevBuf act[kLevels], z[kLevels];               // encoder outputs (after / before GELU)
-    DevBuf cbuf[kLevels - 1];                      // decoder conv outputs U_i (level i + 1 size)
     DevBuf zm[kLevels - 1], am[kLevels - 1];       // merges at level i (before / after GELU)
     DevBuf dA[kLevels], dM[kLevels - 1], dC[kLevels - 1];
     DevBuf dy, tmp, wpart, hpart, lossp, tab_a, tab_u, tab_f;
@@ -162,7 +163,6 @@ int darcy_net::alloc() {
         if (train) NN_TRY(bf(z[l], px(l, B) * kChan[l]), "net activations");
     }
     for (int i = 0; i + 1 < kLevels; ++i) {
-        NN_TRY(bf(cbuf[i], px(i + 1, B) * kChan[i]), "net activations");
         NN_TRY(bf(am[i], px(i, B) * kChan[i]), "net activations");
         if (train) NN_TRY(bf(zm[i], px(i, B) * kChan[i]), "net activations");
     }
@@ -203,7 +203,6 @@ void darcy_net::release() {
         dA[l].release();
     }
     for (int i = 0; i + 1 < kLevels; ++i) {
-        cbuf[i].release();
         zm[i].release();
         am[i].release();
         dM[i].release();
@@ -253,10 +252,8 @@ int darcy_net::forward(int b, int64_t offset) {
     }
     for (int i = kLevels - 2; i >= 0; --i) {
         const int c = u_index(i), h = side(i + 1);
-        NN_TRY(darcy_nn_conv_fwd(x, b, h, h, kChan[i + 1], wbf(c), bias(c), kChan[i], 1, cbuf[i].ptr, nullptr, s),
-               "net decoder");
-        NN_TRY(darcy_nn_up_add_gelu(cbuf[i].ptr, b, h, h, kChan[i], act[i].ptr, train ? zm[i].ptr : nullptr, am[i].ptr,
-                                    s),
+        NN_TRY(darcy_nn_conv_fwd_merge(x, b, h, h, kChan[i + 1], wbf(c), bias(c), kChan[i], act[i].ptr,
+                                       train ? zm[i].ptr : nullptr, am[i].ptr, s),
                "net decoder");
         x = am[i].ptr;
     }

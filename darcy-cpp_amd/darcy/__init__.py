@@ -94,6 +94,27 @@ class BatchResult:
         )
 
 
+_tuned = False
+
+
+def _apply_tuning() -> None:
+    """Once per process: the autotuner's launch shapes for this device, if a
+    table was made (python -m darcy.autotune).  Results never depend on them."""
+    global _tuned
+    if _tuned:
+        return
+    from . import autotune
+
+    try:
+        arch = _core.device_info().split(",")[0]
+    except RuntimeError:  # no device: the call itself reports that (after input checks)
+        return
+    _tuned = True
+    table = autotune.load(arch)
+    for key, value in (table or {}).get("chosen", {}).items():
+        _core.set_tuning(key, int(value))
+
+
 def solve(
     a: Any,
     f: Any = 1.0,
@@ -115,6 +136,7 @@ def solve(
     from an initial guess; ``precision="mixed"`` runs the multigrid cycle in
     FP32 inside an FP64 flexible CG (same stopping rule on the FP64 residual).
     """
+    _apply_tuning()
     raw = _core.solve(
         a,
         f,
@@ -156,6 +178,7 @@ def solve_batch(
     ``threads`` is accepted for compatibility and ignored: the whole batch is
     one persistent GPU launch.
     """
+    _apply_tuning()
     u, iters, residuals, converged = _core.solve_batch(
         a,
         f,
@@ -193,6 +216,7 @@ def solve_batch_uneven(
     Returns the list of solutions and a BatchResult whose ``u`` is an object
     array of the same solutions.
     """
+    _apply_tuning()
     us, iters, residuals, converged, _levels = _core.solve_batch_uneven(
         list(a),
         float(f),

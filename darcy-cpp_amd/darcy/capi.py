@@ -96,12 +96,16 @@ def _ptr(arr: np.ndarray, ctype):
 class Context:
     """A darcy_ctx: one device, one stream, its workspace."""
 
-    def __init__(self, device: int = 0):
+    def __init__(self, device: int = 0, apply_tuning: bool = True):
         self.lib = _load()
         self.ptr = C.c_void_p()
         rc = self.lib.darcy_ctx_create(device, C.byref(self.ptr))
         if rc != OK:
             raise RuntimeError(f"darcy_ctx_create failed (code {rc}): no usable HIP device")
+        if apply_tuning:  # the autotuner's table for this device, if one was made
+            from . import autotune
+
+            autotune.apply(self)
 
     def close(self) -> None:
         if self.ptr:

@@ -122,7 +122,7 @@ def load_params(path: str = DEFAULT_WEIGHTS) -> np.ndarray:
 class WarmStart:
     """The trained network on one device context, for n x n problems."""
 
-    def __init__(self, ctx: capi.Context, n: int = 256, max_batch: int = 64, params: np.ndarray | None = None,
+    def __init__(self, ctx: capi.Context, n: int = 256, max_batch: int = 250, params: np.ndarray | None = None,
                  weights: str = DEFAULT_WEIGHTS):
         self.ctx, self.n = ctx, n
         self.net = capi.Net(ctx, n, max_batch)
@@ -216,7 +216,7 @@ def evaluate(ctx: capi.Context, params: np.ndarray, a: np.ndarray, precision: st
         ap = [base_a + 8 * cells * i for i in range(count)]
         up = [base_u + 8 * cells * i for i in range(count)]
         u0p = [base_0 + 8 * cells * i for i in range(count)]
-        ws = WarmStart(ctx, n, 64, params=params)
+        ws = WarmStart(ctx, n, params=params)
         ws.predict_device(ap, u0p)
         ws.close()
         cold = {p: ctx.solve_batch_device([n] * count, ap, up, precision=p, cycle_index=cycle_index)

@@ -113,4 +113,9 @@ void set_num_threads(int threads) noexcept;
 std::string device_info();
 double last_kernel_ms();
 
+/// Launch-shape tuning of the process-wide context (darcy_ctx_set_tuning
+/// keys; no result depends on them).
+void set_tuning(const std::string& key, int value);
+int get_tuning(const std::string& key);
+
 }  // namespace darcy

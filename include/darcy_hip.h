@@ -92,7 +92,10 @@ void darcy_ctx_destroy(darcy_ctx* ctx);
  *                     multi-workgroup grid path (default 64)
  *   "grid_tail_side"  on the grid path, levels with side <= this run in one
  *                     workgroup (>= 32, default 64)
- * Neither changes any result: only the launch structure. */
+ *   "grid_graph"      1 (default): the grid path captures its cycle in a HIP
+ *                     graph on first use and replays it each iteration;
+ *                     0: launches the cycle's kernels one by one
+ * None changes any result: only the launch structure. */
 int darcy_ctx_set_tuning(darcy_ctx* ctx, const char* key, int32_t value);
 int darcy_ctx_get_tuning(const darcy_ctx* ctx, const char* key, int32_t* value);
 

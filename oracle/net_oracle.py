@@ -20,9 +20,9 @@ Two modes share one code path:
   finite differences of its own loss; its convolution by a direct
   seven-loop sum and by the adjoint identity <conv x, dz> = <x, conv^T dz>.
 * ``emulate=True``: rounds to bf16 (round to nearest even) exactly where the
-  GPU stores bf16 -- input features, conv weights, every stored activation
-  and activation gradient -- so the GPU result differs from it only by FP32
-  accumulation order.
+  GPU stores bf16 -- input features, conv weights, every stored
+  pre-activation z and activation gelu(z), every activation gradient -- so
+  the GPU result differs from it only by FP32 accumulation order.
 """
 
 from __future__ import annotations
@@ -231,14 +231,14 @@ class Net:
         s["x"] = x
         for l in range(LEVELS):
             w, b = self.w[l]
-            v = conv_forward(x, w, b, 1 if l == 0 else 2)
-            s[f"z{l}"], s[f"a{l}"] = R(v), R(gelu(v))
+            z = R(conv_forward(x, w, b, 1 if l == 0 else 2))
+            s[f"z{l}"], s[f"a{l}"] = z, R(gelu(z))
             x = s[f"a{l}"]
         for i in range(LEVELS - 2, -1, -1):
             w, b = self.w[self.u_index(i)]
             c = R(conv_forward(x, w, b, 1))
-            v = upsample2(c) + s[f"a{i}"]
-            s[f"zm{i}"], s[f"am{i}"] = R(v), R(gelu(v))
+            z = R(upsample2(c) + s[f"a{i}"])
+            s[f"zm{i}"], s[f"am{i}"] = z, R(gelu(z))
             x = s[f"am{i}"]
         w, b = self.w[-1]
         s["y"] = conv_forward(x, w, b, 1)[..., 0]

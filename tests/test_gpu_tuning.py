@@ -1,0 +1,65 @@
+"""Launch-shape knobs: HIP-graph replay of the grid path's cycle and the
+autotuner.  No knob may change a result, so every variant is compared bit
+for bit with the eager launch."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import two_phase
+
+pytestmark = pytest.mark.gpu
+
+capi = pytest.importorskip("darcy.capi")
+autotune = pytest.importorskip("darcy.autotune")
+
+
+def _solve(ctx, a, precision):
+    n = a.shape[0]
+    pa, pu = ctx.alloc(a.nbytes), ctx.alloc(a.nbytes)
+    try:
+        ctx.upload(pa, a)
+        st = ctx.solve_device(n, pa, pu, precision=precision)
+        u = np.empty_like(a)
+        ctx.download(u, pu)
+    finally:
+        ctx.free(pa)
+        ctx.free(pu)
+    return u, st["iterations"]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_graph_replay_equals_eager_launches(precision):
+    with capi.Context(apply_tuning=False) as ctx:
+        ctx.set_tuning("grid_min_n", 64)
+        ctx.set_tuning("grid_tail_side", 64)
+        a300, a200 = two_phase(300, seed=3), two_phase(200, seed=4)
+        print(f"graph replay {precision}: eager references", flush=True)
+        ctx.set_tuning("grid_graph", 0)
+        u300, it300 = _solve(ctx, a300, precision)
+        u200, it200 = _solve(ctx, a200, precision)
+        ctx.set_tuning("grid_graph", 1)
+        # capture, reuse, recapture for another size, capture again: every
+        # replay interleaved with the eager CG kernels and host syncs
+        for a, u_ref, it_ref in ((a300, u300, it300), (a300, u300, it300), (a200, u200, it200),
+                                 (a300, u300, it300)):
+            print(f"graph replay {precision}: n={a.shape[0]}", flush=True)
+            u, it = _solve(ctx, a, precision)
+            assert it == it_ref
+            assert np.array_equal(u, u_ref)
+
+
+def test_autotuner_picks_valid_shapes_and_restores_the_context():
+    with capi.Context(apply_tuning=False) as ctx:
+        before = {k: ctx.get_tuning(k) for k in autotune.KEYS}
+        print("autotune: small run", flush=True)
+        table = autotune.tune(ctx, sizes=[128, 200], small=[64, 96], repeats=1, log=lambda s: None)
+        after = {k: ctx.get_tuning(k) for k in autotune.KEYS}
+        assert after == before
+        chosen = table["chosen"]
+        assert chosen["grid_tail_side"] in (32, 64, 128) and chosen["grid_graph"] in (0, 1)
+        assert chosen["grid_min_n"] in (1, 64, 96)
+        assert len(table["results"]["grid"]) == 6 and len(table["results"]["switch"]) == 2
+        autotune.apply(ctx, table)
+        assert {k: ctx.get_tuning(k) for k in autotune.KEYS} == chosen
