@@ -182,6 +182,85 @@ def suite_uneven(args) -> int:
         a_ptrs.append(ab + off * 8)
         u_ptrs.append(ub + off * 8)
         off += m.size
+    out = {}
+    for precision in ("fp64", "mixed"):
+        ctx.solve_batch_device([int(n) for n in sizes], a_ptrs, u_ptrs, precision=precision)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            r = ctx.solve_batch_device([int(n) for n in sizes], a_ptrs, u_ptrs, precision=precision)
+        ms = (time.perf_counter() - t0) / args.steps * 1e3
+        it = r["iterations"]
+        out[precision] = {"solves_per_s": round(count / (ms * 1e-3), 1), "ms_per_step": round(ms, 2),
+                          "kernel_ms": round(ctx.last_kernel_ms(), 2), "all_converged": bool(r["converged"].all()),
+                          "iterations_by_size": {int(s): float(it[sizes == s].mean()) for s in np.unique(sizes)}}
+    ctx.free(ab)
+    ctx.free(ub)
+    ctx.close()
+    line = {"suite": "uneven", "count": count, "cells": total, "sizes": {int(s): int((sizes == s).sum())
+                                                                         for s in np.unique(sizes)}, **out}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+
+        solver = pyoracle.Reference() if pyoracle.reference_available() else pyoracle.Oracle()
+        threads = max(1, min(len(os.sched_getaffinity(0)), 16))
+        t_total, solved = 0.0, 0
+        for s in np.unique(sizes):  # the reference batch needs one n per call
+            idx = np.nonzero(sizes == s)[0][: args.cpu_sample // 7 + 1]
+            batch = np.stack([media[i] for i in idx])
+            t0 = time.perf_counter()
+            solver.solve_batch(batch, threads=threads)
+            t_total += time.perf_counter() - t0
+            solved += len(idx)
+        line["cpu_baseline_sample"] = {"solves_per_s_on_sample": round(solved / t_total, 1), "cores": threads,
+                                       "problems": solved, "note": "per-size-group calls, sample of each group"}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--suite", choices=["metric", "large", "uneven"], default="metric")
+    ap.add_argument("--sizes", type=int, nargs="+", default=[1024, 4096])
+    ap.add_argument("--oracle", action="store_true", help="large suite: compare with the CPU oracle")
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=1000)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--cycle", type=int, default=2)
+    ap.add_argument("--cpu-sample", type=int, default=320)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-only", type=int, default=0,
+                    help="only run the streaming copy kernel on this many MiB (profiler calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: independent solves shard across GPUs with no communication, "
+              "so this build runs and reports N=1 only", file=sys.stderr)
+        return 2
+
+    from darcy.capi import Context
+
+    if args.suite == "large":
+        return suite_large(args)
+    if args.suite == "uneven":
+        return suite_uneven(args)
+    if args.copy_only:
+        with Context(0) as c:
+            print(json.dumps({"copy_MiB": args.copy_only, "GBs": c.bandwidth_probe(args.copy_only << 20, 5)}))
+        return 0
+
+    n, batch = args.n, args.batch
+    a = media(n, batch, SEED0)
+    ctx = Context(0)
+    copy_gbs = ctx.bandwidth_probe(1 << 30, 10)
+    cells = n * n
+    a_base = ctx.alloc(a.nbytes)
+    u_base = ctx.alloc(a.nbytes)
+    ctx.upload(a_base, a)
+    a_ptrs = [a_base + i * cells * 8 for i in range(batch)]
+    u_ptrs = [u_base + i * cells * 8 for i in range(batch)]
+
     from darcy.learned import WarmStart, param_count
 
     warm = WarmStart(ctx, n)  # fails loudly without the trained weights
