@@ -138,11 +138,13 @@ def suite_large(args) -> int:
     ctx = Context(0)
     for n in args.sizes:
         for medium, hi, lo in (("two_phase_12_3", 12.0, 3.0), ("contrast_1e6", 1e3, 1e-3)):
+            if args.media and medium not in args.media:
+                continue
             a = two_phase_permeability(gaussian_random_field(n, seed=0), hi=hi, lo=lo)
             ab, ub = ctx.alloc(a.nbytes), ctx.alloc(a.nbytes)
             ctx.upload(ab, a)
             ref = oracle.solve(a, max_iter=20000) if oracle is not None else None
-            for precision in ("fp64", "mixed"):
+            for precision in args.precisions:
                 ctx.solve_device(n, ab, ub, precision=precision, max_iter=20000)  # warm-up
                 times = []
                 for _ in range(max(1, args.steps)):
@@ -228,6 +230,10 @@ def main() -> int:
     ap.add_argument("--suite", choices=["metric", "large", "uneven"], default="metric")
     ap.add_argument("--sizes", type=int, nargs="+", default=[1024, 4096])
     ap.add_argument("--oracle", action="store_true", help="large suite: compare with the CPU oracle")
+    ap.add_argument("--media", nargs="+", choices=["two_phase_12_3", "contrast_1e6"], default=None,
+                    help="large suite: only these media (default both)")
+    ap.add_argument("--precisions", nargs="+", choices=["fp64", "mixed"], default=["fp64", "mixed"],
+                    help="large suite: precisions to run")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
