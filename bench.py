@@ -136,6 +136,8 @@ def suite_large(args) -> int:
 
         oracle = pyoracle.Oracle()
     ctx = Context(0)
+    if args.no_graph:
+        ctx.set_tuning("grid_graph", 0)
     for n in args.sizes:
         for medium, hi, lo in (("two_phase_12_3", 12.0, 3.0), ("contrast_1e6", 1e3, 1e-3)):
             if args.media and medium not in args.media:
@@ -234,6 +236,8 @@ def main() -> int:
                     help="large suite: only these media (default both)")
     ap.add_argument("--precisions", nargs="+", choices=["fp64", "mixed"], default=["fp64", "mixed"],
                     help="large suite: precisions to run")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="large suite: launch the cycle's kernels one by one instead of replaying a HIP graph")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
