@@ -343,7 +343,12 @@ int run_grid(darcy_ctx* ctx, const Job& job) {
     if (mixed && h.L > 1) HIP_TRY(darcy_grid_tail_matrix(hd, co, tail, s), "tail matrix");
 
     GridCg g{hd, st, a, rhs, job.f ? job.f[0] : 1.0, u0, u, history, o.tol, o.max_iter, mixed ? 1 : 0};
-    const int cg_blocks = blocks_for(ctx, chunks_of(N), 4);
+    // CG kernels: the stencil apply on up to 8 blocks per CU; the streaming
+    // rz and update kernels on at most 2 (each wave takes several rounds of
+    // two chunks; fewer blocks also means fewer arrivals on the one
+    // completion counter).
+    const int apply_blocks = blocks_for(ctx, chunks_of(N), 4);
+    const int cg_blocks = std::min(blocks_for(ctx, chunks_of(N), 8), 2 * std::max(1, ctx->cus));
     GridState hs{};
     auto read_state = [&]() -> int {
         HIP_TRY(hipMemcpyAsync(&hs, st, sizeof(GridState), hipMemcpyDeviceToHost, s), "state readback");
@@ -443,7 +448,7 @@ int run_grid(darcy_ctx* ctx, const Job& job) {
     if (hs.status == kRunning) {
         if (const int r = precondition()) return r;
         for (;;) {
-            HIP_TRY(darcy_grid_cg(2, &g, pold, pnew, cg_blocks, s), "apply");
+            HIP_TRY(darcy_grid_cg(2, &g, pold, pnew, apply_blocks, s), "apply");
             HIP_TRY(darcy_grid_cg(3, &g, pnew, nullptr, cg_blocks, s), "update");
             if (const int r = read_state()) return r;
             if (hs.status != kRunning) break;

@@ -187,6 +187,12 @@ DARCY_INLINE double zval(const Hier& h, int mixed, int k) {
     return mixed ? static_cast<double>(static_cast<const float*>(h.x[0])[k]) : h.z[k];
 }
 
+// The streaming CG kernels (kg_rz, kg_update) take two chunks per round so
+// each wave has twice the loads in flight (kg_apply, whose stencil loads need
+// more waves, takes one); the chunk sums, and so the result, do not depend on
+// it.
+constexpr int kCgChunks = 2;
+
 // <r, z> (and <z, A p> in the flexible variant); sets beta and rz.
 __global__ __launch_bounds__(kBlock) void kg_rz(GridCg g) {
     const Hier& h = *g.hd;
@@ -194,7 +200,7 @@ __global__ __launch_bounds__(kBlock) void kg_rz(GridCg g) {
     struct Rz {
         double r, z, ap;
     };
-    chunk_pass<2>(
+    chunk_pass<2, kCgChunks>(
         N, grid_wave(), grid_waves(), h.part,
         [&](int k) { return Rz{h.r[k], zval(h, g.mixed, k), g.mixed ? h.ap[k] : 0.0}; },
         [&](int, const Rz& v, double* acc) {
@@ -266,7 +272,7 @@ __global__ __launch_bounds__(kBlock) void kg_update(GridCg g, const double* p) {
     struct Upd {
         double u, p, r, ap;
     };
-    chunk_pass<1>(
+    chunk_pass<1, kCgChunks>(
         N, grid_wave(), grid_waves(), h.part, [&](int k) { return Upd{g.u[k], p[k], h.r[k], h.ap[k]}; },
         [&](int k, const Upd& v, double* acc) {
             g.u[k] = v.u + alpha * v.p;

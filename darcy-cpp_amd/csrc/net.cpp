@@ -355,8 +355,8 @@ extern "C" int darcy_net_create(darcy_ctx* ctx, int64_t n, int64_t max_batch, in
     const int64_t unit = int64_t(1) << (kLevels - 1);
     if (n < unit || n % unit != 0 || n > 4096)
         return fail(ctx, DARCY_INVALID_INPUT, "the warm-start network needs n a multiple of 64 in [64, 4096]");
-    if (max_batch < 1 || static_cast<double>(max_batch) * n * n > 6.25e7)
-        return fail(ctx, DARCY_INVALID_INPUT, "max_batch must be at least 1 and max_batch * n * n at most 6.25e7");
+    if (max_batch < 1 || static_cast<double>(max_batch) * n * n > 1.34e8)
+        return fail(ctx, DARCY_INVALID_INPUT, "max_batch must be at least 1 and max_batch * n * n at most 1.34e8");
     NN_TRY(set_device(ctx), "hipSetDevice");
     darcy_net* net = new darcy_net;
     net->ctx = ctx;
