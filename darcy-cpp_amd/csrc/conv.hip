@@ -453,8 +453,190 @@ __global__ __launch_bounds__(kT) void k_conv(ConvArgs c, Epi epi) {
     }
 }
 
+// ------------------------------------------ the halo-tiled conv (stride 1)
+// For stride-1 convs on wide images with few output channels (the decoder's
+// full- and half-resolution convs), the im2col operand dominates: each input
+// pixel is read once per tap.  Here a workgroup owns a TH x TW pixel tile
+// (TH * TW = 256, a wave per 64 pixels) and all BN output channels, and
+// stages the (TH+2) x (TW+2) input patch once per 32-channel chunk, together
+// with that chunk's weights for all 9 taps; the 9 K steps of the chunk read
+// their A fragments from the patch at the tap's offset.  LDS layout:
+// patch [chunk of 8 channels][patch pixel], weights [chunk][tap][n], so the
+// fragment reads of 16 consecutive pixels or channels are conflict-free.
+// The two-phase epilogue of k_conv follows.
+struct HaloArgs {
+    const bf16* x;  // NHWC, in.c channels (a multiple of 32)
+    const bf16* w;  // [cout][9][cin]
+    Geo in;
+    int cout, tiles_x, tiles_y;
+};
+
+// Epilogue of the halo kernels, as in k_conv: bf16(acc + bias) through LDS,
+// then 8 consecutive channels per lane; tile pixel p -> global output row m.
+template <int TW, int BN, class Epi>
+__device__ __forceinline__ void halo_epilogue(char* smem, const v4f (&acc)[4][BN / 16], const Epi& epi, const Geo& in,
+                                              int b, int y0, int x0) {
+    constexpr int RS = BN + 8, TN = BN / 16, C8 = BN / 8;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
+    bf16* tile = reinterpret_cast<bf16*>(smem) + wave * 64 * RS;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int qq = 0; qq < 4; ++qq)
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+                tile[(16 * i + 4 * g + qq) * RS + 16 * j + r] = f2bf(epi.pre(16 * j + r, acc[i][j][qq]));
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < C8; ++s) {
+        const int idx = s * 64 + lane, rr = idx / C8, c8 = idx % C8;
+        const int p = 64 * wave + rr;
+        const int m = (b * in.h + y0 + p / TW) * in.w + x0 + p % TW;
+        epi.put8(m, 8 * c8, *reinterpret_cast<const v8bf*>(tile + rr * RS + 8 * c8));
+    }
+}
+
+template <int TW, int BN, class Epi>
+__global__ __launch_bounds__(kT) void k_conv_halo(HaloArgs c, Epi epi) {
+    constexpr int TH = 256 / TW, PW = TW + 2, PP = PW * (TH + 2), TN = BN / 16;
+    constexpr int kPatch = 4 * PP * 16, kW = 4 * 9 * BN * 16, kEpi = 4 * 64 * (BN + 8) * 2;
+    constexpr int kBytes = kPatch + kW > kEpi ? kPatch + kW : kEpi;
+    __shared__ __align__(16) char smem[kBytes];
+    v8bf(*lp)[PP] = reinterpret_cast<v8bf(*)[PP]>(smem);
+    v8bf(*lw)[9][BN] = reinterpret_cast<v8bf(*)[9][BN]>(smem + kPatch);
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, r = lane & 15, g = lane >> 4;
+    const int nb = gridDim.x;
+    int t = blockIdx.x;
+    if ((nb & 7) == 0) t = (t & 7) * (nb >> 3) + (t >> 3);
+    const int tx = t % c.tiles_x, q = t / c.tiles_x, ty = q % c.tiles_y, b = q / c.tiles_y;
+    const int x0 = tx * TW, y0 = ty * TH, cin = c.in.c, K = 9 * cin;
+
+    v4f acc[4][TN] = {};
+    for (int c0 = 0; c0 < cin; c0 += 32) {
+        // Stage the patch (zero outside the image) and the chunk's weights.
+        for (int qq = tid; qq < 4 * PP; qq += kT) {
+            const int gg = qq / PP, pp = qq - gg * PP;
+            const int iy = y0 - 1 + pp / PW, ix = x0 - 1 + pp % PW;
+            v8bf v = {};
+            if (iy >= 0 && iy < c.in.h && ix >= 0 && ix < c.in.w)
+                v = *reinterpret_cast<const v8bf*>(c.x + at(c.in, b, iy, ix) + c0 + 8 * gg);
+            lp[gg][pp] = v;
+        }
+        for (int qq = tid; qq < 4 * 9 * BN; qq += kT) {
+            const int gg = qq / (9 * BN), rest = qq - gg * 9 * BN, tap = rest / BN, n = rest - tap * BN;
+            v8bf v = {};
+            if (n < c.cout) v = *reinterpret_cast<const v8bf*>(c.w + static_cast<size_t>(n) * K + tap * cin + c0 + 8 * gg);
+            lw[gg][tap][n] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) {
+            const int ky = tap / 3, kx = tap % 3;
+            v8bf fa[4], fb[TN];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int p = 64 * wave + 16 * i + r;  // pixel of the tile
+                fa[i] = lp[g][(p / TW + ky) * PW + p % TW + kx];
+            }
+#pragma unroll
+            for (int j = 0; j < TN; ++j) fb[j] = lw[g][tap][16 * j + r];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < TN; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    halo_epilogue<TW, BN>(smem, acc, epi, c.in, b, y0, x0);
+}
+
+// Cin = 8 (the network's first conv): the patch is one 16-byte chunk per
+// pixel, staged once, and the 72 K values are 3 steps of 32 in which lane
+// group g takes tap 4 s + g (nothing past tap 8) with its 8 channels, in A
+// and in B alike.
+template <int TW, int BN, class Epi>
+__global__ __launch_bounds__(kT) void k_conv_halo8(HaloArgs c, Epi epi) {
+    constexpr int TH = 256 / TW, PW = TW + 2, PP = PW * (TH + 2), TN = BN / 16;
+    constexpr int kPatch = PP * 16, kW = 9 * BN * 16, kEpi = 4 * 64 * (BN + 8) * 2;
+    constexpr int kBytes = kPatch + kW > kEpi ? kPatch + kW : kEpi;
+    __shared__ __align__(16) char smem[kBytes];
+    v8bf* lp = reinterpret_cast<v8bf*>(smem);
+    v8bf(*lw)[BN] = reinterpret_cast<v8bf(*)[BN]>(smem + kPatch);
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, r = lane & 15, g = lane >> 4;
+    const int nb = gridDim.x;
+    int t = blockIdx.x;
+    if ((nb & 7) == 0) t = (t & 7) * (nb >> 3) + (t >> 3);
+    const int tx = t % c.tiles_x, q = t / c.tiles_x, ty = q % c.tiles_y, b = q / c.tiles_y;
+    const int x0 = tx * TW, y0 = ty * TH;
+    for (int pp = tid; pp < PP; pp += kT) {
+        const int iy = y0 - 1 + pp / PW, ix = x0 - 1 + pp % PW;
+        v8bf v = {};
+        if (iy >= 0 && iy < c.in.h && ix >= 0 && ix < c.in.w) v = *reinterpret_cast<const v8bf*>(c.x + at(c.in, b, iy, ix));
+        lp[pp] = v;
+    }
+    for (int qq = tid; qq < 9 * BN; qq += kT) {
+        const int tap = qq / BN, n = qq - tap * BN;
+        v8bf v = {};
+        if (n < c.cout) v = *reinterpret_cast<const v8bf*>(c.w + static_cast<size_t>(n) * 72 + tap * 8);
+        lw[tap][n] = v;
+    }
+    __syncthreads();
+    v4f acc[4][TN] = {};
+#pragma unroll
+    for (int step = 0; step < 3; ++step) {
+        const int tap = 4 * step + g, ky = tap / 3, kx = tap % 3;
+        v8bf fa[4] = {}, fb[TN] = {};
+        if (tap < 9) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int p = 64 * wave + 16 * i + r;
+                fa[i] = lp[(p / TW + ky) * PW + p % TW + kx];
+            }
+#pragma unroll
+            for (int j = 0; j < TN; ++j) fb[j] = lw[tap][16 * j + r];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+    }
+    __syncthreads();
+    halo_epilogue<TW, BN>(smem, acc, epi, c.in, b, y0, x0);
+}
+
+// The halo-tiled kernels apply to stride-1 convs with Cin 8 or a multiple of
+// 32, Cout 32 or 64, and image sides divisible by the tile.
+template <class Epi>
+bool launch_conv_halo(const bf16* x, Geo in, const bf16* w, int cout, Epi epi, hipStream_t s, hipError_t* err) {
+    if ((in.c != 8 && in.c % 32 != 0) || (cout != 32 && cout != 64)) return false;
+    const int tw = in.w % 64 == 0 && in.h % 4 == 0 ? 64 : (in.w % 32 == 0 && in.h % 8 == 0 ? 32 : 0);
+    if (tw == 0) return false;
+    HaloArgs c{x, w, in, cout, in.w / tw, in.h / (256 / tw)};
+    const unsigned grid = static_cast<unsigned>(c.tiles_x) * c.tiles_y * in.b;
+    if (in.c == 8) {
+        if (tw == 64 && cout == 32) k_conv_halo8<64, 32, Epi><<<grid, kT, 0, s>>>(c, epi);
+        if (tw == 64 && cout == 64) k_conv_halo8<64, 64, Epi><<<grid, kT, 0, s>>>(c, epi);
+        if (tw == 32 && cout == 32) k_conv_halo8<32, 32, Epi><<<grid, kT, 0, s>>>(c, epi);
+        if (tw == 32 && cout == 64) k_conv_halo8<32, 64, Epi><<<grid, kT, 0, s>>>(c, epi);
+        *err = hipGetLastError();
+        return true;
+    }
+    if (tw == 64 && cout == 32) k_conv_halo<64, 32, Epi><<<grid, kT, 0, s>>>(c, epi);
+    if (tw == 64 && cout == 64) k_conv_halo<64, 64, Epi><<<grid, kT, 0, s>>>(c, epi);
+    if (tw == 32 && cout == 32) k_conv_halo<32, 32, Epi><<<grid, kT, 0, s>>>(c, epi);
+    if (tw == 32 && cout == 64) k_conv_halo<32, 64, Epi><<<grid, kT, 0, s>>>(c, epi);
+    *err = hipGetLastError();
+    return true;
+}
+
 template <class Epi>
 hipError_t launch_conv(const bf16* x, Geo in, int stride, const bf16* w, int cout, Epi epi, hipStream_t s) {
+    hipError_t err = hipSuccess;
+    if (stride == 1 && launch_conv_halo(x, in, w, cout, epi, s, &err)) return err;
     int lg = 0;
     while ((1 << lg) < in.c) ++lg;
     const int oh = in.h / stride, ow = in.w / stride, M = in.b * oh * ow;

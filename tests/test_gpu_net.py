@@ -57,6 +57,12 @@ SHAPES = [  # b, h, w, cin, cout, stride
     (1, 8, 8, 128, 256, 2),
     (2, 12, 20, 24, 40, 2),
     (4, 10, 6, 16, 24, 1),
+    # halo-tiled kernel (stride 1, width a multiple of 32 or 64, Cout 32 / 64)
+    (2, 8, 64, 64, 32, 1),
+    (1, 16, 32, 32, 64, 1),
+    (2, 4, 128, 128, 64, 1),
+    (2, 8, 64, 8, 32, 1),  # the 8-channel halo kernel
+    (1, 16, 32, 8, 64, 1),
 ]
 
 
