@@ -122,7 +122,7 @@ def load_params(path: str = DEFAULT_WEIGHTS) -> np.ndarray:
 class WarmStart:
     """The trained network on one device context, for n x n problems."""
 
-    def __init__(self, ctx: capi.Context, n: int = 256, max_batch: int = 250, params: np.ndarray | None = None,
+    def __init__(self, ctx: capi.Context, n: int = 256, max_batch: int = 1000, params: np.ndarray | None = None,
                  weights: str = DEFAULT_WEIGHTS):
         self.ctx, self.n = ctx, n
         self.net = capi.Net(ctx, n, max_batch)

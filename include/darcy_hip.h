@@ -180,8 +180,9 @@ typedef struct darcy_net darcy_net;
 /* Length of the parameter vector. */
 int64_t darcy_net_param_count(void);
 
-/* A network for n x n problems, processing at most max_batch at a time;
- * train != 0 also allocates what training needs.  Parameters start at 0. */
+/* A network for n x n problems, processing at most max_batch at a time
+ * (max_batch * n * n at most 1.34e8); train != 0 also allocates what
+ * training needs.  Parameters start at 0. */
 int darcy_net_create(darcy_ctx* ctx, int64_t n, int64_t max_batch, int32_t train, darcy_net** out);
 void darcy_net_destroy(darcy_net* net);
 
