@@ -43,6 +43,190 @@ struct Shared {
     int prob;
 };
 
+// ----------------------------------------------- register-resident wave levels
+// The FP64 levels of side <= 8 on one wave, one cell per lane: lane q works
+// on cell (q / n, q % n) of each such level.  A visit loads its cell's x, b,
+// faces and reciprocal diagonal from the LDS pool once, runs its phases in
+// registers, reading neighbours and children with cross-lane permutes, and
+// stores x (and the next level's b) back: no barrier and no LDS round trip
+// per phase.  Each per-cell expression has the operations and order of
+// gs_at, apply_at, restricted_residual and black_with_prolong, so the cycle
+// gives the bits of the LDS version.  All 64 lanes execute every permute;
+// results are selected afterwards.  LDS accesses of one wave complete in
+// order, so a lane reads what another lane of the wave stored before.
+
+constexpr int kRegLevels = 4;  // sides 8, 4, 2, 1 at most
+
+// Read-only description of one level for this lane.
+struct RegLevel {
+    LvP<double> A;  // the level in the LDS pool
+    int i, j;       // this lane's cell
+    bool on;        // lane < n * n
+};
+
+struct RegCell {
+    double e, w, nn, s, inv, b;
+    bool hE, hW, hN, hS;
+};
+
+DARCY_INLINE RegCell reg_cell(const RegLevel& L) {
+    const int n = L.A.n, i = L.i, j = L.j, f = i * (n + 1) + j;
+    RegCell c{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, false, false, false, false};
+    if (L.on) {
+        c.e = L.A.ex[f + 1];
+        c.w = L.A.ex[f];
+        c.nn = L.A.ny[(i + 1) * n + j];
+        c.s = L.A.ny[i * n + j];
+        c.inv = L.A.id[i * n + j];
+        c.b = L.A.b[i * n + j];
+        c.hE = j + 1 < n;
+        c.hW = j > 0;
+        c.hN = i + 1 < n;
+        c.hS = i > 0;
+    }
+    return c;
+}
+
+DARCY_INLINE double lane_value(double v, int src) { return __shfl(v, src, kWave); }
+
+struct RegNeighbours {
+    double xE, xW, xN, xS;
+};
+
+DARCY_INLINE RegNeighbours reg_neighbours(const RegLevel& L, const RegCell& c, double v) {
+    const int q = static_cast<int>(threadIdx.x & (kWave - 1)), n = L.A.n;
+    return {lane_value(v, c.hE ? q + 1 : q), lane_value(v, c.hW ? q - 1 : q), lane_value(v, c.hN ? q + n : q),
+            lane_value(v, c.hS ? q - n : q)};
+}
+
+// gs_at with the neighbour values gathered.
+DARCY_INLINE double reg_gs(const RegCell& c, const RegNeighbours& x) {
+    double v = c.b;
+    v += c.hE ? c.e * x.xE : 0.0;
+    v += c.hW ? c.w * x.xW : 0.0;
+    v += c.hN ? c.nn * x.xN : 0.0;
+    v += c.hS ? c.s * x.xS : 0.0;
+    return v * c.inv;
+}
+
+// b - A x at this lane's cell (apply_at with the neighbour values gathered).
+DARCY_INLINE double reg_residual(const RegCell& c, double xk, const RegNeighbours& x) {
+    const double d = ((c.e + c.w) + c.nn) + c.s;
+    double v = d * xk;
+    v -= c.hE ? c.e * x.xE : 0.0;
+    v -= c.hW ? c.w * x.xW : 0.0;
+    v -= c.hN ? c.nn * x.xN : 0.0;
+    v -= c.hS ? c.s * x.xS : 0.0;
+    return c.b - v;
+}
+
+DARCY_INLINE bool reg_red(const RegLevel& L) { return ((L.i + L.j) & 1) == 0; }
+
+// One colour of red-black GS on x (0: red, 1: black).
+DARCY_INLINE double reg_colour(const RegLevel& L, const RegCell& c, double x, int colour) {
+    const double v = reg_gs(c, reg_neighbours(L, c, x));
+    return L.on && reg_red(L) == (colour == 0) ? v : x;
+}
+
+// Going down through L (visit_down): pre-smoothing, the first sweep from
+// x = 0 when fresh (presmooth_from_zero), then the restricted residual into
+// the b of the next level C (residual_restrict).  A C of side 1 is the
+// coarsest level and is solved here, x = b * inv_d (coarsest_solve); its
+// repeated W-cycle visit gives the same value and so does nothing.
+DARCY_INLINE void reg_down(const RegLevel& L, const RegLevel& C, bool fresh, const CycleOpts& o) {
+    const int q = static_cast<int>(threadIdx.x & (kWave - 1));
+    const RegCell c = reg_cell(L);
+    double x = !fresh && L.on ? L.A.x[q] : 0.0;
+    for (int s = 0; s < o.pre; ++s) {
+        if (s == 0 && fresh) {
+            const double red = c.b * c.inv;
+            const double black = reg_gs(c, reg_neighbours(L, c, red));
+            x = reg_red(L) ? red : black;
+        } else {
+            x = reg_colour(L, c, x, 0);
+            x = reg_colour(L, c, x, 1);
+        }
+    }
+    const double r = reg_residual(c, x, reg_neighbours(L, c, x));
+    const int n = L.A.n;
+    double s = 0.0;
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+        const int i = 2 * C.i + (d >> 1), j = 2 * C.j + (d & 1);
+        const bool ok = C.on && i < n && j < n;
+        const double rd = lane_value(r, ok ? i * n + j : 0);
+        if (ok) s += rd;
+    }
+    if (L.on) L.A.x[q] = x;
+    if (C.on) {
+        C.A.b[q] = s;
+        if (C.A.n == 1) C.A.x[0] = s * C.A.id[0];
+    }
+    WaveSync{}();
+}
+
+// Coming up through L from C (visit_up): the black sweep reads its red
+// neighbours as x + x_c of their aggregate, then red, then further sweeps.
+DARCY_INLINE void reg_up(const RegLevel& L, const RegLevel& C, const CycleOpts& o) {
+    const int q = static_cast<int>(threadIdx.x & (kWave - 1));
+    const RegCell c = reg_cell(L);
+    double x = L.on ? L.A.x[q] : 0.0;
+    const double xc = L.on ? C.A.x[(L.i >> 1) * C.A.n + (L.j >> 1)] : 0.0;
+    const double v = reg_gs(c, reg_neighbours(L, c, x + xc));
+    if (L.on && !reg_red(L)) x = v;
+    x = reg_colour(L, c, x, 0);
+    for (int s = 1; s < o.post; ++s) {
+        x = reg_colour(L, c, x, 1);
+        x = reg_colour(L, c, x, 0);
+    }
+    if (L.on) L.A.x[q] = x;
+    WaveSync{}();
+}
+
+// Calls f(lv[k], lv[k + 1]) for a wave-uniform k < kRegLevels - 1.
+template <class F>
+DARCY_INLINE void reg_pair(const RegLevel* lv, int k, F&& f) {
+    if (k == 0)
+        f(lv[0], lv[1]);
+    else if (k == 1)
+        f(lv[1], lv[2]);
+    else
+        f(lv[2], lv[3]);
+}
+
+// The cycle over levels 0..nl-1 of lv, the last of side 1 (cycle_levels
+// with the visits above).
+DARCY_INLINE void reg_cycle_levels(const RegLevel* lv, int nl, bool fresh0, const CycleOpts& o) {
+    if (nl == 1) {  // a lone 1 x 1 level
+        if (threadIdx.x == 0) lv[0].A.x[0] = lv[0].A.b[0] * lv[0].A.id[0];
+        WaveSync{}();
+        return;
+    }
+    uint32_t done = 0;
+    int l = 0;
+    bool fresh = fresh0;
+    for (;;) {
+        while (l + 1 < nl) {
+            reg_pair(lv, l, [&](const RegLevel& L, const RegLevel& C) { reg_down(L, C, fresh, o); });
+            done &= ~(1u << l);
+            ++l;
+            fresh = true;
+        }
+        // level l is the 1 x 1 level, solved when its b was restricted
+        for (;;) {
+            if (l == 0) return;
+            --l;
+            if (o.gamma == 2 && !((done >> l) & 1u)) {
+                done |= 1u << l;
+                ++l;
+                fresh = false;
+                break;
+            }
+            reg_pair(lv, l, [&](const RegLevel& L, const RegLevel& C) { reg_up(L, C, o); });
+        }
+    }
+}
+
 // --------------------------------------------------------------- reductions
 
 __device__ inline double wave_sum(double s) {
@@ -145,9 +329,31 @@ struct Cycle {
         return {h.n[l], pool + f[0], pool + f[1], pool + f[2], pool + f[3], pool + f[4]};
     }
 
-    // --- one wave, everything in LDS (levels >= t.wave): the recursive cycle
+    // --- one wave, everything in LDS (levels >= t.wave): the recursive cycle,
+    // in FP64 on the register-resident levels when they end at side 1.
     DARCY_INLINE void wave_cycle(int l0, bool fresh0, Exec e) const {
+        if constexpr (sizeof(T) == 8) {
+            if (h.L - l0 <= kRegLevels && h.n[h.L - 1] == 1) {
+                reg_cycle(l0, fresh0);
+                return;
+            }
+        }
         cycle_levels<T>(l0, h.L, fresh0, o, e, WaveSync{}, [&](int l) { return S(l); });
+    }
+
+    // The register-resident cycle over levels l0..L-1 (all in LDS).
+    DARCY_INLINE void reg_cycle(int l0, bool fresh0) const {
+        const int q = static_cast<int>(threadIdx.x & (kWave - 1)), nl = h.L - l0;
+        RegLevel lv[kRegLevels];
+#pragma unroll
+        for (int k = 0; k < kRegLevels; ++k) {
+            const LvP<T> A = S(k < nl ? l0 + k : l0);
+            const int n = A.n;
+            const bool on = k < nl && q < n * n;
+            const int i = on ? q / n : 0;
+            lv[k] = RegLevel{A, i, on ? q - i * n : 0, on};
+        }
+        reg_cycle_levels(lv, nl, fresh0, o);
     }
 
     // --- FP32 cycle: the tail as its matrix.  Fresh: x = M b; repeated

@@ -497,10 +497,11 @@ struct LdsTable {
 
 DARCY_INLINE int align4(int v) { return (v + 3) & ~3; }
 
+// Levels from `from` on whose side is <= NL go to LDS.
 template <class T, int NL>
-DARCY_INLINE void plan_lds(const Hier& h, LdsTable& t) {
+DARCY_INLINE void plan_lds(const Hier& h, LdsTable& t, int from = 1) {
     int first = h.L;
-    for (int l = 1; l < h.L; ++l)
+    for (int l = from; l < h.L; ++l)
         if (h.n[l] <= NL) {
             first = l;
             break;

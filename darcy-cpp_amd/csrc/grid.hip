@@ -79,17 +79,66 @@ __global__ __launch_bounds__(kBlock) void kg_postsmooth(const Hier* hd, int l, i
         gs_colour(L, kind == 1 ? 0 : 1, e);  // 1: red, 2: black
 }
 
-// The cycle from level l0 (side <= T) down, in one workgroup.
+// Largest side of the levels the tail kernel keeps in LDS.  It runs alone on
+// its CU, so it holds far more than a batch workgroup: sides <= 64 in FP32
+// (111 KB with the tail matrix beside it), <= 32 in FP64 (57 KB).  The tail
+// then reads the grid only once per call -- the operators, b of its first
+// level (and x on a repeated visit) -- instead of once per phase.
+template <class T>
+struct TailLdsSide {
+    static constexpr int value = sizeof(T) == 4 ? 64 : 32;
+};
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+using gvec = __attribute__((address_space(1))) const u32x4;
+using gvec_out = __attribute__((address_space(1))) u32x4;
+
+// count elements global -> LDS (both 16-byte aligned): 16 bytes per thread
+// and step, four steps' loads issued before their stores.  Loads past the
+// end re-read the last vector (kept in registers: no per-thread arrays) and
+// are not stored.
+template <class T>
+DARCY_INLINE void stage_in(T* lds, const T* src, int count, Exec e) {
+    constexpr int per = 16 / static_cast<int>(sizeof(T));
+    const int nv = count / per, st = e.nthr;
+    gvec* g = (gvec*)src;  // generic -> global address space
+    u32x4* d = reinterpret_cast<u32x4*>(lds);
+    for (int v0 = e.tid; v0 < nv; v0 += 4 * st) {
+        const int v1 = v0 + st, v2 = v0 + 2 * st, v3 = v0 + 3 * st;
+        const u32x4 t0 = g[v0], t1 = g[v1 < nv ? v1 : nv - 1], t2 = g[v2 < nv ? v2 : nv - 1],
+                    t3 = g[v3 < nv ? v3 : nv - 1];
+        d[v0] = t0;
+        if (v1 < nv) d[v1] = t1;
+        if (v2 < nv) d[v2] = t2;
+        if (v3 < nv) d[v3] = t3;
+    }
+    for (int k = nv * per + e.tid; k < count; k += e.nthr) lds[k] = src[k];
+}
+
+// count elements LDS -> global (both 16-byte aligned).
+template <class T>
+DARCY_INLINE void stage_out(T* dst, const T* lds, int count, Exec e) {
+    constexpr int per = 16 / static_cast<int>(sizeof(T));
+    const int nv = count / per;
+    gvec_out* g = (gvec_out*)dst;
+    const u32x4* s = reinterpret_cast<const u32x4*>(lds);
+    for (int v = e.tid; v < nv; v += e.nthr) g[v] = s[v];
+    for (int k = nv * per + e.tid; k < count; k += e.nthr) dst[k] = lds[k];
+}
+
+// The cycle from level l0 (side <= T) down, in one workgroup.  When l0 is an
+// LDS level its b (and x unless fresh) is staged in and its x staged out.
 template <class T>
 __global__ __launch_bounds__(kBlock) void kg_tail(const Hier* hd, int l0, int fresh, CycleOpts o,
                                                   const float* tail_in) {
+    constexpr int NL = TailLdsSide<T>::value;
     __shared__ Shared s;
-    __shared__ __align__(16) T pool[pool_elems<T>()];
+    __shared__ __align__(16) T pool[LdsPool<T, NL>::kElems];
     __shared__ __align__(16) float tail[sizeof(T) == 4 ? kTailLds : 1];
     const Exec e{static_cast<int>(threadIdx.x), kBlock};
     if (threadIdx.x == 0) {
         s.h = *hd;
-        plan_lds<T, LdsSide<T>::value>(s.h, s.t);
+        plan_lds<T, NL>(s.h, s.t, l0);
     }
     __syncthreads();
     const Hier& h = s.h;
@@ -97,21 +146,27 @@ __global__ __launch_bounds__(kBlock) void kg_tail(const Hier* hd, int l0, int fr
         const LvP<T> g = global_level<T>(h, l);
         const int* f = s.t.off[l];
         const int nf = g.n * (g.n + 1), nc = g.n * g.n;
-        for (int q = e.tid; q < nf; q += e.nthr) {
-            pool[f[0] + q] = g.ex[q];
-            pool[f[1] + q] = g.ny[q];
+        stage_in(pool + f[0], g.ex, nf, e);
+        stage_in(pool + f[1], g.ny, nf, e);
+        stage_in(pool + f[2], g.id, nc, e);
+        if (l == l0) {
+            stage_in(pool + f[4], static_cast<const T*>(g.b), nc, e);
+            if (!fresh) stage_in(pool + f[3], static_cast<const T*>(g.x), nc, e);
         }
-        for (int q = e.tid; q < nc; q += e.nthr) pool[f[2] + q] = g.id[q];
     }
     if constexpr (sizeof(T) == 4) {
         if (s.t.wave < h.L) {
             const int dim = h.n[s.t.wave] * h.n[s.t.wave];
-            for (int q = e.tid; q < dim * dim; q += e.nthr) tail[q] = tail_in[q];
+            stage_in(tail, tail_in, dim * dim, e);
         }
     }
     __syncthreads();
     const Cycle<T> cyc{h, s.t, pool, tail, o};
     cyc.run(l0, fresh != 0, e);
+    if (s.t.first <= l0) {  // the cycle ends with a barrier
+        const LvP<T> g = global_level<T>(h, l0);
+        stage_out(g.x, pool + s.t.off[l0][3], g.n * g.n, e);
+    }
 }
 
 // ------------------------------------------------------------ CG kernels
