@@ -65,7 +65,6 @@ struct BatchArgs {
 struct GridState {
     double bnorm, rz, alpha, beta, res, pap;
     int it, status, first, pad;
-    unsigned int arrive;
 };
 
 struct GridCg {

@@ -2,8 +2,9 @@
 // larger than the tail side T is one launch over the whole GPU; levels of
 // side <= T run inside one workgroup (the block cycle of block.hpp, with the
 // LDS levels and the dense FP32 tail); the CG vector operations are flat
-// launches with the canonical chunked reductions, finished by the last
-// workgroup to arrive, which also advances the device-resident CG scalars.
+// launches storing the chunk sums of the canonical reductions, each followed
+// by a one-workgroup kernel that adds them and advances the device-resident
+// CG scalars.
 //
 // The per-cell arithmetic, the cycle and the reduction order are those of
 // the persistent batch kernel, so a solve on either path gives the same
@@ -170,26 +171,106 @@ __global__ __launch_bounds__(kBlock) void kg_tail(const Hier* hd, int l0, int fr
 }
 
 // ------------------------------------------------------------ CG kernels
+// Each streaming kernel stores its chunk sums; kg_final, one workgroup
+// launched after it on the same stream, adds them in the canonical order and
+// advances the device-resident CG scalars.
 
-// Called by every workgroup after its chunk sums are stored: true in the
-// last workgroup to arrive, which then sees every chunk sum.
-DARCY_INLINE bool last_to_arrive(GridState* st) {
-    __shared__ bool last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) last = atomicAdd(&st->arrive, 1u) == gridDim.x - 1;
-    __syncthreads();
-    if (last) __threadfence();
-    return last;
+constexpr int kFinalThreads = 1024;
+constexpr int kFinalPer = 16;                           // chunk sums per thread and slice
+constexpr int kFinalSlice = kFinalThreads * kFinalPer;  // 16384 doubles: 128 KB of LDS
+
+// The value of final_sum (lane l adds chunk sums l, l+64, ... in order, then
+// the butterfly) with the chunk sums staged through LDS by the whole
+// workgroup a slice at a time, so that a slice's loads are in flight
+// together instead of one wave's sixteen; wave 0 adds.  Valid in wave 0;
+// every thread must call it.
+__device__ double staged_sum(const double* part, int C, double* buf) {
+    const int lane = threadIdx.x & (kWave - 1);
+    double s = 0.0;
+    for (int base = 0; base < C; base += kFinalSlice) {
+        const int cnt = C - base < kFinalSlice ? C - base : kFinalSlice;
+        double v[kFinalPer];
+#pragma unroll
+        for (int u = 0; u < kFinalPer; ++u) {
+            const int q = static_cast<int>(threadIdx.x) + u * kFinalThreads;
+            v[u] = part[base + (q < cnt ? q : cnt - 1)];
+        }
+#pragma unroll
+        for (int u = 0; u < kFinalPer; ++u) {
+            const int q = static_cast<int>(threadIdx.x) + u * kFinalThreads;
+            if (q < cnt) buf[q] = v[u];
+        }
+        __syncthreads();
+        if (threadIdx.x < kWave) {  // slices hold whole rounds of 64: lane l owns q = l mod 64
+            int q = lane;
+            for (; q + 7 * kWave < cnt; q += 8 * kWave) {
+                double t[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) t[u] = buf[q + u * kWave];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) s += t[u];
+            }
+            for (; q < cnt; q += kWave) s += buf[q];
+        }
+        __syncthreads();
+    }
+    return wave_sum(s);
+}
+
+// The scalar step after CG kernel KIND: 0 init (<b,b>, <r,r>; solver.hpp:
+// 92-108), 1 rz (<r,z> and, flexible, <z,Ap>; sets beta and rz), 2 apply
+// (<p,Ap>; alpha or breakdown, solver.hpp:147-155), 3 update (<r,r>;
+// residual, history, convergence, solver.hpp:156-166).
+template <int KIND>
+__global__ __launch_bounds__(kFinalThreads) void kg_final(GridCg g) {
+    __shared__ double buf[kFinalSlice];
+    const Hier& h = *g.hd;
+    const int C = chunks_of(h.n[0] * h.n[0]);
+    const double s0 = staged_sum(h.part[0], C, buf);
+    const bool two = KIND == 0 || (KIND == 1 && g.mixed);
+    const double s1 = two ? staged_sum(h.part[1], C, buf) : 0.0;
+    if (threadIdx.x != 0) return;
+    GridState& s = *g.st;
+    if constexpr (KIND == 0) {
+        s.bnorm = sqrt(s0);
+        s.res = g.u0 ? sqrt(s1) / s.bnorm : 1.0;
+        s.it = 0;
+        s.first = 1;
+        s.beta = 0.0;
+        s.status = kRunning;
+        if (s.bnorm == 0.0)
+            s.status = kConverged;
+        else if (g.u0 && s.res < g.tol)
+            s.status = kConverged;
+        if (g.history) g.history[0] = s.res;
+    } else if constexpr (KIND == 1) {
+        if (!s.first) s.beta = g.mixed ? (-s.alpha * s1) / s.rz : s0 / s.rz;
+        s.rz = s0;
+    } else if constexpr (KIND == 2) {
+        s.pap = s0;
+        if (!(fabs(s0) > 0.0) || !isfinite(s0))
+            s.status = kBreakdown;
+        else
+            s.alpha = s.rz / s0;
+        s.first = 0;
+    } else {
+        s.res = sqrt(s0) / s.bnorm;
+        s.it += 1;
+        if (g.history) g.history[s.it] = s.res;
+        if (s.res < g.tol)
+            s.status = kConverged;
+        else if (s.it >= g.max_iter)
+            s.status = kMaxIter;
+    }
 }
 
 DARCY_INLINE int grid_wave() { return static_cast<int>(blockIdx.x * kWaves + threadIdx.x / kWave); }
 DARCY_INLINE int grid_waves() { return static_cast<int>(gridDim.x * kWaves); }
 
-// r = b - A u0 or b; u = u0 or 0; <b, b> and <r, r> (solver.hpp:92-108).
+// r = b - A u0 or b; u = u0 or 0; chunk sums of <b, b> and <r, r>.
 __global__ __launch_bounds__(kBlock) void kg_init(GridCg g) {
     const Hier& h = *g.hd;
-    const int n = h.n[0], N = n * n, C = chunks_of(N);
+    const int n = h.n[0], N = n * n;
     const double hh = 1.0 / static_cast<double>(n + 1), h2 = hh * hh;
     const LvP<double> A0 = global_level<double>(h, 0);
     float* b0f = g.mixed ? static_cast<float*>(h.b[0]) : nullptr;
@@ -217,25 +298,6 @@ __global__ __launch_bounds__(kBlock) void kg_init(GridCg g) {
             acc[1] += v.rk * v.rk;
             if (b0f) b0f[k] = static_cast<float>(v.rk);
         });
-    if (!last_to_arrive(g.st)) return;
-    if (threadIdx.x < kWave) {
-        const double bb = final_sum(h.part[0], C), rr = final_sum(h.part[1], C);
-        if (threadIdx.x == 0) {
-            GridState& s = *g.st;
-            s.bnorm = sqrt(bb);
-            s.res = g.u0 ? sqrt(rr) / s.bnorm : 1.0;
-            s.it = 0;
-            s.first = 1;
-            s.beta = 0.0;
-            s.status = kRunning;
-            if (s.bnorm == 0.0)
-                s.status = kConverged;
-            else if (g.u0 && s.res < g.tol)
-                s.status = kConverged;
-            if (g.history) g.history[0] = s.res;
-            s.arrive = 0;
-        }
-    }
 }
 
 DARCY_INLINE double zval(const Hier& h, int mixed, int k) {
@@ -248,10 +310,10 @@ DARCY_INLINE double zval(const Hier& h, int mixed, int k) {
 // it.
 constexpr int kCgChunks = 2;
 
-// <r, z> (and <z, A p> in the flexible variant); sets beta and rz.
+// Chunk sums of <r, z> (and <z, A p> in the flexible variant).
 __global__ __launch_bounds__(kBlock) void kg_rz(GridCg g) {
     const Hier& h = *g.hd;
-    const int N = h.n[0] * h.n[0], C = chunks_of(N);
+    const int N = h.n[0] * h.n[0];
     struct Rz {
         double r, z, ap;
     };
@@ -262,23 +324,12 @@ __global__ __launch_bounds__(kBlock) void kg_rz(GridCg g) {
             acc[0] += v.r * v.z;
             if (g.mixed) acc[1] += v.z * v.ap;
         });
-    if (!last_to_arrive(g.st)) return;
-    if (threadIdx.x < kWave) {
-        const double rz = final_sum(h.part[0], C);
-        const double zap = g.mixed ? final_sum(h.part[1], C) : 0.0;
-        if (threadIdx.x == 0) {
-            GridState& s = *g.st;
-            if (!s.first) s.beta = g.mixed ? (-s.alpha * zap) / s.rz : rz / s.rz;
-            s.rz = rz;
-            s.arrive = 0;
-        }
-    }
 }
 
-// p = z + beta p (first: p = z), ap = A p, <p, ap>; sets alpha or breakdown.
+// p = z + beta p (first: p = z), ap = A p, chunk sums of <p, ap>.
 __global__ __launch_bounds__(kBlock) void kg_apply(GridCg g, const double* pold, double* pnew) {
     const Hier& h = *g.hd;
-    const int n = h.n[0], N = n * n, C = chunks_of(N);
+    const int n = h.n[0], N = n * n;
     const LvP<double> A0 = global_level<double>(h, 0);
     const bool first = g.st->first != 0;
     const double beta = g.st->beta;
@@ -302,26 +353,12 @@ __global__ __launch_bounds__(kBlock) void kg_apply(GridCg g, const double* pold,
             h.ap[k] = v.ap;
             acc[0] += v.p * v.ap;
         });
-    if (!last_to_arrive(g.st)) return;
-    if (threadIdx.x < kWave) {
-        const double pap = final_sum(h.part[0], C);
-        if (threadIdx.x == 0) {
-            GridState& s = *g.st;
-            s.pap = pap;
-            if (!(fabs(pap) > 0.0) || !isfinite(pap))  // solver.hpp:149-155
-                s.status = kBreakdown;
-            else
-                s.alpha = s.rz / pap;
-            s.first = 0;
-            s.arrive = 0;
-        }
-    }
 }
 
-// u += alpha p, r -= alpha ap, <r, r>; advances the iteration count.
+// u += alpha p, r -= alpha ap, chunk sums of <r, r>.
 __global__ __launch_bounds__(kBlock) void kg_update(GridCg g, const double* p) {
     const Hier& h = *g.hd;
-    const int N = h.n[0] * h.n[0], C = chunks_of(N);
+    const int N = h.n[0] * h.n[0];
     const double alpha = g.st->alpha;
     float* b0f = g.mixed ? static_cast<float*>(h.b[0]) : nullptr;
     struct Upd {
@@ -336,21 +373,6 @@ __global__ __launch_bounds__(kBlock) void kg_update(GridCg g, const double* p) {
             acc[0] += ri * ri;
             if (b0f) b0f[k] = static_cast<float>(ri);
         });
-    if (!last_to_arrive(g.st)) return;
-    if (threadIdx.x < kWave) {
-        const double rr = final_sum(h.part[0], C);
-        if (threadIdx.x == 0) {
-            GridState& s = *g.st;
-            s.res = sqrt(rr) / s.bnorm;
-            s.it += 1;
-            if (g.history) g.history[s.it] = s.res;
-            if (s.res < g.tol)
-                s.status = kConverged;
-            else if (s.it >= g.max_iter)
-                s.status = kMaxIter;
-            s.arrive = 0;
-        }
-    }
 }
 
 template __global__ void kg_presmooth<double>(const Hier*, int, int);
@@ -421,15 +443,19 @@ extern "C" hipError_t darcy_grid_cg(int which, const GridCg* g, const double* po
     switch (which) {
         case 0:
             hipLaunchKernelGGL(kg_init, dim3(blocks), dim3(kBlock), 0, s, *g);
+            hipLaunchKernelGGL(kg_final<0>, dim3(1), dim3(kFinalThreads), 0, s, *g);
             break;
         case 1:
             hipLaunchKernelGGL(kg_rz, dim3(blocks), dim3(kBlock), 0, s, *g);
+            hipLaunchKernelGGL(kg_final<1>, dim3(1), dim3(kFinalThreads), 0, s, *g);
             break;
         case 2:
             hipLaunchKernelGGL(kg_apply, dim3(blocks), dim3(kBlock), 0, s, *g, pold, pnew);
+            hipLaunchKernelGGL(kg_final<2>, dim3(1), dim3(kFinalThreads), 0, s, *g);
             break;
         default:
             hipLaunchKernelGGL(kg_update, dim3(blocks), dim3(kBlock), 0, s, *g, pold);
+            hipLaunchKernelGGL(kg_final<3>, dim3(1), dim3(kFinalThreads), 0, s, *g);
             break;
     }
     return hipGetLastError();

@@ -63,3 +63,38 @@ def test_autotuner_picks_valid_shapes_and_restores_the_context():
         assert len(table["results"]["grid"]) == 6 and len(table["results"]["switch"]) == 2
         autotune.apply(ctx, table)
         assert {k: ctx.get_tuning(k) for k in autotune.KEYS} == chosen
+
+
+def _solve_persistent(ctx, a, precision):
+    n = a.shape[0]
+    pa, pu = ctx.alloc(a.nbytes), ctx.alloc(a.nbytes)
+    try:
+        ctx.upload(pa, a)
+        st = ctx.solve_batch_device([n], [pa], [pu], precision=precision)
+        u = np.empty_like(a)
+        ctx.download(u, pu)
+    finally:
+        ctx.free(pa)
+        ctx.free(pu)
+    return u, int(st["iterations"][0])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_tail_side_equals_persistent_kernel(precision):
+    """n = 300 has sides 300, 150, 75, 38, 19, 10, 5, 3, 2, 1.  Tail sides
+    32, 64 and 128 start the single-workgroup tail at 19, 38 and 75: its
+    first level then sits in LDS (staged in and out, with odd lengths) or
+    in global memory, depending on the precision's LDS limit (64 in FP32,
+    32 in FP64).  Every variant gives the persistent kernel's bits."""
+    a = two_phase(300, seed=11)
+    with capi.Context(apply_tuning=False) as ctx:
+        ctx.set_tuning("grid_min_n", 64)
+        u_ref, it_ref = _solve_persistent(ctx, a, precision)
+        for side in (32, 64, 128):
+            for graph in (0, 1):
+                print(f"tail side {side} graph {graph} {precision}", flush=True)
+                ctx.set_tuning("grid_tail_side", side)
+                ctx.set_tuning("grid_graph", graph)
+                u, it = _solve(ctx, a, precision)
+                assert it == it_ref
+                assert np.array_equal(u, u_ref)
