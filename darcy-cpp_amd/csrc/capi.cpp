@@ -344,11 +344,10 @@ int run_grid(darcy_ctx* ctx, const Job& job) {
 
     GridCg g{hd, st, a, rhs, job.f ? job.f[0] : 1.0, u0, u, history, o.tol, o.max_iter, mixed ? 1 : 0};
     // CG kernels: the stencil apply on up to 8 blocks per CU; the streaming
-    // rz and update kernels on at most 2 (each wave takes several rounds of
-    // two chunks; fewer blocks also means fewer arrivals on the one
-    // completion counter).
+    // rz and update kernels on grid_cg_per_cu (each wave takes several rounds
+    // of two chunks).
     const int apply_blocks = blocks_for(ctx, chunks_of(N), 4);
-    const int cg_blocks = std::min(blocks_for(ctx, chunks_of(N), 8), 2 * std::max(1, ctx->cus));
+    const int cg_blocks = std::min(blocks_for(ctx, chunks_of(N), 8), ctx->grid_cg_per_cu * std::max(1, ctx->cus));
     GridState hs{};
     auto read_state = [&]() -> int {
         HIP_TRY(hipMemcpyAsync(&hs, st, sizeof(GridState), hipMemcpyDeviceToHost, s), "state readback");
@@ -403,8 +402,8 @@ int run_grid(darcy_ctx* ctx, const Job& job) {
             HIP_TRY(darcy_grid_cg(1, &g, nullptr, nullptr, cg_blocks, s), "rz");
             return DARCY_OK;
         }
-        const int key[6] = {n, mixed ? 1 : 0, o.pre_sweeps, o.post_sweeps, o.cycle_index, lT};
-        const bool hit = ctx->cycle_exec && ctx->cycle_base == base && std::equal(key, key + 6, ctx->cycle_key);
+        const int key[7] = {n, mixed ? 1 : 0, o.pre_sweeps, o.post_sweeps, o.cycle_index, lT, cg_blocks};
+        const bool hit = ctx->cycle_exec && ctx->cycle_base == base && std::equal(key, key + 7, ctx->cycle_key);
         if (!hit) {
             if (ctx->cycle_exec) (void)hipGraphExecDestroy(ctx->cycle_exec);
             if (ctx->cycle_graph) (void)hipGraphDestroy(ctx->cycle_graph);
@@ -421,7 +420,7 @@ int run_grid(darcy_ctx* ctx, const Job& job) {
             ctx->cycle_graph = graph;
             HIP_TRY(hipGraphInstantiate(&ctx->cycle_exec, graph, nullptr, nullptr, 0), "graph instantiate");
             ctx->cycle_base = base;
-            std::copy(key, key + 6, ctx->cycle_key);
+            std::copy(key, key + 7, ctx->cycle_key);
         }
         HIP_TRY(hipGraphLaunch(ctx->cycle_exec, s), "graph launch");
         return DARCY_OK;
@@ -686,6 +685,8 @@ extern "C" int darcy_ctx_set_tuning(darcy_ctx* ctx, const char* key, int32_t val
         ctx->grid_tail_side = value;
     } else if (k == "grid_graph" && (value == 0 || value == 1)) {
         ctx->grid_graph = value;
+    } else if (k == "grid_cg_per_cu" && value >= 1 && value <= 8) {
+        ctx->grid_cg_per_cu = value;
     } else {
         return fail(ctx, DARCY_INVALID_INPUT, "unknown tuning key or value out of range: " + k);
     }
@@ -701,6 +702,8 @@ extern "C" int darcy_ctx_get_tuning(const darcy_ctx* ctx, const char* key, int32
         *value = ctx->grid_tail_side;
     else if (k == "grid_graph")
         *value = ctx->grid_graph;
+    else if (k == "grid_cg_per_cu")
+        *value = ctx->grid_cg_per_cu;
     else
         return DARCY_INVALID_INPUT;
     return DARCY_OK;

@@ -53,11 +53,13 @@ struct darcy_ctx {
     int grid_min_n = 64;
     int grid_tail_side = 64;
     int grid_graph = 1;
+    // Workgroups per CU of the streaming CG kernels (rz, update) on the grid path.
+    int grid_cg_per_cu = 4;
     // The last captured cycle (+ the rz kernel) of the grid path and what it
     // was captured for: the workspace address, n, precision, options and
     // tail level.  Reused while all of them match.
     hipGraphExec_t cycle_exec = nullptr;
     hipGraph_t cycle_graph = nullptr;
     const void* cycle_base = nullptr;
-    int cycle_key[6] = {0, 0, 0, 0, 0, 0};
+    int cycle_key[7] = {0, 0, 0, 0, 0, 0, 0};
 };

@@ -95,6 +95,8 @@ void darcy_ctx_destroy(darcy_ctx* ctx);
  *   "grid_graph"      1 (default): the grid path captures its cycle in a HIP
  *                     graph on first use and replays it each iteration;
  *                     0: launches the cycle's kernels one by one
+ *   "grid_cg_per_cu"  workgroups per CU of the grid path's streaming CG
+ *                     kernels (1..8, default 4; not autotuned)
  * None changes any result: only the launch structure. */
 int darcy_ctx_set_tuning(darcy_ctx* ctx, const char* key, int32_t value);
 int darcy_ctx_get_tuning(const darcy_ctx* ctx, const char* key, int32_t* value);

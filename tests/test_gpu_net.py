@@ -63,6 +63,10 @@ SHAPES = [  # b, h, w, cin, cout, stride
     (2, 4, 128, 128, 64, 1),
     (2, 8, 64, 8, 32, 1),  # the 8-channel halo kernel
     (1, 16, 32, 8, 64, 1),
+    # stride 2 at the encoder's channel counts: several K chunks and N tiles
+    (1, 16, 32, 32, 64, 2),
+    (2, 32, 64, 64, 128, 2),
+    (1, 16, 32, 128, 256, 2),
 ]
 
 
