@@ -130,9 +130,7 @@ DARCY_INLINE double reg_colour(const RegLevel& L, const RegCell& c, double x, in
 
 // Going down through L (visit_down): pre-smoothing, the first sweep from
 // x = 0 when fresh (presmooth_from_zero), then the restricted residual into
-// the b of the next level C (residual_restrict).  A C of side 1 is the
-// coarsest level and is solved here, x = b * inv_d (coarsest_solve); its
-// repeated W-cycle visit gives the same value and so does nothing.
+// the b of the next level C (residual_restrict).
 DARCY_INLINE void reg_down(const RegLevel& L, const RegLevel& C, bool fresh, const CycleOpts& o) {
     const int q = static_cast<int>(threadIdx.x & (kWave - 1));
     const RegCell c = reg_cell(L);
@@ -158,10 +156,7 @@ DARCY_INLINE void reg_down(const RegLevel& L, const RegLevel& C, bool fresh, con
         if (ok) s += rd;
     }
     if (L.on) L.A.x[q] = x;
-    if (C.on) {
-        C.A.b[q] = s;
-        if (C.A.n == 1) C.A.x[0] = s * C.A.id[0];
-    }
+    if (C.on) C.A.b[q] = s;
     WaveSync{}();
 }
 
@@ -183,6 +178,72 @@ DARCY_INLINE void reg_up(const RegLevel& L, const RegLevel& C, const CycleOpts& 
     WaveSync{}();
 }
 
+// Lane exchange within a quad (DPP quad_perm, no LDS): CTRL packs the
+// source lane of each quad lane in 2 bits.
+template <int CTRL>
+DARCY_INLINE double quad_value(double v) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, true);
+    return __hiloint2double(hi, lo);
+}
+constexpr int kQuadSwap1 = 0xB1;  // lanes (1, 0, 3, 2): east / west in a 2 x 2 level
+constexpr int kQuadSwap2 = 0x4E;  // lanes (2, 3, 0, 1): north / south
+
+// A visit of the 2 x 2 level L together with the 1 x 1 level C below it, on
+// lane quads: quad lane q holds cell (q / 2, q % 2) and reads its neighbours
+// (q ^ 1 east / west, q ^ 2 north / south) by quad permutes; the restricted
+// residual gathers the four residuals by quad broadcasts and adds them in
+// restricted_residual's order.  The phases of reg_down, the coarsest solve
+// and reg_up in their order; every quad computes the same values and lanes
+// 0..3 store x.
+DARCY_INLINE void reg_visit2(const RegLevel& L, const RegLevel& C, bool fresh, const CycleOpts& o) {
+    const int lane = static_cast<int>(threadIdx.x & (kWave - 1)), q = lane & 3, i = q >> 1, j = q & 1;
+    const int f = 3 * i + j;
+    const RegCell c{L.A.ex[f + 1], L.A.ex[f], L.A.ny[2 * (i + 1) + j], L.A.ny[2 * i + j], L.A.id[q], L.A.b[q],
+                    j == 0, j == 1, i == 0, i == 1};
+    const bool red = q == 0 || q == 3;
+    auto nb = [](double v) {
+        const double ew = quad_value<kQuadSwap1>(v), ns = quad_value<kQuadSwap2>(v);
+        return RegNeighbours{ew, ew, ns, ns};
+    };
+    double x = fresh ? 0.0 : L.A.x[q];
+    for (int s = 0; s < o.pre; ++s) {
+        if (s == 0 && fresh) {
+            const double r = c.b * c.inv;
+            const double v = reg_gs(c, nb(r));
+            x = red ? r : v;
+        } else {
+            const double vr = reg_gs(c, nb(x));
+            x = red ? vr : x;
+            const double vb = reg_gs(c, nb(x));
+            x = red ? x : vb;
+        }
+    }
+    const double r = reg_residual(c, x, nb(x));
+    double sum = 0.0;
+    sum += quad_value<0x00>(r);
+    sum += quad_value<0x55>(r);
+    sum += quad_value<0xAA>(r);
+    sum += quad_value<0xFF>(r);
+    const double xc = sum * C.A.id[0];  // the 1 x 1 level: x = b * inv_d
+    {
+        const double v = reg_gs(c, nb(x + xc));
+        x = red ? x : v;
+    }
+    {
+        const double v = reg_gs(c, nb(x));
+        x = red ? v : x;
+    }
+    for (int s = 1; s < o.post; ++s) {
+        const double vb = reg_gs(c, nb(x));
+        x = red ? x : vb;
+        const double vr = reg_gs(c, nb(x));
+        x = red ? vr : x;
+    }
+    if (lane < 4) L.A.x[q] = x;
+    WaveSync{}();
+}
+
 // Calls f(lv[k], lv[k + 1]) for a wave-uniform k < kRegLevels - 1.
 template <class F>
 DARCY_INLINE void reg_pair(const RegLevel* lv, int k, F&& f) {
@@ -195,24 +256,27 @@ DARCY_INLINE void reg_pair(const RegLevel* lv, int k, F&& f) {
 }
 
 // The cycle over levels 0..nl-1 of lv, the last of side 1 (cycle_levels
-// with the visits above).
+// with the visits above).  The last two levels (sides 2 and 1) are visited
+// together by reg_visit2; a repeated visit of the 1 x 1 level recomputes
+// x = b * inv_d from an unchanged b, so it is not repeated.
 DARCY_INLINE void reg_cycle_levels(const RegLevel* lv, int nl, bool fresh0, const CycleOpts& o) {
     if (nl == 1) {  // a lone 1 x 1 level
         if (threadIdx.x == 0) lv[0].A.x[0] = lv[0].A.b[0] * lv[0].A.id[0];
         WaveSync{}();
         return;
     }
+    const int bottom = nl - 2;
     uint32_t done = 0;
     int l = 0;
     bool fresh = fresh0;
     for (;;) {
-        while (l + 1 < nl) {
+        while (l < bottom) {
             reg_pair(lv, l, [&](const RegLevel& L, const RegLevel& C) { reg_down(L, C, fresh, o); });
             done &= ~(1u << l);
             ++l;
             fresh = true;
         }
-        // level l is the 1 x 1 level, solved when its b was restricted
+        reg_pair(lv, l, [&](const RegLevel& L, const RegLevel& C) { reg_visit2(L, C, fresh, o); });
         for (;;) {
             if (l == 0) return;
             --l;
