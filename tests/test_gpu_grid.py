@@ -121,3 +121,21 @@ def test_variable_forcing_on_grid_path(oracle):
     got = darcy.solve(a, f, tol=1e-12)
     want = oracle.solve(a, f, tol=1e-12)
     assert got.iterations == want.iterations and rel_l2(got.u, want.u) <= 1e-9
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("cycle_index", [1, 2])
+def test_grid_path_with_more_sweeps(precision, cycle_index, oracle):
+    """pre 2 / post 3 sweeps on the grid path (n = 150: levels 150 .. 1, the
+    tail from 38 down, odd sides on the way): the persistent kernel's bits,
+    and in FP64 the oracle's iterations and u within 1e-9."""
+    a = two_phase(150, seed=7)
+    kw = dict(pre_sweeps=2, post_sweeps=3, cycle_index=cycle_index, precision=precision)
+    single = darcy.solve(a, **kw)
+    us, batch = darcy.solve_batch_uneven([a], **kw)
+    assert single.converged and single.iterations == batch.iterations[0]
+    assert np.array_equal(single.u, us[0])
+    if precision == "fp64":
+        want = oracle.solve(a, pre_sweeps=2, post_sweeps=3, cycle_index=cycle_index)
+        assert single.iterations == want.iterations
+        assert rel_l2(single.u, want.u) <= 1e-9
