@@ -271,7 +271,10 @@ int blocks_for(const darcy_ctx* ctx, int64_t work, int per_block) {
 // multigrid.hpp:97-127 recursed on the host over the large levels).
 int run_grid(darcy_ctx* ctx, const Job& job) {
     const darcy_opts& o = *job.opts;
-    const bool mixed = o.precision == DARCY_MIXED;
+    // Jacobi and no preconditioner need level 0 only and run in FP64, as on
+    // the persistent kernel.
+    const bool mg = o.preconditioner == DARCY_PC_MULTIGRID;
+    const bool mixed = mg && o.precision == DARCY_MIXED;
     const int n = static_cast<int>(job.n[0]);
     const int64_t N = static_cast<int64_t>(n) * n;
     const size_t cells = static_cast<size_t>(N) * 8;
@@ -329,7 +332,7 @@ int run_grid(darcy_ctx* ctx, const Job& job) {
     HIP_TRY(hipEventRecord(ctx->ev0, s), "event record");
     // Hierarchy (level.hpp:79-97, :171-211).
     HIP_TRY(darcy_grid_fine(a, hd, bad, blocks_for(ctx, N, 1024), s), "fine level");
-    for (int l = 0; l + 1 < h.L; ++l) {
+    for (int l = 0; mg && l + 1 < h.L; ++l) {
         const int64_t nc = static_cast<int64_t>(h.n[l + 1]) * h.n[l + 1];
         HIP_TRY(darcy_grid_galerkin(hd, l, blocks_for(ctx, nc, 256), s), "coarsening");
         if (mixed) HIP_TRY(darcy_grid_fp32(hd, l, blocks_for(ctx, static_cast<int64_t>(h.n[l]) * h.n[l], 1024), s),
@@ -396,6 +399,13 @@ int run_grid(darcy_ctx* ctx, const Job& job) {
     // replayed from a graph captured on first use (the sequence depends only
     // on the hierarchy, the options and the workspace address).
     auto precondition = [&]() -> int {
+        if (!mg) {
+            HIP_TRY(darcy_grid_cg(o.preconditioner == DARCY_PC_JACOBI ? 4 : 5, &g, nullptr, nullptr,
+                                  blocks_for(ctx, N, 1024), s),
+                    "diagonal preconditioner");
+            HIP_TRY(darcy_grid_cg(1, &g, nullptr, nullptr, cg_blocks, s), "rz");
+            return DARCY_OK;
+        }
         if (!ctx->grid_graph || lT == 0) {
             cycle(0, true);
             if (rc) return rc;
@@ -472,7 +482,7 @@ int run_grid(darcy_ctx* ctx, const Job& job) {
         darcy_stats& d = job.stats[0];
         d.iterations = iters;
         d.converged = hs.status == kConverged ? 1 : 0;
-        d.levels = zero_rhs ? 1 : h.L;
+        d.levels = zero_rhs || !mg ? 1 : h.L;
         d.status = hs.status;
         d.residual = zero_rhs ? 0.0 : hs.res;
         d.bad_index = -1;
@@ -546,7 +556,7 @@ extern "C" int darcy_solve(darcy_ctx* ctx, const double* a, const double* rhs, d
     if (const int rc = check_opts(ctx, o)) return rc;
     Job job{kModeSolve, &o, 1, &n, &a, &rhs, &f, &u0, &u_out, stats, history, history ? history_cap : 0,
             o.device_pointers != 0};
-    if (o.preconditioner == DARCY_PC_MULTIGRID && n > ctx->grid_min_n) return run_grid(ctx, job);
+    if (n > ctx->grid_min_n) return run_grid(ctx, job);
     return run(ctx, job);
 }
 

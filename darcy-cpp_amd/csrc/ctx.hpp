@@ -47,7 +47,7 @@ struct darcy_ctx {
     std::string error;
     std::mutex mu;
     darcy_hip::DevBuf slots, probs, stats, order, counter, staging, grid;
-    // Single multigrid solves with n > grid_min_n run on the grid path, with
+    // Single solves with n > grid_min_n run on the grid path, with
     // levels of side <= grid_tail_side in one workgroup (tuned values);
     // grid_graph != 0 replays the cycle from a captured HIP graph.
     int grid_min_n = 64;

@@ -355,6 +355,17 @@ __global__ __launch_bounds__(kBlock) void kg_apply(GridCg g, const double* pold,
         });
 }
 
+// z = r * inv_d (Jacobi, solver.hpp:127-132) or z = r (no preconditioner,
+// :124-126), as the persistent kernel forms it.
+__global__ __launch_bounds__(kBlock) void kg_diag(GridCg g, int jacobi) {
+    const Hier& h = *g.hd;
+    const int N = h.n[0] * h.n[0];
+    const double* id = h.id[0];
+    for (int k = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); k < N;
+         k += static_cast<int>(gridDim.x * blockDim.x))
+        h.z[k] = jacobi ? h.r[k] * id[k] : h.r[k];
+}
+
 // u += alpha p, r -= alpha ap, chunk sums of <r, r>.
 __global__ __launch_bounds__(kBlock) void kg_update(GridCg g, const double* p) {
     const Hier& h = *g.hd;
@@ -453,9 +464,12 @@ extern "C" hipError_t darcy_grid_cg(int which, const GridCg* g, const double* po
             hipLaunchKernelGGL(kg_apply, dim3(blocks), dim3(kBlock), 0, s, *g, pold, pnew);
             hipLaunchKernelGGL(kg_final<2>, dim3(1), dim3(kFinalThreads), 0, s, *g);
             break;
-        default:
+        case 3:
             hipLaunchKernelGGL(kg_update, dim3(blocks), dim3(kBlock), 0, s, *g, pold);
             hipLaunchKernelGGL(kg_final<3>, dim3(1), dim3(kFinalThreads), 0, s, *g);
+            break;
+        default:  // 4: Jacobi, 5: no preconditioner
+            hipLaunchKernelGGL(kg_diag, dim3(blocks), dim3(kBlock), 0, s, *g, which == 4 ? 1 : 0);
             break;
     }
     return hipGetLastError();

@@ -1,11 +1,11 @@
-"""Command line: ``python -m darcy solve | dataset | info``.
+"""Command line: ``python -m darcy solve | dataset | bench | info``.
 
 The commands, flags, defaults, printed lines, output files and exit codes
-are those of the reference's ``darcy`` command (python/darcy/cli.py:16-200);
-the solves run on the GPU through this package.  One option is added:
-``--precision`` (``fp64``, the reference's arithmetic, or ``mixed``, the FP32
-cycle inside FP64 flexible CG).  The reference's ``bench`` command (a CPU
-timing table against SciPy) has its GPU counterpart in ``bench.py`` at the
+are those of the reference's ``darcy`` command (python/darcy/cli.py:16-200,
+benchmark.py:54-140); the solves run on the GPU through this package.  One
+option is added: ``--precision`` (``fp64``, the reference's arithmetic, or
+``mixed``, the FP32 cycle inside FP64 flexible CG).  The headline benchmark
+(batched 256^2 solves with the learned warm start) is ``bench.py`` at the
 repository root.
 """
 
@@ -101,6 +101,67 @@ def cmd_dataset(args: argparse.Namespace) -> int:
     return 0
 
 
+def _best_time(fn, repeats: int):
+    best, result = float("inf"), None
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        result = fn()
+        best = min(best, time.perf_counter() - t0)
+    return best, result
+
+
+def _table(rows: list[dict]) -> str:
+    """The Markdown table of benchmark.py:123-140."""
+    lines = ["| grid | preconditioner | CG iters | levels | time (ms) | SciPy direct (ms) | speed-up | rel. err |",
+             "|" + "|".join(["---"] * 8) + "|"]
+    for r in rows:
+        d, t, e = r["scipy_seconds"], r["seconds"], r["relative_error"]
+        direct = "-" if d is None else f"{d * 1e3:.1f}"
+        speed = "-" if d is None else f"{d / t:.2f}x"
+        err = "-" if e is None else f"{e:.1e}"
+        lines.append(f"| {r['grid']} | {r['preconditioner']} | {r['iterations']} | {r['levels']} | {t * 1e3:.1f} | "
+                     f"{direct} | {speed} | {err} |")
+    return "\n".join(lines)
+
+
+def cmd_bench(args: argparse.Namespace) -> int:
+    """The reference's benchmark rows (benchmark.py:54-140) on the GPU: per
+    grid the seed-0 two-phase medium, the minimum wall time over repeats of
+    darcy.solve per preconditioner (iteration cap 1000 for multigrid, 100000
+    otherwise), and a SciPy direct solve of the assembled operator as the
+    baseline and the error reference.  Prints the rows and the Markdown table
+    of the reference's README; --json writes the rows."""
+    from darcy.direct import solve_direct
+
+    info = darcy.build_info()
+    print(f"darcy-cpp {info['version']} | {info['compiler']}")
+    print(f"device: {darcy.device_info()}")
+    print()
+    rows = []
+    for n in args.grids:
+        a = _medium(n, 0, 2.0, 3.0)
+        direct_s, u_ref = (None, None) if args.no_scipy else _best_time(lambda a=a: solve_direct(a), 1)
+        for name in args.preconditioners:
+            cap = 1_000 if name == "multigrid" else 100_000
+            secs, r = _best_time(lambda a=a, name=name, cap=cap: darcy.solve(
+                a, tol=1e-10, preconditioner=name, max_iter=cap, precision=args.precision), args.repeats)
+            err = None if u_ref is None else float(np.linalg.norm(r.u - u_ref) / np.linalg.norm(u_ref))
+            row = {"grid": n, "preconditioner": name, "seconds": secs, "iterations": r.iterations,
+                   "levels": r.levels, "residual": r.residual, "scipy_seconds": direct_s, "relative_error": err}
+            rows.append(row)
+            speed = f"{direct_s / secs:.2f}x" if direct_s else "-"
+            err_s = "-" if err is None else f"{err:.1e}"
+            print(f"  {n:>5} {name:>10}  {secs * 1e3:>9.1f} ms  {r.iterations:>6} iters  speedup {speed:>7}  "
+                  f"rel.err {err_s}")
+    print()
+    print(_table(rows))
+    if args.json is not None:
+        args.json.parent.mkdir(parents=True, exist_ok=True)
+        args.json.write_text(json.dumps(rows, indent=2))
+        print(f"wrote {args.json}")
+    return 0
+
+
 def cmd_info(_: argparse.Namespace) -> int:
     info = dict(darcy.build_info())
     try:
@@ -112,7 +173,7 @@ def cmd_info(_: argparse.Namespace) -> int:
 
 
 def build_parser() -> argparse.ArgumentParser:
-    """cli.py:138-193, without ``bench``."""
+    """cli.py:138-193."""
     parser = argparse.ArgumentParser(prog="python -m darcy",
                                      description="Matrix-free multigrid solver for 2-D Darcy flow (MI355X).",
                                      formatter_class=argparse.ArgumentDefaultsHelpFormatter)
@@ -144,6 +205,16 @@ def build_parser() -> argparse.ArgumentParser:
     dataset.add_argument("--output", type=Path, default=Path("data/darcy_train.npz"))
     _solver_flags(dataset)
     dataset.set_defaults(func=cmd_dataset)
+
+    bench = sub.add_parser("bench", help="run the benchmark suite", formatter_class=fmt)
+    bench.add_argument("--grids", type=int, nargs="+", default=[64, 128, 256, 512])
+    bench.add_argument("--repeats", type=int, default=3)
+    bench.add_argument("--no-scipy", action="store_true", help="skip the SciPy direct-solver baseline")
+    bench.add_argument("--preconditioners", nargs="+", default=["multigrid", "jacobi"],
+                       choices=["multigrid", "jacobi", "none"])
+    bench.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    bench.add_argument("--json", type=Path, help="also write the results as JSON")
+    bench.set_defaults(func=cmd_bench)
 
     info = sub.add_parser("info", help="print build, threading and device information")
     info.set_defaults(func=cmd_info)

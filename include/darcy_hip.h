@@ -88,7 +88,7 @@ void darcy_ctx_destroy(darcy_ctx* ctx);
 
 /* Launch-shape tuning of a context (no reference counterpart; set by the
  * autotuner, or by hand):
- *   "grid_min_n"      single multigrid solves with n above this use the
+ *   "grid_min_n"      single solves with n above this use the
  *                     multi-workgroup grid path (default 64)
  *   "grid_tail_side"  on the grid path, levels with side <= this run in one
  *                     workgroup (>= 32, default 64)

@@ -42,3 +42,29 @@ def test_info_prints_build_details(capsys):
     assert cli.main(["info"]) == 0
     out = capsys.readouterr().out
     assert '"version"' in out and '"device"' in out
+
+
+def test_bench_defaults_are_the_references():
+    b = cli.build_parser().parse_args(["bench"])
+    assert (b.grids, b.repeats, b.no_scipy, b.preconditioners, b.json) == (
+        [64, 128, 256, 512], 3, False, ["multigrid", "jacobi"], None)
+
+
+@pytest.mark.parametrize("n", [1, 2, 5, 16])
+def test_direct_operator_is_the_oracles(n, oracle):
+    """The benchmark's assembled matrix applies the solver's operator."""
+    direct = pytest.importorskip("darcy.direct")
+    from conftest import two_phase
+
+    a = two_phase(n, seed=n)
+    x = np.random.default_rng(n).standard_normal((n, n))
+    got = (direct.assemble(a) @ x.ravel()).reshape(n, n)
+    np.testing.assert_allclose(got, oracle.apply_operator(a, x), rtol=1e-13, atol=1e-13 * np.abs(got).max())
+
+
+def test_direct_solution_is_the_oracles(oracle):
+    direct = pytest.importorskip("darcy.direct")
+    from conftest import rel_l2, two_phase
+
+    a = two_phase(48, seed=2)
+    assert rel_l2(direct.solve_direct(a), oracle.solve(a).u) <= 1e-9

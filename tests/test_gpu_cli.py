@@ -46,3 +46,20 @@ def test_solve_mixed_and_nonconvergence(tmp_path):
     print("cli solve mixed / max-iter 2", flush=True)
     assert cli.main(["solve", "--grid", "96", "--precision", "mixed"]) == 0
     assert cli.main(["solve", "--grid", "64", "--max-iter", "2"]) == 1
+
+
+def test_bench_rows(tmp_path, capsys):
+    out = tmp_path / "b.json"
+    print("cli bench 32, 64", flush=True)
+    assert cli.main(["bench", "--grids", "32", "64", "--repeats", "1", "--json", str(out)]) == 0
+    assert "| grid | preconditioner | CG iters |" in capsys.readouterr().out
+    import json
+
+    rows = json.loads(out.read_text())
+    mg = {r["grid"]: r for r in rows if r["preconditioner"] == "multigrid"}
+    assert (mg[32]["iterations"], mg[32]["levels"], mg[64]["iterations"], mg[64]["levels"]) == (18, 6, 21, 7)
+    # Both stop at a 1e-10 relative residual; the error is up to the condition
+    # number (~ n^2 x contrast) times that, which multigrid CG's iterates do
+    # not approach but Jacobi CG's may.
+    assert all(r["relative_error"] <= 1e-9 for r in mg.values())
+    assert all(r["relative_error"] <= 1e-5 for r in rows)

@@ -139,3 +139,20 @@ def test_grid_path_with_more_sweeps(precision, cycle_index, oracle):
         want = oracle.solve(a, pre_sweeps=2, post_sweeps=3, cycle_index=cycle_index)
         assert single.iterations == want.iterations
         assert rel_l2(single.u, want.u) <= 1e-9
+
+
+@pytest.mark.parametrize("preconditioner", ["jacobi", "none"])
+def test_grid_path_diagonal_preconditioners(preconditioner, oracle):
+    """Jacobi / no preconditioner on the grid path (level 0 only, FP64 also
+    when mixed is asked, one level reported): the persistent kernel's bits,
+    and the oracle's solution.  Plain CG runs hundreds of iterations, so its
+    last iteration may move by one with the dot-product summation order."""
+    a = two_phase(100, seed=5)
+    single = darcy.solve(a, preconditioner=preconditioner, precision="mixed", max_iter=20000)
+    us, batch = darcy.solve_batch_uneven([a], preconditioner=preconditioner, max_iter=20000)
+    assert single.converged and single.levels == 1
+    assert single.iterations == batch.iterations[0] and np.array_equal(single.u, us[0])
+    want = oracle.solve(a, preconditioner=preconditioner, max_iter=20000)
+    slack = 0 if preconditioner == "jacobi" else 1
+    assert abs(single.iterations - want.iterations) <= slack
+    assert rel_l2(single.u, want.u) <= 1e-9
