@@ -203,6 +203,15 @@ class Reference:
         L.ref_num_levels.argtypes = [i64]
         L.ref_num_levels.restype = i32
         L.ref_max_threads.restype = i32
+        L.ref_set_num_threads.argtypes = [i32]
+
+    def max_threads(self) -> int:
+        return int(self.lib.ref_max_threads())
+
+    def set_num_threads(self, threads: int) -> None:
+        """Threads of the reductions inside a solve (the library's own
+        OpenMP runtime)."""
+        self.lib.ref_set_num_threads(threads)
 
     def solve(self, a, f=1.0, *, tol=1e-10, max_iter=10000, preconditioner="multigrid", pre_sweeps=1,
               post_sweeps=1, cycle_index=2, record_history=False) -> Result:

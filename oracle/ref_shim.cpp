@@ -146,3 +146,13 @@ extern "C" int32_t ref_num_levels(int64_t n) {
 }
 
 extern "C" int32_t ref_max_threads(void) { return darcy::max_threads(); }
+
+// The OpenMP thread count seen by this library's parallel regions (the
+// reductions inside a solve), set through the runtime this library binds to.
+extern "C" void ref_set_num_threads(int32_t threads) {
+#if DARCY_HAS_OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#else
+    (void)threads;
+#endif
+}

@@ -79,14 +79,18 @@ def test_solutions_bit_exact_with_reference(n, cycle_index, oracle):
     ref = pyoracle.Reference()
     batch = np.stack([two_phase(n, seed=s) for s in range(2)])
     batch[1] = two_phase(n, seed=7, hi=1e3, lo=1e-3)
-    # one OpenMP thread: every reduction runs serially on both sides
-    threads = oracle.max_threads()
+    # one OpenMP thread: every reduction runs serially on both sides, each
+    # library set through the runtime it binds to (another OpenMP runtime
+    # loaded into the process, e.g. by torch, must not decide the order)
+    threads, ref_threads = oracle.max_threads(), ref.max_threads()
     oracle.set_num_threads(1)
+    ref.set_num_threads(1)
     try:
         uo, io, ro, co = oracle.solve_batch(batch, threads=1, cycle_index=cycle_index)
         ur, ir, rr, cr = ref.solve_batch(batch, threads=1, cycle_index=cycle_index)
     finally:
         oracle.set_num_threads(threads)
+        ref.set_num_threads(ref_threads)
     assert np.array_equal(io, ir) and np.array_equal(co, cr)
     assert np.array_equal(uo, ur)
     assert np.array_equal(ro, rr)
