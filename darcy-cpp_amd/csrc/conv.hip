@@ -232,14 +232,20 @@ struct EpiMerge {
     }
     __device__ float pre(int n, float acc) const { return n < N ? acc + bias[n] : acc; }
     // c = the staged bf16 conv output of coarse row m, channels n .. n+7.
+    // The 4 fine pixels' skip values are loaded before any store, so the
+    // loads are in flight together (skip and the outputs may alias as far as
+    // the compiler knows).
     __device__ void put8(int m, int n, const v8bf& cv) const {
         if (m >= M || n >= N) return;
         const RowOff r = row(m);
         const size_t wn = 2 * static_cast<size_t>(ow) * N;
+        v8bf sk[4];
+#pragma unroll
+        for (int d = 0; d < 4; ++d) sk[d] = *reinterpret_cast<const v8bf*>(skip + r.off + n + (d >> 1) * wn + (d & 1) * N);
 #pragma unroll
         for (int d = 0; d < 4; ++d) {
             const size_t i = r.off + n + (d >> 1) * wn + (d & 1) * N;
-            const v8bf sv = *reinterpret_cast<const v8bf*>(skip + i);
+            const v8bf sv = sk[d];
             v8bf zv, av;
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
@@ -453,6 +459,31 @@ __global__ __launch_bounds__(kT) void k_conv(ConvArgs c, Epi epi) {
     }
 }
 
+// Stages COUNT 16-byte items into LDS: item q is read by load(q) and stored
+// by put(q, v).  Each thread issues up to kStageBatch loads before their
+// stores, so they are in flight together: the loads go through generic
+// pointers, which the compiler cannot tell apart from the LDS stores, and a
+// load-store loop would wait for each load in turn.  The batch bounds the
+// registers held (occupancy of the MFMA kernels).
+constexpr int kStageBatch = 6;
+template <int COUNT, class Load, class Put>
+__device__ __forceinline__ void stage_items(Load&& load, Put&& put) {
+    constexpr int PER = (COUNT + kT - 1) / kT;
+    for (int s0 = 0; s0 < PER; s0 += kStageBatch) {
+        v8bf v[kStageBatch];
+#pragma unroll
+        for (int s = 0; s < kStageBatch; ++s) {
+            const int q = static_cast<int>(threadIdx.x) + (s0 + s) * kT;
+            v[s] = s0 + s < PER && q < COUNT ? load(q) : v8bf{};
+        }
+#pragma unroll
+        for (int s = 0; s < kStageBatch; ++s) {
+            const int q = static_cast<int>(threadIdx.x) + (s0 + s) * kT;
+            if (s0 + s < PER && q < COUNT) put(q, v[s]);
+        }
+    }
+}
+
 // ------------------------------------------ the halo-tiled conv (stride 1)
 // For stride-1 convs on wide images with few output channels (the decoder's
 // full- and half-resolution convs), the im2col operand dominates: each input
@@ -515,20 +546,32 @@ __global__ __launch_bounds__(kT) void k_conv_halo(HaloArgs c, Epi epi) {
     v4f acc[4][TN] = {};
     for (int c0 = 0; c0 < cin; c0 += 32) {
         // Stage the patch (zero outside the image) and the chunk's weights.
-        for (int qq = tid; qq < 4 * PP; qq += kT) {
-            const int gg = qq / PP, pp = qq - gg * PP;
-            const int iy = y0 - 1 + pp / PW, ix = x0 - 1 + pp % PW;
-            v8bf v = {};
-            if (iy >= 0 && iy < c.in.h && ix >= 0 && ix < c.in.w)
-                v = *reinterpret_cast<const v8bf*>(c.x + at(c.in, b, iy, ix) + c0 + 8 * gg);
-            lp[gg][pp] = v;
-        }
-        for (int qq = tid; qq < 4 * 9 * BN; qq += kT) {
-            const int gg = qq / (9 * BN), rest = qq - gg * 9 * BN, tap = rest / BN, n = rest - tap * BN;
-            v8bf v = {};
-            if (n < c.cout) v = *reinterpret_cast<const v8bf*>(c.w + static_cast<size_t>(n) * K + tap * cin + c0 + 8 * gg);
-            lw[gg][tap][n] = v;
-        }
+        // items [0, 4 PP): the patch; [4 PP, 4 PP + 36 BN): the weights
+        stage_items<4 * PP + 4 * 9 * BN>(
+            [&](int qq) {
+                v8bf v = {};
+                if (qq < 4 * PP) {
+                    const int gg = qq / PP, pp = qq - gg * PP;
+                    const int iy = y0 - 1 + pp / PW, ix = x0 - 1 + pp % PW;
+                    if (iy >= 0 && iy < c.in.h && ix >= 0 && ix < c.in.w)
+                        v = *reinterpret_cast<const v8bf*>(c.x + at(c.in, b, iy, ix) + c0 + 8 * gg);
+                } else {
+                    const int q = qq - 4 * PP, gg = q / (9 * BN), rest = q - gg * 9 * BN, tap = rest / BN;
+                    const int n = rest - tap * BN;
+                    if (n < c.cout)
+                        v = *reinterpret_cast<const v8bf*>(c.w + static_cast<size_t>(n) * K + tap * cin + c0 + 8 * gg);
+                }
+                return v;
+            },
+            [&](int qq, const v8bf& v) {
+                if (qq < 4 * PP) {
+                    const int gg = qq / PP;
+                    lp[gg][qq - gg * PP] = v;
+                } else {
+                    const int q = qq - 4 * PP, gg = q / (9 * BN), rest = q - gg * 9 * BN, tap = rest / BN;
+                    lw[gg][tap][rest - tap * BN] = v;
+                }
+            });
         __syncthreads();
 #pragma unroll
         for (int tap = 0; tap < 9; ++tap) {
@@ -571,18 +614,27 @@ __global__ __launch_bounds__(kT) void k_conv_halo8(HaloArgs c, Epi epi) {
     if ((nb & 7) == 0) t = (t & 7) * (nb >> 3) + (t >> 3);
     const int tx = t % c.tiles_x, q = t / c.tiles_x, ty = q % c.tiles_y, b = q / c.tiles_y;
     const int x0 = tx * TW, y0 = ty * TH;
-    for (int pp = tid; pp < PP; pp += kT) {
-        const int iy = y0 - 1 + pp / PW, ix = x0 - 1 + pp % PW;
-        v8bf v = {};
-        if (iy >= 0 && iy < c.in.h && ix >= 0 && ix < c.in.w) v = *reinterpret_cast<const v8bf*>(c.x + at(c.in, b, iy, ix));
-        lp[pp] = v;
-    }
-    for (int qq = tid; qq < 9 * BN; qq += kT) {
-        const int tap = qq / BN, n = qq - tap * BN;
-        v8bf v = {};
-        if (n < c.cout) v = *reinterpret_cast<const v8bf*>(c.w + static_cast<size_t>(n) * 72 + tap * 8);
-        lw[tap][n] = v;
-    }
+    stage_items<PP + 9 * BN>(  // the patch, then the weights
+        [&](int qq) {
+            v8bf v = {};
+            if (qq < PP) {
+                const int iy = y0 - 1 + qq / PW, ix = x0 - 1 + qq % PW;
+                if (iy >= 0 && iy < c.in.h && ix >= 0 && ix < c.in.w)
+                    v = *reinterpret_cast<const v8bf*>(c.x + at(c.in, b, iy, ix));
+            } else {
+                const int q = qq - PP, tap = q / BN, n = q - tap * BN;
+                if (n < c.cout) v = *reinterpret_cast<const v8bf*>(c.w + static_cast<size_t>(n) * 72 + tap * 8);
+            }
+            return v;
+        },
+        [&](int qq, const v8bf& v) {
+            if (qq < PP) {
+                lp[qq] = v;
+            } else {
+                const int q = qq - PP, tap = q / BN;
+                lw[tap][q - tap * BN] = v;
+            }
+        });
     __syncthreads();
     v4f acc[4][TN] = {};
 #pragma unroll
@@ -696,6 +748,12 @@ __global__ __launch_bounds__(kT) void k_head_fwd(const bf16* x, Geo in, const fl
 // The head for 32 input channels: a workgroup computes a 16 x 16 pixel tile
 // from an 18 x 18 halo tile of x staged once in LDS (pixel stride padded to
 // 40 channels), instead of every thread re-reading its 9 neighbours.
+// The head, 3x3 conv from 32 channels to 1 in FP32: a 16 x 16 output tile
+// per workgroup over its 18 x 18 bf16 input patch in LDS (a pixel's 32
+// channels padded to 40, so 16 lanes reading consecutive pixels hit disjoint
+// banks), 288 fused multiply-adds per pixel.  Sum order: bias, then taps,
+// then channels.  The bf16 patch (20 KB) leaves room for 7 workgroups per CU,
+// which the staging loads need more than the arithmetic needs FP32 operands.
 constexpr int kHeadTile = 16, kHeadC = 32, kHeadPad = 40;
 __global__ __launch_bounds__(kT) void k_head_fwd_tiled(const bf16* x, Geo in, const float* w, const float* bias,
                                                        float* y) {
@@ -705,13 +763,14 @@ __global__ __launch_bounds__(kT) void k_head_fwd_tiled(const bf16* x, Geo in, co
     for (int i = threadIdx.x; i < 9 * kHeadC; i += kT) ws[i] = w[i];
     if (threadIdx.x == 0) ws[9 * kHeadC] = bias[0];
     constexpr int side = kHeadTile + 2, chunks = side * side * (kHeadC / 8);
-    for (int q = threadIdx.x; q < chunks; q += kT) {
-        const int ch = (q & 3) * 8, p = q >> 2, py = p / side, px = p % side;
-        const int iy = ty0 + py - 1, ix = tx0 + px - 1;
-        v8bf v = {};
-        if (iy >= 0 && iy < in.h && ix >= 0 && ix < in.w) v = *reinterpret_cast<const v8bf*>(x + at(in, b, iy, ix) + ch);
-        *reinterpret_cast<v8bf*>(tile + p * kHeadPad + ch) = v;
-    }
+    stage_items<chunks>(
+        [&](int q) {
+            const int ch = (q & 3) * 8, p = q >> 2, iy = ty0 + p / side - 1, ix = tx0 + p % side - 1;
+            v8bf v = {};
+            if (iy >= 0 && iy < in.h && ix >= 0 && ix < in.w) v = *reinterpret_cast<const v8bf*>(x + at(in, b, iy, ix) + ch);
+            return v;
+        },
+        [&](int q, const v8bf& v) { *reinterpret_cast<v8bf*>(tile + (q >> 2) * kHeadPad + (q & 3) * 8) = v; });
     __syncthreads();
     const int ly = threadIdx.x / kHeadTile, lx = threadIdx.x % kHeadTile;
     const int oy = ty0 + ly, ox = tx0 + lx;
@@ -723,7 +782,7 @@ __global__ __launch_bounds__(kT) void k_head_fwd_tiled(const bf16* x, Geo in, co
         for (int ci = 0; ci < kHeadC; ci += 8) {
             const v8bf v = *reinterpret_cast<const v8bf*>(xp + ci);
 #pragma unroll
-            for (int e = 0; e < 8; ++e) s += wt[ci + e] * bf2f(v[e]);
+            for (int e = 0; e < 8; ++e) s = fmaf(wt[ci + e], bf2f(v[e]), s);
         }
     }
     if (oy < in.h && ox < in.w) y[(static_cast<size_t>(b) * in.h + oy) * in.w + ox] = s;
