@@ -155,9 +155,6 @@ DARCY_INLINE void run_problem(Shared& s, T* pool, float* tail, const Problem& pb
     double beta = 0.0;
     bool first = true;
     int status = kMaxIter, iters = 0;
-    struct PAp {
-        double p, ap;
-    };
     struct Upd {
         double u, p, r, ap;
     };
@@ -165,23 +162,12 @@ DARCY_INLINE void run_problem(Shared& s, T* pool, float* tail, const Problem& pb
         double r, z, ap;
     };
     for (int it = 0; it < P.max_iter; ++it) {
-        // p = z + beta p (solver.hpp:171, blas.hpp:50-56), fused with
-        // ap = A p (solver.hpp:147) and <p, ap> (solver.hpp:148).
-        chunk_pass<1>(
-            N, wave, kWaves, h.part,
-            [&](int k) {
-                const int i = k / n, j = k - i * n;
-                auto pv = [&](int t) -> double { return first ? zval(t) : zval(t) + beta * pold[t]; };
-                PAp v;
-                v.ap = apply_at(n, A0.ex, A0.ny, i, j, pv);
-                v.p = pv(k);
-                return v;
-            },
-            [&](int k, const PAp& v, double* acc) {
-                pnew[k] = v.p;
-                h.ap[k] = v.ap;
-                acc[0] += v.p * v.ap;
-            });
+        // p = z + beta p, ap = A p, <p, ap>.
+        if constexpr (mixed)
+            cg_direction<1>(n, A0, x0f, pold, beta, first, pnew, h.ap, wave, kWaves, h.part);
+        else
+            cg_direction<1>(n, A0, static_cast<const double*>(h.z), pold, beta, first, pnew, h.ap, wave, kWaves,
+                            h.part);
         reduce_parts(s, 1, C);
         const double pap = s.red[0];
         if (!(fabs(pap) > 0.0) || !isfinite(pap)) {  // solver.hpp:149-155

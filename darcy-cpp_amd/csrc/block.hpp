@@ -335,6 +335,62 @@ DARCY_INLINE void chunk_pass(int N, int w, int nw, double* const* part, Load&& l
     }
 }
 
+// The CG direction pass: p = z + beta p_old, or p = z on the first iteration
+// (solver.hpp:171, blas.hpp:50-56), fused with ap = A p (solver.hpp:147) and
+// the chunk sums of <p, ap> (solver.hpp:148).  z is stored as Zt (float for
+// the FP32 cycle) and widened to double.
+// The first-iteration test stays outside the cell loop: inside it, the load
+// of p_old would be conditional, each stencil point would branch around its
+// load, and every load would wait for the one before.
+template <int CPR, class Zt>
+DARCY_INLINE void cg_direction(int n, const LvP<double>& A0, const Zt* z, const double* pold, double beta, bool first,
+                               double* pnew, double* ap, int w, int nw, double* const* part) {
+    struct PAp {
+        double p, ap;
+    };
+    // `pv(t, loaded)` forms p at t.  All of a cell's loads are issued before
+    // any arithmetic (the scheduling barrier keeps them together): left to
+    // itself, the scheduler of this register-bound kernel waits for each
+    // stencil point's loads before issuing the next point's.
+    auto pass = [&](auto&& load, auto&& pv) {
+        chunk_pass<1, CPR>(
+            n * n, w, nw, part,
+            [&](int k) {
+                const int i = k / n, j = k - i * n, f = i * (n + 1) + j;
+                const bool hE = j + 1 < n, hW = j > 0, hN = i + 1 < n, hS = i > 0;
+                const int t[5] = {k, hE ? k + 1 : k, hW ? k - 1 : k, hN ? k + n : k, hS ? k - n : k};
+                const double e = A0.ex[f + 1], wf = A0.ex[f], nn = A0.ny[(i + 1) * n + j], s = A0.ny[i * n + j];
+                decltype(load(0)) l[5];
+#pragma unroll
+                for (int q = 0; q < 5; ++q) l[q] = load(t[q]);
+                __builtin_amdgcn_sched_barrier(0);
+                // apply_at's operations, in its order.
+                const double d = ((e + wf) + nn) + s;
+                const double x0 = pv(l[0]), xE = pv(l[1]), xW = pv(l[2]), xN = pv(l[3]), xS = pv(l[4]);
+                double a = d * x0;
+                a -= hE ? e * xE : 0.0;
+                a -= hW ? wf * xW : 0.0;
+                a -= hN ? nn * xN : 0.0;
+                a -= hS ? s * xS : 0.0;
+                return PAp{x0, a};
+            },
+            [&](int k, const PAp& v, double* acc) {
+                pnew[k] = v.p;
+                ap[k] = v.ap;
+                acc[0] += v.p * v.ap;
+            });
+    };
+    struct ZP {
+        Zt z;
+        double p;
+    };
+    if (first)
+        pass([&](int t) { return z[t]; }, [](Zt v) { return static_cast<double>(v); });
+    else
+        pass([&](int t) { return ZP{z[t], pold[t]}; },
+             [&](const ZP& v) { return static_cast<double>(v.z) + beta * v.p; });
+}
+
 // Lane l adds chunk sums l, l+64, l+128, ... in that order; the loads of
 // sixteen terms are issued before their adds.
 __device__ inline double final_sum(const double* part, int C) {

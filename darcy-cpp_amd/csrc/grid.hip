@@ -306,10 +306,6 @@ __global__ __launch_bounds__(kBlock) void kg_init(GridCg g) {
         });
 }
 
-DARCY_INLINE double zval(const Hier& h, int mixed, int k) {
-    return mixed ? static_cast<double>(static_cast<const float*>(h.x[0])[k]) : h.z[k];
-}
-
 // The streaming CG kernels (kg_rz, kg_update) take two chunks per round so
 // each wave has twice the loads in flight (kg_apply, whose stencil loads need
 // more waves, takes one); the chunk sums, and so the result, do not depend on
@@ -317,48 +313,44 @@ DARCY_INLINE double zval(const Hier& h, int mixed, int k) {
 constexpr int kCgChunks = 2;
 
 // Chunk sums of <r, z> (and <z, A p> in the flexible variant).
+// The precision test stays outside the cell loops, as in cg_direction.
 __global__ __launch_bounds__(kBlock) void kg_rz(GridCg g) {
     const Hier& h = *g.hd;
     const int N = h.n[0] * h.n[0];
-    struct Rz {
-        double r, z, ap;
-    };
-    chunk_pass<2, kCgChunks>(
-        N, grid_wave(), grid_waves(), h.part,
-        [&](int k) { return Rz{h.r[k], zval(h, g.mixed, k), g.mixed ? h.ap[k] : 0.0}; },
-        [&](int, const Rz& v, double* acc) {
-            acc[0] += v.r * v.z;
-            if (g.mixed) acc[1] += v.z * v.ap;
-        });
+    if (g.mixed) {
+        struct Rz {
+            double r, z, ap;
+        };
+        const float* z = static_cast<const float*>(h.x[0]);
+        chunk_pass<2, kCgChunks>(
+            N, grid_wave(), grid_waves(), h.part,
+            [&](int k) { return Rz{h.r[k], static_cast<double>(z[k]), h.ap[k]}; },
+            [&](int, const Rz& v, double* acc) {
+                acc[0] += v.r * v.z;
+                acc[1] += v.z * v.ap;
+            });
+    } else {
+        struct Rz {
+            double r, z;
+        };
+        chunk_pass<1, kCgChunks>(
+            N, grid_wave(), grid_waves(), h.part, [&](int k) { return Rz{h.r[k], h.z[k]}; },
+            [&](int, const Rz& v, double* acc) { acc[0] += v.r * v.z; });
+    }
 }
 
 // p = z + beta p (first: p = z), ap = A p, chunk sums of <p, ap>.
 __global__ __launch_bounds__(kBlock) void kg_apply(GridCg g, const double* pold, double* pnew) {
     const Hier& h = *g.hd;
-    const int n = h.n[0], N = n * n;
     const LvP<double> A0 = global_level<double>(h, 0);
     const bool first = g.st->first != 0;
     const double beta = g.st->beta;
-    struct PAp {
-        double p, ap;
-    };
-    chunk_pass<1>(
-        N, grid_wave(), grid_waves(), h.part,
-        [&](int k) {
-            const int i = k / n, j = k - i * n;
-            auto pv = [&](int t) -> double {
-                return first ? zval(h, g.mixed, t) : zval(h, g.mixed, t) + beta * pold[t];
-            };
-            PAp v;
-            v.ap = apply_at(n, A0.ex, A0.ny, i, j, pv);
-            v.p = pv(k);
-            return v;
-        },
-        [&](int k, const PAp& v, double* acc) {
-            pnew[k] = v.p;
-            h.ap[k] = v.ap;
-            acc[0] += v.p * v.ap;
-        });
+    if (g.mixed)
+        cg_direction<1>(A0.n, A0, static_cast<const float*>(h.x[0]), pold, beta, first, pnew, h.ap, grid_wave(),
+                        grid_waves(), h.part);
+    else
+        cg_direction<1>(A0.n, A0, static_cast<const double*>(h.z), pold, beta, first, pnew, h.ap, grid_wave(),
+                        grid_waves(), h.part);
 }
 
 // z = r * inv_d (Jacobi, solver.hpp:127-132) or z = r (no preconditioner,
