@@ -158,8 +158,12 @@ DARCY_INLINE void run_problem(Shared& s, T* pool, float* tail, const Problem& pb
     struct Upd {
         double u, p, r, ap;
     };
+    // z is read as stored (float for the FP32 cycle) and widened after the
+    // loads, so that no conversion waits between them.
     struct Rz {
-        double r, z, ap;
+        double r;
+        T z;
+        double ap;
     };
     for (int it = 0; it < P.max_iter; ++it) {
         // p = z + beta p, ap = A p, <p, ap>.
@@ -196,11 +200,13 @@ DARCY_INLINE void run_problem(Shared& s, T* pool, float* tail, const Problem& pb
         }
         if (iters == P.max_iter) break;
         precondition(cyc, h, P, N, e);
+        const T* const z0 = mixed ? reinterpret_cast<const T*>(h.x[0]) : reinterpret_cast<const T*>(h.z);
         chunk_pass<2>(
-            N, wave, kWaves, h.part, [&](int k) { return Rz{h.r[k], zval(k), mixed ? h.ap[k] : 0.0}; },
+            N, wave, kWaves, h.part, [&](int k) { return Rz{h.r[k], z0[k], mixed ? h.ap[k] : 0.0}; },
             [&](int, const Rz& v, double* acc) {
-                acc[0] += v.r * v.z;
-                if (mixed) acc[1] += v.z * v.ap;
+                const double z = static_cast<double>(v.z);
+                acc[0] += v.r * z;
+                if (mixed) acc[1] += z * v.ap;
             });
         reduce_parts(s, mixed ? 2 : 1, C);
         beta = mixed ? (-alpha * s.red[1]) / rz : s.red[0] / rz;

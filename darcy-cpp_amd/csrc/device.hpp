@@ -91,10 +91,11 @@ template <class T, class X>
 DARCY_INLINE T apply_at(int n, const T* ex, const T* ny, int i, int j, X&& x) {
     const int k = i * n + j, f = i * (n + 1) + j;
     const T e = ex[f + 1], w = ex[f], nn = ny[(i + 1) * n + j], s = ny[i * n + j];
-    const T d = ((e + w) + nn) + s;
     const bool hE = j + 1 < n, hW = j > 0, hN = i + 1 < n, hS = i > 0;
     const T xk = x(k), xE = x(hE ? k + 1 : k), xW = x(hW ? k - 1 : k), xN = x(hN ? k + n : k),
             xS = x(hS ? k - n : k);
+    __builtin_amdgcn_sched_barrier(0);  // all loads in flight together, as in gs_at
+    const T d = ((e + w) + nn) + s;
     T v = d * xk;
     v -= hE ? e * xE : T(0);
     v -= hW ? w * xW : T(0);
@@ -103,16 +104,29 @@ DARCY_INLINE T apply_at(int n, const T* ex, const T* ny, int i, int j, X&& x) {
     return v;
 }
 
+struct Ident {
+    template <class V>
+    DARCY_INLINE V operator()(const V& v) const {
+        return v;
+    }
+};
+
 // Gauss-Seidel value of cell (i, j) from its neighbours (level.hpp:153-158).
-// `X` maps grid coordinates (i, j) to the value of x there.
-template <class T, class X>
-DARCY_INLINE T gs_at(const LvP<T>& L, int i, int j, T bk, X&& x) {
+// `ld(i, j)` reads what x at grid coordinates (i, j) is formed from and
+// `cv` forms it (by default x is read as stored).  All of the cell's loads
+// are issued before its arithmetic -- the scheduling barrier keeps them
+// together: left to itself, the scheduler of a register-bound kernel waits
+// for each neighbour's loads before issuing the next neighbour's.
+template <class T, class Ld, class Cv = Ident>
+DARCY_INLINE T gs_at(const LvP<T>& L, int i, int j, T bk, Ld&& ld, Cv&& cv = Cv{}) {
     const int n = L.n, f = i * (n + 1) + j;
     const bool hE = j + 1 < n, hW = j > 0, hN = i + 1 < n, hS = i > 0;
     const T e = L.ex[f + 1], w = L.ex[f], nn = L.ny[(i + 1) * n + j], s = L.ny[i * n + j];
-    const T xE = x(i, hE ? j + 1 : j), xW = x(i, hW ? j - 1 : j), xN = x(hN ? i + 1 : i, j),
-            xS = x(hS ? i - 1 : i, j);
+    const auto rE = ld(i, hE ? j + 1 : j), rW = ld(i, hW ? j - 1 : j), rN = ld(hN ? i + 1 : i, j),
+               rS = ld(hS ? i - 1 : i, j);
     const T inv = L.id[i * n + j];
+    __builtin_amdgcn_sched_barrier(0);
+    const T xE = cv(rE), xW = cv(rW), xN = cv(rN), xS = cv(rS);
     T v = bk;
     v += hE ? e * xE : T(0);
     v += hW ? w * xW : T(0);
@@ -168,6 +182,11 @@ DARCY_INLINE void presmooth_from_zero(const LvP<T>& L, Exec e) {
     T* x = L.x;
     const T* b = L.b;
     const T* id = L.id;
+    struct BI {
+        T b, id;
+    };
+    auto red_ld = [&](int a, int c) { return BI{b[a * n + c], id[a * n + c]}; };
+    auto red_cv = [](const BI& v) { return v.b * v.id; };
     auto red = [&](int a, int c) { return b[a * n + c] * id[a * n + c]; };
     for (int q0 = e.tid; q0 < total; q0 += 4 * e.nthr) {
         T vr[4], vb[4];
@@ -183,7 +202,7 @@ DARCY_INLINE void presmooth_from_zero(const LvP<T>& L, Exec e) {
             }
             if (q < total && colour_cell(q, n, h, 1, i, j)) {
                 kb[u] = i * n + j;
-                vb[u] = gs_at(L, i, j, b[kb[u]], red);
+                vb[u] = gs_at(L, i, j, b[kb[u]], red_ld, red_cv);
             }
         }
 #pragma unroll
@@ -194,6 +213,26 @@ DARCY_INLINE void presmooth_from_zero(const LvP<T>& L, Exec e) {
     }
 }
 
+// x and the coarse correction x_c of its aggregate, read together; XPlusC
+// forms the prolongated value x + x_c.
+template <class T>
+struct XC {
+    T x, c;
+};
+template <class T>
+struct Prolonged {
+    const T* x;
+    const T* xc;
+    int n, m;
+    DARCY_INLINE XC<T> operator()(int a, int c) const { return XC<T>{x[a * n + c], xc[(a >> 1) * m + (c >> 1)]}; }
+};
+struct XPlusC {
+    template <class T>
+    DARCY_INLINE T operator()(const XC<T>& v) const {
+        return v.x + v.c;
+    }
+};
+
 // First post-smoothing sweep (black) fused with the prolongation
 // x += P x_c (level.hpp:230-237): red neighbours are read as x + x_c of their
 // aggregate, the value the reference stores before smoothing.  Red cells keep
@@ -203,7 +242,7 @@ template <class T>
 DARCY_INLINE void black_with_prolong(const LvP<T>& L, const T* xc, Exec e) {
     const int n = L.n, h = (n + 1) >> 1, total = n * h, m = (n + 1) >> 1;
     T* x = L.x;
-    auto prolonged = [&](int a, int c) { return x[a * n + c] + xc[(a >> 1) * m + (c >> 1)]; };
+    const Prolonged<T> pr{x, xc, n, m};
     for (int q0 = e.tid; q0 < total; q0 += 4 * e.nthr) {
         T v[4];
         int kk[4];
@@ -214,7 +253,7 @@ DARCY_INLINE void black_with_prolong(const LvP<T>& L, const T* xc, Exec e) {
             kk[u] = -1;
             if (q < total && colour_cell(q, n, h, 1, i, j)) {
                 kk[u] = i * n + j;
-                v[u] = gs_at(L, i, j, L.b[kk[u]], prolonged);
+                v[u] = gs_at(L, i, j, L.b[kk[u]], pr, XPlusC{});
             }
         }
 #pragma unroll
@@ -456,12 +495,12 @@ DARCY_INLINE void up_stream(const LvP<T>& L, const T* xc, Exec e) {
     T* x = L.x;
     const T* b = L.b;
     auto xv = [&](int a, int c) { return x[a * n + c]; };
-    auto prolonged = [&](int a, int c) { return x[a * n + c] + xc[(a >> 1) * m + (c >> 1)]; };
+    const Prolonged<T> pr{x, xc, n, m};
     for (int r0 = 0;; r0 += R) {
         const int black_hi = r0 + R < n ? r0 + R : n;
         if (r0 < n)
             update_rows(x, n, 1, r0, black_hi, e,
-                        [&](int i, int j) { return gs_at(L, i, j, b[i * n + j], prolonged); });
+                        [&](int i, int j) { return gs_at(L, i, j, b[i * n + j], pr, XPlusC{}); });
         __syncthreads();
         // red row i needs black rows i-1 and i+1; red rows of step t never
         // meet the black rows of step t+1, so no barrier follows.

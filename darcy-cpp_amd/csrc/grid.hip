@@ -319,15 +319,17 @@ __global__ __launch_bounds__(kBlock) void kg_rz(GridCg g) {
     const int N = h.n[0] * h.n[0];
     if (g.mixed) {
         struct Rz {
-            double r, z, ap;
+            double r;
+            float z;  // widened after the loads
+            double ap;
         };
         const float* z = static_cast<const float*>(h.x[0]);
         chunk_pass<2, kCgChunks>(
-            N, grid_wave(), grid_waves(), h.part,
-            [&](int k) { return Rz{h.r[k], static_cast<double>(z[k]), h.ap[k]}; },
+            N, grid_wave(), grid_waves(), h.part, [&](int k) { return Rz{h.r[k], z[k], h.ap[k]}; },
             [&](int, const Rz& v, double* acc) {
-                acc[0] += v.r * v.z;
-                acc[1] += v.z * v.ap;
+                const double zk = static_cast<double>(v.z);
+                acc[0] += v.r * zk;
+                acc[1] += zk * v.ap;
             });
     } else {
         struct Rz {
