@@ -155,6 +155,10 @@ DARCY_INLINE void run_problem(Shared& s, T* pool, float* tail, const Problem& pb
     double beta = 0.0;
     bool first = true;
     int status = kMaxIter, iters = 0;
+    // With the FP32 cycle, u += alpha p of iteration it is applied by the
+    // direction pass of it + 1 (or after the loop), where p is read anyway.
+    // The FP64 kernel, short of registers, keeps it in the update pass.
+    constexpr bool defer = mixed;
     struct Upd {
         double u, p, r, ap;
     };
@@ -165,13 +169,17 @@ DARCY_INLINE void run_problem(Shared& s, T* pool, float* tail, const Problem& pb
         T z;
         double ap;
     };
+    double alpha = 0.0;
+    bool pending = false;  // a deferred u += alpha p
     for (int it = 0; it < P.max_iter; ++it) {
-        // p = z + beta p, ap = A p, <p, ap>.
+        // p = z + beta p, ap = A p, <p, ap>; the pending u update.
         if constexpr (mixed)
-            cg_direction<1>(n, A0, x0f, pold, beta, first, pnew, h.ap, wave, kWaves, h.part);
+            cg_direction<1, defer>(n, A0, x0f, pold, beta, first, pnew, h.ap, wave, kWaves, h.part,
+                                   pending ? u : nullptr, alpha);
         else
-            cg_direction<1>(n, A0, static_cast<const double*>(h.z), pold, beta, first, pnew, h.ap, wave, kWaves,
-                            h.part);
+            cg_direction<1, defer>(n, A0, static_cast<const double*>(h.z), pold, beta, first, pnew, h.ap, wave,
+                                   kWaves, h.part);
+        pending = false;
         reduce_parts(s, 1, C);
         const double pap = s.red[0];
         if (!(fabs(pap) > 0.0) || !isfinite(pap)) {  // solver.hpp:149-155
@@ -179,12 +187,14 @@ DARCY_INLINE void run_problem(Shared& s, T* pool, float* tail, const Problem& pb
             iters = it;
             break;
         }
-        const double alpha = rz / pap;
-        // u += alpha p; r -= alpha ap; <r, r>  (blas.hpp:62-77)
+        alpha = rz / pap;
+        pending = defer;
+        // u += alpha p (unless deferred); r -= alpha ap; <r, r>  (blas.hpp:62-77)
         chunk_pass<1>(
-            N, wave, kWaves, h.part, [&](int k) { return Upd{u[k], pnew[k], h.r[k], h.ap[k]}; },
+            N, wave, kWaves, h.part,
+            [&](int k) { return defer ? Upd{0.0, 0.0, h.r[k], h.ap[k]} : Upd{u[k], pnew[k], h.r[k], h.ap[k]}; },
             [&](int k, const Upd& v, double* acc) {
-                u[k] = v.u + alpha * v.p;
+                if (!defer) u[k] = v.u + alpha * v.p;
                 const double ri = v.r - alpha * v.ap;
                 h.r[k] = ri;
                 acc[0] += ri * ri;
@@ -215,6 +225,9 @@ DARCY_INLINE void run_problem(Shared& s, T* pool, float* tail, const Problem& pb
         pold = pnew;
         pnew = t;
         first = false;
+    }
+    if (pending) {  // the last u += alpha p; reduce_parts ended with a barrier
+        for (int k = e.tid; k < N; k += e.nthr) u[k] = u[k] + alpha * pnew[k];
     }
     if (threadIdx.x == 0) st = ProblemStats{iters, status, levels, hist ? iters + 1 : 0, res, -1};
 }

@@ -299,6 +299,11 @@ __device__ inline double wave_sum(double s) {
     return __shfl(s, 0, kWave);
 }
 
+template <bool B>
+struct Flag {
+    static constexpr bool value = B;
+};
+
 // Canonical chunked reduction.  For each chunk owned by this wave, every
 // lane loads its four cells first (`load(k)` returns a value bundle), then
 // commits them in order (`commit(k, bundle, acc)` adds to acc[0..ND)).
@@ -338,21 +343,24 @@ DARCY_INLINE void chunk_pass(int N, int w, int nw, double* const* part, Load&& l
 // The CG direction pass: p = z + beta p_old, or p = z on the first iteration
 // (solver.hpp:171, blas.hpp:50-56), fused with ap = A p (solver.hpp:147) and
 // the chunk sums of <p, ap> (solver.hpp:148).  z is stored as Zt (float for
-// the FP32 cycle) and widened to double.
-// The first-iteration test stays outside the cell loop: inside it, the load
-// of p_old would be conditional, each stencil point would branch around its
-// load, and every load would wait for the one before.
-template <int CPR, class Zt>
+// the FP32 cycle) and widened to double.  With DEFER and a non-null `u`, the
+// previous iteration's u += alpha_u p_old (blas.hpp:62-67) is applied here
+// too, where p_old is read anyway, instead of in the update pass: the same
+// operation on the same values, one read of p fewer per iteration.
+template <int CPR, bool DEFER, class Zt>
 DARCY_INLINE void cg_direction(int n, const LvP<double>& A0, const Zt* z, const double* pold, double beta, bool first,
-                               double* pnew, double* ap, int w, int nw, double* const* part) {
+                               double* pnew, double* ap, int w, int nw, double* const* part, double* u = nullptr,
+                               double alpha_u = 0.0) {
     struct PAp {
-        double p, ap;
+        double p, ap, u;
     };
-    // `pv(t, loaded)` forms p at t.  All of a cell's loads are issued before
-    // any arithmetic (the scheduling barrier keeps them together): left to
-    // itself, the scheduler of this register-bound kernel waits for each
-    // stencil point's loads before issuing the next point's.
-    auto pass = [&](auto&& load, auto&& pv) {
+    // `load(t)` reads what p at t is formed from and `pv` forms it.  All of a
+    // cell's loads are issued before any arithmetic (the scheduling barrier
+    // keeps them together): left to itself, the scheduler of this
+    // register-bound kernel waits for each stencil point's loads before
+    // issuing the next point's.
+    auto pass = [&](auto&& load, auto&& pv, auto upd) {
+        constexpr bool U = decltype(upd)::value;
         chunk_pass<1, CPR>(
             n * n, w, nw, part,
             [&](int k) {
@@ -363,6 +371,8 @@ DARCY_INLINE void cg_direction(int n, const LvP<double>& A0, const Zt* z, const 
                 decltype(load(0)) l[5];
 #pragma unroll
                 for (int q = 0; q < 5; ++q) l[q] = load(t[q]);
+                double uk = 0.0;
+                if constexpr (U) uk = u[k];
                 __builtin_amdgcn_sched_barrier(0);
                 // apply_at's operations, in its order.
                 const double d = ((e + wf) + nn) + s;
@@ -372,11 +382,13 @@ DARCY_INLINE void cg_direction(int n, const LvP<double>& A0, const Zt* z, const 
                 a -= hW ? wf * xW : 0.0;
                 a -= hN ? nn * xN : 0.0;
                 a -= hS ? s * xS : 0.0;
-                return PAp{x0, a};
+                if constexpr (U) uk = uk + alpha_u * l[0].p;
+                return PAp{x0, a, uk};
             },
             [&](int k, const PAp& v, double* acc) {
                 pnew[k] = v.p;
                 ap[k] = v.ap;
+                if constexpr (U) u[k] = v.u;
                 acc[0] += v.p * v.ap;
             });
     };
@@ -384,11 +396,17 @@ DARCY_INLINE void cg_direction(int n, const LvP<double>& A0, const Zt* z, const 
         Zt z;
         double p;
     };
+    auto zp = [&](int t) { return ZP{z[t], pold[t]}; };
+    auto zpv = [&](const ZP& v) { return static_cast<double>(v.z) + beta * v.p; };
     if (first)
-        pass([&](int t) { return z[t]; }, [](Zt v) { return static_cast<double>(v); });
-    else
-        pass([&](int t) { return ZP{z[t], pold[t]}; },
-             [&](const ZP& v) { return static_cast<double>(v.z) + beta * v.p; });
+        pass([&](int t) { return z[t]; }, [](Zt v) { return static_cast<double>(v); }, Flag<false>{});
+    else if constexpr (DEFER) {
+        if (u)
+            pass(zp, zpv, Flag<true>{});
+        else
+            pass(zp, zpv, Flag<false>{});
+    } else
+        pass(zp, zpv, Flag<false>{});
 }
 
 // Lane l adds chunk sums l, l+64, l+128, ... in that order; the loads of

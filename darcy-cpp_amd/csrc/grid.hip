@@ -348,10 +348,10 @@ __global__ __launch_bounds__(kBlock) void kg_apply(GridCg g, const double* pold,
     const bool first = g.st->first != 0;
     const double beta = g.st->beta;
     if (g.mixed)
-        cg_direction<1>(A0.n, A0, static_cast<const float*>(h.x[0]), pold, beta, first, pnew, h.ap, grid_wave(),
+        cg_direction<1, false>(A0.n, A0, static_cast<const float*>(h.x[0]), pold, beta, first, pnew, h.ap, grid_wave(),
                         grid_waves(), h.part);
     else
-        cg_direction<1>(A0.n, A0, static_cast<const double*>(h.z), pold, beta, first, pnew, h.ap, grid_wave(),
+        cg_direction<1, false>(A0.n, A0, static_cast<const double*>(h.z), pold, beta, first, pnew, h.ap, grid_wave(),
                         grid_waves(), h.part);
 }
 

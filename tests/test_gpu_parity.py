@@ -102,6 +102,20 @@ def test_mixed_precision_solution(oracle):
     assert rel_l2(got.u, ref.u) <= 1e-8
 
 
+@pytest.mark.parametrize("cap", [1, 3])
+def test_mixed_precision_capped_iterations(oracle, rng, cap):
+    """The FP32-cycle kernel applies each u += alpha p one pass late; a solve
+    stopped by the iteration cap must still carry the last one.  Without it u
+    would be off by tens of percent; the two sides differ only in dot-product
+    rounding.  Cold and warm starts."""
+    a = two_phase(64, seed=5)
+    for u0 in (None, 1e-3 * rng.standard_normal((64, 64))):
+        got = darcy.solve(a, precision="mixed", tol=1e-14, max_iter=cap, u0=u0)
+        want = oracle.solve(a, tol=1e-14, max_iter=cap, fp32=True, u0=u0)
+        assert got.iterations == want.iterations == cap and not got.converged
+        assert rel_l2(got.u, want.u) <= 1e-6
+
+
 def test_uneven_batch_matches_oracle(oracle):
     sizes = [64, 17, 96, 33, 128, 5, 64]
     media = [two_phase(n, seed=i) for i, n in enumerate(sizes)]
