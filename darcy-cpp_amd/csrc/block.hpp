@@ -460,6 +460,10 @@ struct Cycle {
     T* pool;
     float* tail;  // FP32 kernel: tail matrix by columns, then a residual vector
     CycleOpts o;
+    // Cells per step of the row-pipelined global levels (rows_per_step):
+    // kStepCellsShared where many workgroups share each L2, kStepCellsAlone
+    // for a workgroup that runs alone.
+    int step_cells;
 
     DARCY_INLINE LvP<T> G(int l) const { return global_level<T>(h, l); }
     DARCY_INLINE LvP<T> S(int l) const {
@@ -526,9 +530,11 @@ struct Cycle {
         if (l >= t.first)
             visit_down(S(l), S(l + 1).b, fresh, o, e, sync);
         else if (l + 1 >= t.first)
-            o.pre == 1 ? down_stream(G(l), S(l + 1).b, fresh, e) : visit_down(G(l), S(l + 1).b, fresh, o, e, sync);
+            o.pre == 1 ? down_stream(G(l), S(l + 1).b, fresh, step_cells, e)
+                       : visit_down(G(l), S(l + 1).b, fresh, o, e, sync);
         else
-            o.pre == 1 ? down_stream(G(l), G(l + 1).b, fresh, e) : visit_down(G(l), G(l + 1).b, fresh, o, e, sync);
+            o.pre == 1 ? down_stream(G(l), G(l + 1).b, fresh, step_cells, e)
+                       : visit_down(G(l), G(l + 1).b, fresh, o, e, sync);
     }
 
     DARCY_INLINE void up(int l, Exec e) const {
@@ -536,10 +542,10 @@ struct Cycle {
         if (l >= t.first)
             visit_up(S(l), S(l + 1).x, o, e, sync);
         else if (l + 1 >= t.first)
-            o.post == 1 ? up_stream(G(l), static_cast<const T*>(S(l + 1).x), e)
+            o.post == 1 ? up_stream(G(l), static_cast<const T*>(S(l + 1).x), step_cells, e)
                         : visit_up(G(l), static_cast<const T*>(S(l + 1).x), o, e, sync);
         else
-            o.post == 1 ? up_stream(G(l), static_cast<const T*>(G(l + 1).x), e)
+            o.post == 1 ? up_stream(G(l), static_cast<const T*>(G(l + 1).x), step_cells, e)
                         : visit_up(G(l), static_cast<const T*>(G(l + 1).x), o, e, sync);
     }
 

@@ -168,7 +168,7 @@ __global__ __launch_bounds__(kTailThreads) void kg_tail(const Hier* hd, int l0, 
         }
     }
     __syncthreads();
-    const Cycle<T> cyc{h, s.t, pool, tail, o};
+    const Cycle<T> cyc{h, s.t, pool, tail, o, kStepCellsAlone};
     cyc.run(l0, fresh != 0, e);
     if (s.t.first <= l0) {  // the cycle ends with a barrier
         const LvP<T> g = global_level<T>(h, l0);
