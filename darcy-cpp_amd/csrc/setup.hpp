@@ -11,84 +11,109 @@ namespace darcy_hip {
 // level.hpp:57: 2xy / (x + y), evaluated as (2x)y / (x + y).
 __device__ inline double hmean(double x, double y) { return 2.0 * x * y / (x + y); }
 
+// out[q] = value(q) for q in [0, count): four items per thread and step,
+// the loads of all four issued before the first store (`value` sees only
+// in-range indices: the items past `count` recompute the last one and are not
+// stored).  The values read by the setup loops below are clamped to the grid
+// and their boundary cases selected afterwards, so no load is conditional.
+template <class T, class F>
+DARCY_INLINE void map4(int count, T* __restrict__ out, Exec e, F&& value) {
+    for (int q0 = e.tid; q0 < count; q0 += 4 * e.nthr) {
+        T v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int q = q0 + u * e.nthr;
+            v[u] = value(q < count ? q : count - 1);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (q0 + u * e.nthr < count) out[q0 + u * e.nthr] = v[u];
+    }
+}
+
 // Fine faces, level.hpp:83-91 in face form: the interior face between cells
 // k-1 and k is the e of cell k-1, hmean(a[k-1], a[k]); boundary faces carry
 // the cell's own permeability.
 __device__ inline void build_fine_faces(const double* __restrict__ a, int n, double* __restrict__ ex,
                                         double* __restrict__ ny, Exec e) {
     const int fx = n * (n + 1);
-    for (int q = e.tid; q < fx; q += e.nthr) {
+    map4(fx, ex, e, [&](int q) {
         const int i = q / (n + 1), j = q - i * (n + 1);
         const double* row = a + static_cast<size_t>(i) * n;
-        ex[q] = (j == 0) ? row[0] : (j == n) ? row[n - 1] : hmean(row[j - 1], row[j]);
-    }
-    for (int q = e.tid; q < fx; q += e.nthr) {
+        const double xl = row[j > 0 ? j - 1 : 0], xr = row[j < n ? j : n - 1];
+        return (j == 0) ? xr : (j == n) ? xl : hmean(xl, xr);
+    });
+    map4(fx, ny, e, [&](int q) {
         const int i = q / n, j = q - i * n;
-        ny[q] = (i == 0) ? a[j]
-              : (i == n) ? a[static_cast<size_t>(n - 1) * n + j]
-                         : hmean(a[static_cast<size_t>(i - 1) * n + j], a[static_cast<size_t>(i) * n + j]);
-    }
+        const double up = a[static_cast<size_t>(i > 0 ? i - 1 : 0) * n + j];
+        const double dn = a[static_cast<size_t>(i < n ? i : n - 1) * n + j];
+        return (i == 0) ? dn : (i == n) ? up : hmean(up, dn);
+    });
 }
 
 // inv_diag of the fine level (level.hpp:88-94), recomputed from a so that it
 // needs no barrier after the faces.
 __device__ inline void build_fine_invdiag(const double* __restrict__ a, int n, double* __restrict__ id, Exec e) {
-    const int N = n * n;
-    for (int k = e.tid; k < N; k += e.nthr) {
+    map4(n * n, id, e, [&](int k) {
         const int i = k / n, j = k - i * n;
-        const double c = a[k];
-        const double fe = (j + 1 < n) ? hmean(c, a[k + 1]) : c;
-        const double fw = (j > 0) ? hmean(c, a[k - 1]) : c;
-        const double fn = (i + 1 < n) ? hmean(c, a[k + n]) : c;
-        const double fs = (i > 0) ? hmean(c, a[k - n]) : c;
-        id[k] = 1.0 / (((fe + fw) + fn) + fs);
-    }
+        const bool hE = j + 1 < n, hW = j > 0, hN = i + 1 < n, hS = i > 0;
+        const double c = a[k], aE = a[hE ? k + 1 : k], aW = a[hW ? k - 1 : k], aN = a[hN ? k + n : k],
+                     aS = a[hS ? k - n : k];
+        const double fe = hE ? hmean(c, aE) : c;
+        const double fw = hW ? hmean(c, aW) : c;
+        const double fn = hN ? hmean(c, aN) : c;
+        const double fs = hS ? hmean(c, aS) : c;
+        return 1.0 / (((fe + fw) + fn) + fs);
+    });
 }
 
 // Coarse faces (level.hpp:171-211 in face form).  Coarse ex column J sums
 // the fine ex column min(2J, n) over the aggregate's rows; coarse ny row I
-// sums the fine ny row min(2I, n) over the aggregate's columns.
+// sums the fine ny row min(2I, n) over the aggregate's columns.  An
+// aggregate has one or two rows (columns); the sum starts from 0.0 and adds
+// them in order.
 __device__ inline void galerkin_faces(int n, const double* __restrict__ ex, const double* __restrict__ ny,
                                       double* __restrict__ cex, double* __restrict__ cny, Exec e) {
     const int m = (n + 1) >> 1, fx = m * (m + 1);
-    for (int q = e.tid; q < fx; q += e.nthr) {
+    map4(fx, cex, e, [&](int q) {
         const int I = q / (m + 1), J = q - I * (m + 1);
         const int i0 = 2 * I, i1 = (2 * I + 1 < n) ? 2 * I + 1 : 2 * I;
         const int col = (2 * J < n) ? 2 * J : n;
-        double s = 0.0;
-        for (int i = i0; i <= i1; ++i) s += ex[i * (n + 1) + col];
-        cex[q] = s;
-    }
-    for (int q = e.tid; q < fx; q += e.nthr) {
+        const double v0 = ex[i0 * (n + 1) + col], v1 = ex[i1 * (n + 1) + col];
+        const double s = 0.0 + v0;
+        return i1 != i0 ? s + v1 : s;
+    });
+    map4(fx, cny, e, [&](int q) {
         const int I = q / m, J = q - I * m;
         const int j0 = 2 * J, j1 = (2 * J + 1 < n) ? 2 * J + 1 : 2 * J;
         const int row = (2 * I < n) ? 2 * I : n;
-        double s = 0.0;
-        for (int j = j0; j <= j1; ++j) s += ny[row * n + j];
-        cny[q] = s;
-    }
+        const double v0 = ny[row * n + j0], v1 = ny[row * n + j1];
+        const double s = 0.0 + v0;
+        return j1 != j0 ? s + v1 : s;
+    });
 }
 
 // Coarse inv_diag from the fine faces directly (no barrier after
 // galerkin_faces): d = ((ce + cw) + cn) + cs, level.hpp:205-207.
 __device__ inline void galerkin_invdiag(int n, const double* __restrict__ ex, const double* __restrict__ ny,
                                         double* __restrict__ cid, Exec e) {
-    const int m = (n + 1) >> 1, M = m * m;
-    for (int q = e.tid; q < M; q += e.nthr) {
+    const int m = (n + 1) >> 1;
+    map4(m * m, cid, e, [&](int q) {
         const int I = q / m, J = q - I * m;
         const int i0 = 2 * I, i1 = (2 * I + 1 < n) ? 2 * I + 1 : 2 * I;
         const int j0 = 2 * J, j1 = (2 * J + 1 < n) ? 2 * J + 1 : 2 * J;
-        double ce = 0.0, cw = 0.0, cn = 0.0, cs = 0.0;
-        for (int i = i0; i <= i1; ++i) {
-            ce += ex[i * (n + 1) + j1 + 1];
-            cw += ex[i * (n + 1) + j0];
-        }
-        for (int j = j0; j <= j1; ++j) {
-            cn += ny[(i1 + 1) * n + j];
-            cs += ny[i0 * n + j];
-        }
-        cid[q] = 1.0 / (((ce + cw) + cn) + cs);
-    }
+        const bool two_i = i1 != i0, two_j = j1 != j0;
+        const double e0 = ex[i0 * (n + 1) + j1 + 1], e1 = ex[i1 * (n + 1) + j1 + 1];
+        const double w0 = ex[i0 * (n + 1) + j0], w1 = ex[i1 * (n + 1) + j0];
+        const double n0 = ny[(i1 + 1) * n + j0], n1 = ny[(i1 + 1) * n + j1];
+        const double s0 = ny[i0 * n + j0], s1 = ny[i0 * n + j1];
+        double ce = 0.0 + e0, cw = 0.0 + w0, cn = 0.0 + n0, cs = 0.0 + s0;
+        ce = two_i ? ce + e1 : ce;
+        cw = two_i ? cw + w1 : cw;
+        cn = two_j ? cn + n1 : cn;
+        cs = two_j ? cs + s1 : cs;
+        return 1.0 / (((ce + cw) + cn) + cs);
+    });
 }
 
 // FP32 copies of one level's operator for the mixed-precision cycle.  The
@@ -98,11 +123,9 @@ __device__ inline void to_fp32(int n, const double* __restrict__ ex, const doubl
                                const double* __restrict__ id, float* __restrict__ ex32, float* __restrict__ ny32,
                                float* __restrict__ id32, Exec e) {
     const int fx = n * (n + 1), N = n * n;
-    for (int q = e.tid; q < fx; q += e.nthr) {
-        ex32[q] = static_cast<float>(ex[q]);
-        ny32[q] = static_cast<float>(ny[q]);
-    }
-    for (int q = e.tid; q < N; q += e.nthr) id32[q] = static_cast<float>(id[q]);
+    map4(fx, ex32, e, [&](int q) { return static_cast<float>(ex[q]); });
+    map4(fx, ny32, e, [&](int q) { return static_cast<float>(ny[q]); });
+    map4(N, id32, e, [&](int q) { return static_cast<float>(id[q]); });
 }
 
 // Smallest flat index with !(a > 0) || !isfinite(a) among this thread's
