@@ -41,20 +41,18 @@ DARCY_INLINE void run_problem(Shared& s, T* pool, float* tail, const Problem& pb
     __syncthreads();
     const Hier& h = s.h;
 
-    // Hierarchy (level.hpp:79-97, multigrid.hpp:55-71, level.hpp:171-211).
-    build_fine_faces(pb.a, n, h.ex[0], h.ny[0], e);
-    build_fine_invdiag(pb.a, n, h.id[0], e);
+    // Hierarchy (level.hpp:79-97, multigrid.hpp:55-71, level.hpp:171-211),
+    // with the FP32 cycle's operator copies written where each level's FP64
+    // values are formed.
+    const bool c32 = mixed && mg;
+    build_fine_faces(pb.a, n, h.ex[0], h.ny[0], e, c32 ? h.ex32[0] : nullptr, c32 ? h.ny32[0] : nullptr);
+    build_fine_invdiag(pb.a, n, h.id[0], e, c32 ? h.id32[0] : nullptr);
     __syncthreads();
     if (mg) {
         for (int l = 0; l + 1 < h.L; ++l) {
-            galerkin_faces(h.n[l], h.ex[l], h.ny[l], h.ex[l + 1], h.ny[l + 1], e);
-            galerkin_invdiag(h.n[l], h.ex[l], h.ny[l], h.id[l + 1], e);
-            if (mixed) to_fp32(h.n[l], h.ex[l], h.ny[l], h.id[l], h.ex32[l], h.ny32[l], h.id32[l], e);
-            __syncthreads();
-        }
-        if (mixed) {
-            const int l = h.L - 1;
-            to_fp32(h.n[l], h.ex[l], h.ny[l], h.id[l], h.ex32[l], h.ny32[l], h.id32[l], e);
+            galerkin_faces(h.n[l], h.ex[l], h.ny[l], h.ex[l + 1], h.ny[l + 1], e, c32 ? h.ex32[l + 1] : nullptr,
+                           c32 ? h.ny32[l + 1] : nullptr);
+            galerkin_invdiag(h.n[l], h.ex[l], h.ny[l], h.id[l + 1], e, c32 ? h.id32[l + 1] : nullptr);
             __syncthreads();
         }
         if constexpr (mixed) {

@@ -16,8 +16,10 @@ __device__ inline double hmean(double x, double y) { return 2.0 * x * y / (x + y
 // in-range indices: the items past `count` recompute the last one and are not
 // stored).  The values read by the setup loops below are clamped to the grid
 // and their boundary cases selected afterwards, so no load is conditional.
+// With `out32`, the value rounded to FP32 is stored there too (the operator
+// copies of the FP32 cycle, written where the FP64 values are formed).
 template <class T, class F>
-DARCY_INLINE void map4(int count, T* __restrict__ out, Exec e, F&& value) {
+DARCY_INLINE void map4(int count, T* __restrict__ out, Exec e, F&& value, float* __restrict__ out32 = nullptr) {
     for (int q0 = e.tid; q0 < count; q0 += 4 * e.nthr) {
         T v[4];
 #pragma unroll
@@ -27,7 +29,10 @@ DARCY_INLINE void map4(int count, T* __restrict__ out, Exec e, F&& value) {
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u)
-            if (q0 + u * e.nthr < count) out[q0 + u * e.nthr] = v[u];
+            if (q0 + u * e.nthr < count) {
+                out[q0 + u * e.nthr] = v[u];
+                if (out32) out32[q0 + u * e.nthr] = static_cast<float>(v[u]);
+            }
     }
 }
 
@@ -35,25 +40,27 @@ DARCY_INLINE void map4(int count, T* __restrict__ out, Exec e, F&& value) {
 // k-1 and k is the e of cell k-1, hmean(a[k-1], a[k]); boundary faces carry
 // the cell's own permeability.
 __device__ inline void build_fine_faces(const double* __restrict__ a, int n, double* __restrict__ ex,
-                                        double* __restrict__ ny, Exec e) {
+                                        double* __restrict__ ny, Exec e, float* ex32 = nullptr,
+                                        float* ny32 = nullptr) {
     const int fx = n * (n + 1);
     map4(fx, ex, e, [&](int q) {
         const int i = q / (n + 1), j = q - i * (n + 1);
         const double* row = a + static_cast<size_t>(i) * n;
         const double xl = row[j > 0 ? j - 1 : 0], xr = row[j < n ? j : n - 1];
         return (j == 0) ? xr : (j == n) ? xl : hmean(xl, xr);
-    });
+    }, ex32);
     map4(fx, ny, e, [&](int q) {
         const int i = q / n, j = q - i * n;
         const double up = a[static_cast<size_t>(i > 0 ? i - 1 : 0) * n + j];
         const double dn = a[static_cast<size_t>(i < n ? i : n - 1) * n + j];
         return (i == 0) ? dn : (i == n) ? up : hmean(up, dn);
-    });
+    }, ny32);
 }
 
 // inv_diag of the fine level (level.hpp:88-94), recomputed from a so that it
 // needs no barrier after the faces.
-__device__ inline void build_fine_invdiag(const double* __restrict__ a, int n, double* __restrict__ id, Exec e) {
+__device__ inline void build_fine_invdiag(const double* __restrict__ a, int n, double* __restrict__ id, Exec e,
+                                          float* id32 = nullptr) {
     map4(n * n, id, e, [&](int k) {
         const int i = k / n, j = k - i * n;
         const bool hE = j + 1 < n, hW = j > 0, hN = i + 1 < n, hS = i > 0;
@@ -64,7 +71,7 @@ __device__ inline void build_fine_invdiag(const double* __restrict__ a, int n, d
         const double fn = hN ? hmean(c, aN) : c;
         const double fs = hS ? hmean(c, aS) : c;
         return 1.0 / (((fe + fw) + fn) + fs);
-    });
+    }, id32);
 }
 
 // Coarse faces (level.hpp:171-211 in face form).  Coarse ex column J sums
@@ -73,7 +80,8 @@ __device__ inline void build_fine_invdiag(const double* __restrict__ a, int n, d
 // aggregate has one or two rows (columns); the sum starts from 0.0 and adds
 // them in order.
 __device__ inline void galerkin_faces(int n, const double* __restrict__ ex, const double* __restrict__ ny,
-                                      double* __restrict__ cex, double* __restrict__ cny, Exec e) {
+                                      double* __restrict__ cex, double* __restrict__ cny, Exec e,
+                                      float* cex32 = nullptr, float* cny32 = nullptr) {
     const int m = (n + 1) >> 1, fx = m * (m + 1);
     map4(fx, cex, e, [&](int q) {
         const int I = q / (m + 1), J = q - I * (m + 1);
@@ -82,7 +90,7 @@ __device__ inline void galerkin_faces(int n, const double* __restrict__ ex, cons
         const double v0 = ex[i0 * (n + 1) + col], v1 = ex[i1 * (n + 1) + col];
         const double s = 0.0 + v0;
         return i1 != i0 ? s + v1 : s;
-    });
+    }, cex32);
     map4(fx, cny, e, [&](int q) {
         const int I = q / m, J = q - I * m;
         const int j0 = 2 * J, j1 = (2 * J + 1 < n) ? 2 * J + 1 : 2 * J;
@@ -90,13 +98,13 @@ __device__ inline void galerkin_faces(int n, const double* __restrict__ ex, cons
         const double v0 = ny[row * n + j0], v1 = ny[row * n + j1];
         const double s = 0.0 + v0;
         return j1 != j0 ? s + v1 : s;
-    });
+    }, cny32);
 }
 
 // Coarse inv_diag from the fine faces directly (no barrier after
 // galerkin_faces): d = ((ce + cw) + cn) + cs, level.hpp:205-207.
 __device__ inline void galerkin_invdiag(int n, const double* __restrict__ ex, const double* __restrict__ ny,
-                                        double* __restrict__ cid, Exec e) {
+                                        double* __restrict__ cid, Exec e, float* cid32 = nullptr) {
     const int m = (n + 1) >> 1;
     map4(m * m, cid, e, [&](int q) {
         const int I = q / m, J = q - I * m;
@@ -113,7 +121,7 @@ __device__ inline void galerkin_invdiag(int n, const double* __restrict__ ex, co
         cn = two_j ? cn + n1 : cn;
         cs = two_j ? cs + s1 : cs;
         return 1.0 / (((ce + cw) + cn) + cs);
-    });
+    }, cid32);
 }
 
 // FP32 copies of one level's operator for the mixed-precision cycle.  The
