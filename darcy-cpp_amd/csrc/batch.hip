@@ -104,27 +104,35 @@ DARCY_INLINE void run_problem(Shared& s, T* pool, float* tail, const Problem& pb
     struct Init {
         double bk, rk, u0;
     };
-    chunk_pass<2>(
-        N, wave, kWaves, h.part,
-        [&](int k) {
-            Init v;
-            v.bk = (pb.rhs ? pb.rhs[k] : pb.f) * h2;
-            v.rk = v.bk;
-            v.u0 = 0.0;
-            if (pb.in) {
-                const int i = k / n, j = k - i * n;
-                v.rk = v.bk - apply_at(n, A0.ex, A0.ny, i, j, [&](int t) { return pb.in[t]; });
-                v.u0 = pb.in[k];
-            }
-            return v;
-        },
-        [&](int k, const Init& v, double* acc) {
-            acc[0] += v.bk * v.bk;
-            u[k] = v.u0;
-            h.r[k] = v.rk;
-            acc[1] += v.rk * v.rk;
-            if (mixed) b0f[k] = static_cast<float>(v.rk);
-        });
+    // The warm-start test stays outside the cell loop (a conditional stencil
+    // would branch around its loads).
+    auto init = [&](auto warm) {
+        chunk_pass<2>(
+            N, wave, kWaves, h.part,
+            [&](int k) {
+                Init v;
+                v.bk = (pb.rhs ? pb.rhs[k] : pb.f) * h2;
+                v.rk = v.bk;
+                v.u0 = 0.0;
+                if constexpr (decltype(warm)::value) {
+                    const int i = k / n, j = k - i * n;
+                    v.rk = v.bk - apply_at(n, A0.ex, A0.ny, i, j, [&](int t) { return pb.in[t]; });
+                    v.u0 = pb.in[k];
+                }
+                return v;
+            },
+            [&](int k, const Init& v, double* acc) {
+                acc[0] += v.bk * v.bk;
+                u[k] = v.u0;
+                h.r[k] = v.rk;
+                acc[1] += v.rk * v.rk;
+                if (mixed) b0f[k] = static_cast<float>(v.rk);
+            });
+    };
+    if (pb.in)
+        init(Flag<true>{});
+    else
+        init(Flag<false>{});
     reduce_parts(s, 2, C);
     const double bnorm = sqrt(s.red[0]);
     const int levels = mg ? h.L : 1;

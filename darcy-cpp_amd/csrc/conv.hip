@@ -754,9 +754,17 @@ __global__ __launch_bounds__(kT) void k_head_fwd(const bf16* x, Geo in, const fl
 // banks), 288 fused multiply-adds per pixel.  Sum order: bias, then taps,
 // then channels.  The bf16 patch (20 KB) leaves room for 7 workgroups per CU,
 // which the staging loads need more than the arithmetic needs FP32 operands.
+// With `out.u0` the kernel stores the warm start itself, u0 = f scale pos y
+// in FP64 as k_output forms it, instead of y.
+struct HeadOut {
+    const float* pos;
+    const double* f;
+    double scale;
+    double* const* u0;
+};
 constexpr int kHeadTile = 16, kHeadC = 32, kHeadPad = 40;
 __global__ __launch_bounds__(kT) void k_head_fwd_tiled(const bf16* x, Geo in, const float* w, const float* bias,
-                                                       float* y) {
+                                                       float* y, HeadOut out) {
     __shared__ __align__(16) bf16 tile[(kHeadTile + 2) * (kHeadTile + 2) * kHeadPad];
     __shared__ float ws[9 * kHeadC + 1];
     const int tx0 = blockIdx.x * kHeadTile, ty0 = blockIdx.y * kHeadTile, b = blockIdx.z;
@@ -785,7 +793,13 @@ __global__ __launch_bounds__(kT) void k_head_fwd_tiled(const bf16* x, Geo in, co
             for (int e = 0; e < 8; ++e) s = fmaf(wt[ci + e], bf2f(v[e]), s);
         }
     }
-    if (oy < in.h && ox < in.w) y[(static_cast<size_t>(b) * in.h + oy) * in.w + ox] = s;
+    if (oy < in.h && ox < in.w) {
+        const size_t k = static_cast<size_t>(oy) * in.w + ox;
+        if (out.u0)
+            out.u0[b][k] = out.f[b] * out.scale * static_cast<double>(out.pos[k] * s);
+        else
+            y[static_cast<size_t>(b) * in.h * in.w + k] = s;
+    }
 }
 
 // dX[pixel][ci] = sum_tap w[tap][ci] dy[pixel - tap].
@@ -1114,7 +1128,8 @@ hipError_t darcy_nn_head_fwd(const void* x, int b, int h, int w, int cin, const 
                              hipStream_t s) {
     if (cin == kHeadC) {
         const dim3 grid((w + kHeadTile - 1) / kHeadTile, (h + kHeadTile - 1) / kHeadTile, b);
-        k_head_fwd_tiled<<<grid, kT, 0, s>>>(static_cast<const bf16*>(x), Geo{b, h, w, cin}, wgt, bias, y);
+        k_head_fwd_tiled<<<grid, kT, 0, s>>>(static_cast<const bf16*>(x), Geo{b, h, w, cin}, wgt, bias, y,
+                                             HeadOut{nullptr, nullptr, 0.0, nullptr});
         return hipGetLastError();
     }
     k_head_fwd<<<ew_grid(static_cast<size_t>(b) * h * w), kT, 0, s>>>(static_cast<const bf16*>(x), Geo{b, h, w, cin}, wgt,
@@ -1188,6 +1203,21 @@ hipError_t darcy_nn_output(const float* y, const float* pos, int n, int batch, c
                            double* const* u0, hipStream_t s) {
     k_output<<<ew_grid(static_cast<size_t>(n) * n * batch), kT, 0, s>>>(y, pos, n, batch, f, scale, u0);
     return hipGetLastError();
+}
+
+// The head and the output in one: u0 = f scale pos head(x) for n x n
+// problems (the 32-channel head stores u0 itself; other widths go through y).
+hipError_t darcy_nn_head_output(const void* x, int b, int n, int cin, const float* wgt, const float* bias, float* y,
+                                const float* pos, const double* f, double scale, double* const* u0, hipStream_t s) {
+    if (cin == kHeadC) {
+        const dim3 grid((n + kHeadTile - 1) / kHeadTile, (n + kHeadTile - 1) / kHeadTile, b);
+        k_head_fwd_tiled<<<grid, kT, 0, s>>>(static_cast<const bf16*>(x), Geo{b, n, n, cin}, wgt, bias, y,
+                                             HeadOut{pos, f, scale, u0});
+        return hipGetLastError();
+    }
+    const hipError_t e = darcy_nn_head_fwd(x, b, n, n, cin, wgt, bias, y, s);
+    if (e != hipSuccess) return e;
+    return darcy_nn_output(y, pos, n, b, f, scale, u0, s);
 }
 
 // partial: `blocks` floats.

@@ -41,6 +41,8 @@ hipError_t darcy_nn_add(void* dst, const void* src, size_t count, hipStream_t s)
 hipError_t darcy_nn_transpose(const void* in, int P, int C, void* out, hipStream_t s);
 hipError_t darcy_nn_output(const float* y, const float* pos, int n, int batch, const double* f, double scale,
                            double* const* u0, hipStream_t s);
+hipError_t darcy_nn_head_output(const void* x, int b, int n, int cin, const float* wgt, const float* bias, float* y,
+                                const float* pos, const double* f, double scale, double* const* u0, hipStream_t s);
 hipError_t darcy_nn_loss(const float* y, const double* const* u, const double* const* a, const float* pos, int n,
                          int batch, double scale, float lam, float* dy, float* partial, int blocks, hipStream_t s);
 hipError_t darcy_nn_adam(float* w, const float* g, float* m, float* v, void* wb, size_t count, float lr, float b1,
@@ -140,7 +142,7 @@ struct darcy_net {
     int alloc();
     int refresh_weights();
     int upload_tables(const void* const* a, const void* const* u, const double* f, int64_t count);
-    int forward(int b, int64_t offset);
+    int forward(int b, int64_t offset, bool to_u0 = false);
     int backward(int b, int64_t offset, double* loss);
     int wgrad(int c, const void* x, const void* dz, int b);
     void release();
@@ -237,8 +239,9 @@ int darcy_net::upload_tables(const void* const* a, const void* const* u, const d
     return DARCY_OK;
 }
 
-// Forward pass of problems [offset, offset + b) of the uploaded tables.
-int darcy_net::forward(int b, int64_t offset) {
+// Forward pass of problems [offset, offset + b) of the uploaded tables: the
+// head's output y, or with `to_u0` the warm starts u0 themselves.
+int darcy_net::forward(int b, int64_t offset, bool to_u0) {
     hipStream_t s = ctx->stream;
     NN_TRY(darcy_nn_features(tab_a.as<const double*>() + offset, pos.as<float>(), n, b, feat.ptr, s), "net features");
     const void* x = feat.ptr;
@@ -257,9 +260,16 @@ int darcy_net::forward(int b, int64_t offset) {
                "net decoder");
         x = am[i].ptr;
     }
-    NN_TRY(darcy_nn_head_fwd(x, b, n, n, kHeadCin, params.as<float>() + kLayout.w[kConvs],
-                             params.as<float>() + kLayout.b[kConvs], y.as<float>(), s),
-           "net head");
+    if (to_u0) {
+        NN_TRY(darcy_nn_head_output(x, b, n, kHeadCin, params.as<float>() + kLayout.w[kConvs],
+                                    params.as<float>() + kLayout.b[kConvs], y.as<float>(), pos.as<float>(),
+                                    tab_f.as<double>() + offset, kScale, tab_u.as<double*>() + offset, s),
+               "net head");
+    } else {
+        NN_TRY(darcy_nn_head_fwd(x, b, n, n, kHeadCin, params.as<float>() + kLayout.w[kConvs],
+                                 params.as<float>() + kLayout.b[kConvs], y.as<float>(), s),
+               "net head");
+    }
     return DARCY_OK;
 }
 
@@ -421,10 +431,7 @@ extern "C" int darcy_net_predict(darcy_net* net, int64_t count, const double* co
     NN_TRY(hipEventRecord(ctx->ev0, ctx->stream), "hipEventRecord");
     for (int64_t off = 0; off < count; off += net->max_batch) {
         const int b = static_cast<int>(count - off < net->max_batch ? count - off : net->max_batch);
-        if (int r = net->forward(b, off)) return r;
-        NN_TRY(darcy_nn_output(net->y.as<float>(), net->pos.as<float>(), net->n, b, net->tab_f.as<double>() + off,
-                               kScale, net->tab_u.as<double*>() + off, ctx->stream),
-               "net output");
+        if (int r = net->forward(b, off, true)) return r;
     }
     NN_TRY(hipEventRecord(ctx->ev1, ctx->stream), "hipEventRecord");
     NN_TRY(hipEventSynchronize(ctx->ev1), "net predict");
