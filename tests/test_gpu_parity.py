@@ -116,6 +116,41 @@ def test_mixed_precision_capped_iterations(oracle, rng, cap):
         assert rel_l2(got.u, want.u) <= 1e-6
 
 
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_uneven_warm_batch_matches_oracle(oracle, rng, precision):
+    """Uneven sizes (tiny, odd, even) from random warm starts in one launch:
+    the warm-start init pass, the hierarchy setup at odd sizes and, with the
+    FP32 cycle, the deferred u update; each problem as the oracle solves it
+    (same cycle arithmetic; dot products summed in another order)."""
+    from darcy import capi
+
+    sizes = [1, 2, 3, 5, 17, 33, 64]
+    media = [two_phase(n, seed=40 + i) for i, n in enumerate(sizes)]
+    u0s = [1e-3 * rng.standard_normal((n, n)) for n in sizes]
+    with capi.Context() as ctx:
+        bufs = [(ctx.alloc(a.nbytes), ctx.alloc(a.nbytes), ctx.alloc(a.nbytes)) for a in media]
+        try:
+            for (pa, _, p0), a, u0 in zip(bufs, media, u0s):
+                ctx.upload(pa, a)
+                ctx.upload(p0, u0)
+            res = ctx.solve_batch_device(sizes, [b[0] for b in bufs], [b[1] for b in bufs],
+                                         u0_ptrs=[b[2] for b in bufs], precision=precision)
+            us = []
+            for (_, pu, _), a in zip(bufs, media):
+                u = np.empty_like(a)
+                ctx.download(u, pu)
+                us.append(u)
+        finally:
+            for b in bufs:
+                for p in b:
+                    ctx.free(p)
+    assert res["converged"].all()
+    for a, u0, u, it in zip(media, u0s, us, res["iterations"]):
+        want = oracle.solve(a, u0=u0, fp32=precision == "mixed")
+        assert abs(int(it) - want.iterations) <= 1
+        assert rel_l2(u, want.u) <= 1e-9
+
+
 def test_uneven_batch_matches_oracle(oracle):
     sizes = [64, 17, 96, 33, 128, 5, 64]
     media = [two_phase(n, seed=i) for i, n in enumerate(sizes)]
