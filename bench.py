@@ -52,8 +52,10 @@ SEED0 = 20000
 # read or write counted once at its storage size; the operator as two face
 # arrays per level; halo re-reads and scratch not counted).  The FP32-cycle
 # kernel applies u += alpha p in the next direction pass, which reads p
-# anyway: 8 bytes fewer than the FP64 layout of the same passes.
-B_ITER = {("fp64", 2): 324.0, ("fp64", 1): 248.0, ("mixed", 2): 222.0, ("mixed", 1): 184.0}
+# anyway: 8 bytes fewer than the FP64 layout of the same passes.  The
+# smoothers form inv_diag from the faces they read, so the cycle reads no
+# inv_diag array (2 reads per visit fewer: 2 x 2 s for W, 4/3 x 2 s for V).
+B_ITER = {("fp64", 2): 292.0, ("fp64", 1): 226.7, ("mixed", 2): 206.0, ("mixed", 1): 173.3}
 B_SETUP = {"fp64": 51.0, "mixed": 62.0}
 
 

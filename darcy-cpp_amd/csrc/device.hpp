@@ -104,6 +104,22 @@ DARCY_INLINE T apply_at(int n, const T* ex, const T* ny, int i, int j, X&& x) {
     return v;
 }
 
+// The reciprocal diagonal from the cell's four faces: ((e + w) + n) + s
+// (level.hpp:92), then 1 / d in T.  In FP64 this is the stored inv_diag bit
+// for bit (the faces are the ones it was formed from, and the harmonic mean
+// is symmetric bit for bit); in FP32 it is the FP32 cycle's definition, which
+// its stored copy follows too.  Forming it saves the kernels its load.
+template <class T>
+DARCY_INLINE T inv_from_faces(T e, T w, T n, T s) {
+    return T(1) / (((e + w) + n) + s);
+}
+
+template <class T>
+DARCY_INLINE T inv_diag_at(const LvP<T>& L, int i, int j) {
+    const int n = L.n, f = i * (n + 1) + j;
+    return inv_from_faces(L.ex[f + 1], L.ex[f], L.ny[(i + 1) * n + j], L.ny[i * n + j]);
+}
+
 struct Ident {
     template <class V>
     DARCY_INLINE V operator()(const V& v) const {
@@ -124,8 +140,8 @@ DARCY_INLINE T gs_at(const LvP<T>& L, int i, int j, T bk, Ld&& ld, Cv&& cv = Cv{
     const T e = L.ex[f + 1], w = L.ex[f], nn = L.ny[(i + 1) * n + j], s = L.ny[i * n + j];
     const auto rE = ld(i, hE ? j + 1 : j), rW = ld(i, hW ? j - 1 : j), rN = ld(hN ? i + 1 : i, j),
                rS = ld(hS ? i - 1 : i, j);
-    const T inv = L.id[i * n + j];
     __builtin_amdgcn_sched_barrier(0);
+    const T inv = inv_from_faces(e, w, nn, s);
     const T xE = cv(rE), xW = cv(rW), xN = cv(rN), xS = cv(rS);
     T v = bk;
     v += hE ? e * xE : T(0);
@@ -466,14 +482,14 @@ DARCY_INLINE void down_stream(const LvP<T>& L, T* bc, bool fresh, int step_cells
     const int n = L.n, m = (n + 1) >> 1, R = rows_per_step(n, step_cells);
     T* x = L.x;
     const T* b = L.b;
-    const T* id = L.id;
     auto xv = [&](int a, int c) { return x[a * n + c]; };
     int coarse_done = 0;
     for (int r0 = 0;; r0 += R) {
         const int red_hi = r0 + R < n ? r0 + R : n;
         if (r0 < n) {
             if (fresh)  // the red sweep from zero: b * inv_d (each neighbour term is e * 0)
-                update_rows(x, n, 0, r0, red_hi, e, [&](int i, int j) { return b[i * n + j] * id[i * n + j]; });
+                update_rows(x, n, 0, r0, red_hi, e,
+                            [&](int i, int j) { return b[i * n + j] * inv_diag_at(L, i, j); });
             else
                 update_rows(x, n, 0, r0, red_hi, e, [&](int i, int j) { return gs_at(L, i, j, b[i * n + j], xv); });
         }

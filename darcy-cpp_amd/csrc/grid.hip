@@ -37,7 +37,7 @@ __global__ __launch_bounds__(kBlock) void kg_galerkin(const Hier* hd, int l) {
 
 __global__ __launch_bounds__(kBlock) void kg_fp32(const Hier* hd, int l) {
     const Exec e = grid_exec();
-    to_fp32(hd->n[l], hd->ex[l], hd->ny[l], hd->id[l], hd->ex32[l], hd->ny32[l], hd->id32[l], e);
+    to_fp32(hd->n[l], hd->ex[l], hd->ny[l], hd->ex32[l], hd->ny32[l], hd->id32[l], e);
 }
 
 // One workgroup: the FP32 tail matrix into global memory.

@@ -57,11 +57,42 @@ __device__ inline void build_fine_faces(const double* __restrict__ a, int n, dou
     }, ny32);
 }
 
+// The FP32 cycle's reciprocal diagonal: 1 / d with d summed in FP32 from the
+// rounded faces in the order of level.hpp:92.
+DARCY_INLINE float inv_diag32(double e, double w, double n, double s) {
+    return 1.0f / (((static_cast<float>(e) + static_cast<float>(w)) + static_cast<float>(n)) + static_cast<float>(s));
+}
+
+// An FP64 value together with an FP32 one stored beside it.
+struct DF {
+    double d;
+    float f;
+};
+
+// map4 for a DF value: d to out, f to out32 (when given).
+template <class F>
+DARCY_INLINE void map4_df(int count, double* __restrict__ out, float* __restrict__ out32, Exec e, F&& value) {
+    for (int q0 = e.tid; q0 < count; q0 += 4 * e.nthr) {
+        DF v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int q = q0 + u * e.nthr;
+            v[u] = value(q < count ? q : count - 1);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (q0 + u * e.nthr < count) {
+                out[q0 + u * e.nthr] = v[u].d;
+                if (out32) out32[q0 + u * e.nthr] = v[u].f;
+            }
+    }
+}
+
 // inv_diag of the fine level (level.hpp:88-94), recomputed from a so that it
-// needs no barrier after the faces.
+// needs no barrier after the faces; with id32, the FP32 cycle's too.
 __device__ inline void build_fine_invdiag(const double* __restrict__ a, int n, double* __restrict__ id, Exec e,
                                           float* id32 = nullptr) {
-    map4(n * n, id, e, [&](int k) {
+    map4_df(n * n, id, id32, e, [&](int k) {
         const int i = k / n, j = k - i * n;
         const bool hE = j + 1 < n, hW = j > 0, hN = i + 1 < n, hS = i > 0;
         const double c = a[k], aE = a[hE ? k + 1 : k], aW = a[hW ? k - 1 : k], aN = a[hN ? k + n : k],
@@ -70,8 +101,8 @@ __device__ inline void build_fine_invdiag(const double* __restrict__ a, int n, d
         const double fw = hW ? hmean(c, aW) : c;
         const double fn = hN ? hmean(c, aN) : c;
         const double fs = hS ? hmean(c, aS) : c;
-        return 1.0 / (((fe + fw) + fn) + fs);
-    }, id32);
+        return DF{1.0 / (((fe + fw) + fn) + fs), inv_diag32(fe, fw, fn, fs)};
+    });
 }
 
 // Coarse faces (level.hpp:171-211 in face form).  Coarse ex column J sums
@@ -106,7 +137,7 @@ __device__ inline void galerkin_faces(int n, const double* __restrict__ ex, cons
 __device__ inline void galerkin_invdiag(int n, const double* __restrict__ ex, const double* __restrict__ ny,
                                         double* __restrict__ cid, Exec e, float* cid32 = nullptr) {
     const int m = (n + 1) >> 1;
-    map4(m * m, cid, e, [&](int q) {
+    map4_df(m * m, cid, cid32, e, [&](int q) {
         const int I = q / m, J = q - I * m;
         const int i0 = 2 * I, i1 = (2 * I + 1 < n) ? 2 * I + 1 : 2 * I;
         const int j0 = 2 * J, j1 = (2 * J + 1 < n) ? 2 * J + 1 : 2 * J;
@@ -120,20 +151,21 @@ __device__ inline void galerkin_invdiag(int n, const double* __restrict__ ex, co
         cw = two_i ? cw + w1 : cw;
         cn = two_j ? cn + n1 : cn;
         cs = two_j ? cs + s1 : cs;
-        return 1.0 / (((ce + cw) + cn) + cs);
-    }, cid32);
+        return DF{1.0 / (((ce + cw) + cn) + cs), inv_diag32(ce, cw, cn, cs)};
+    });
 }
 
-// FP32 copies of one level's operator for the mixed-precision cycle.  The
-// reciprocal diagonal is rounded from its FP64 value; the cycle's diagonal
-// is summed in FP32 from the rounded faces where a kernel needs it.
+// FP32 copies of one level's operator for the mixed-precision cycle: the
+// rounded faces and inv_diag32 of them.
 __device__ inline void to_fp32(int n, const double* __restrict__ ex, const double* __restrict__ ny,
-                               const double* __restrict__ id, float* __restrict__ ex32, float* __restrict__ ny32,
-                               float* __restrict__ id32, Exec e) {
+                               float* __restrict__ ex32, float* __restrict__ ny32, float* __restrict__ id32, Exec e) {
     const int fx = n * (n + 1), N = n * n;
     map4(fx, ex32, e, [&](int q) { return static_cast<float>(ex[q]); });
     map4(fx, ny32, e, [&](int q) { return static_cast<float>(ny[q]); });
-    map4(N, id32, e, [&](int q) { return static_cast<float>(id[q]); });
+    map4(N, id32, e, [&](int k) {
+        const int i = k / n, j = k - i * n, f = i * (n + 1) + j;
+        return inv_diag32(ex[f + 1], ex[f], ny[(i + 1) * n + j], ny[i * n + j]);
+    });
 }
 
 // Smallest flat index with !(a > 0) || !isfinite(a) among this thread's

@@ -137,9 +137,9 @@ Op<T> convert(const Op<double>& D) {
         L.no[k] = static_cast<T>(D.no[k]);
         L.so[k] = static_cast<T>(D.so[k]);
         // The cycle's diagonal is summed in T from the rounded faces, in the
-        // order of level.hpp:92; the reciprocal is rounded from FP64.
+        // order of level.hpp:92, and its reciprocal taken in T.
         L.dg[k] = ((L.e[k] + L.w[k]) + L.no[k]) + L.so[k];
-        L.inv[k] = static_cast<T>(D.inv[k]);
+        L.inv[k] = T(1) / L.dg[k];
     }
     return L;
 }
