@@ -45,7 +45,7 @@ NET_FLOPS_PER_SAMPLE = 4.7563e9
 # learned line (1000 x 256^2, W-cycle, committed weights): rocprofv3
 # FETCH_SIZE (x 2) + WRITE_SIZE (x 1), the factors that the same counters give
 # for the library's copy kernel of 1 GiB.  Reported for that configuration only.
-TRAFFIC_DEFAULT_BYTES = 379.3e9
+TRAFFIC_DEFAULT_BYTES = 345.8e9
 SEED0 = 20000
 
 # Algorithmic bytes per fine cell (counting rule: each array a kernel must
