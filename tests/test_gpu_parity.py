@@ -102,7 +102,7 @@ def test_mixed_precision_solution(oracle):
     assert rel_l2(got.u, ref.u) <= 1e-8
 
 
-@pytest.mark.parametrize("cap", [1, 3])
+@pytest.mark.parametrize("cap", [1, 2, 3])
 def test_mixed_precision_capped_iterations(oracle, rng, cap):
     """The FP32-cycle kernel applies each u += alpha p one pass late; a solve
     stopped by the iteration cap must still carry the last one.  Without it u
