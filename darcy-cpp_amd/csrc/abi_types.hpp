@@ -47,6 +47,7 @@ struct SolveParams {
     int32_t mixed;  // FP32 cycle inside FP64 flexible CG
     int32_t mode;
     int32_t record_history;
+    int32_t step_cells;  // cells per row-pipeline step of the global levels
 };
 
 struct BatchArgs {

@@ -71,7 +71,8 @@ DARCY_INLINE void run_problem(Shared& s, T* pool, float* tail, const Problem& pb
         }
         __syncthreads();
     }
-    const Cycle<T> cyc{h, s.t, pool, tail, CycleOpts{P.pre, P.post, P.gamma}, kStepCellsShared};
+    const Cycle<T> cyc{h, s.t, pool, tail, CycleOpts{P.pre, P.post, P.gamma},
+                       P.step_cells > 0 ? P.step_cells : kStepCellsShared};
     const LvP<double> A0 = global_level<double>(h, 0);
     float* const b0f = mixed ? static_cast<float*>(h.b[0]) : nullptr;
     const float* const x0f = mixed ? static_cast<const float*>(h.x[0]) : nullptr;

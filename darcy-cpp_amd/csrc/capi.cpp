@@ -206,6 +206,7 @@ int run(darcy_ctx* ctx, const Job& job) {
     args.prm.mixed = mixed ? 1 : 0;
     args.prm.mode = job.mode;
     args.prm.record_history = hist ? 1 : 0;
+    args.prm.step_cells = ctx->batch_step_cells;
 
     HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream), "event record");
     HIP_TRY(darcy_hip_launch_batch(&args, static_cast<int>(grid), mixed, ctx->stream), "solve kernel launch");
@@ -697,6 +698,8 @@ extern "C" int darcy_ctx_set_tuning(darcy_ctx* ctx, const char* key, int32_t val
         ctx->grid_graph = value;
     } else if (k == "grid_cg_per_cu" && value >= 1 && value <= 8) {
         ctx->grid_cg_per_cu = value;
+    } else if (k == "batch_step_cells" && value >= 128 && value <= 16384) {
+        ctx->batch_step_cells = value;
     } else {
         return fail(ctx, DARCY_INVALID_INPUT, "unknown tuning key or value out of range: " + k);
     }
@@ -714,6 +717,8 @@ extern "C" int darcy_ctx_get_tuning(const darcy_ctx* ctx, const char* key, int32
         *value = ctx->grid_graph;
     else if (k == "grid_cg_per_cu")
         *value = ctx->grid_cg_per_cu;
+    else if (k == "batch_step_cells")
+        *value = ctx->batch_step_cells;
     else
         return DARCY_INVALID_INPUT;
     return DARCY_OK;

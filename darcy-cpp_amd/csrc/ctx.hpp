@@ -55,6 +55,9 @@ struct darcy_ctx {
     int grid_graph = 1;
     // Workgroups per CU of the streaming CG kernels (rz, update) on the grid path.
     int grid_cg_per_cu = 4;
+    // Cells per row-pipeline step of the persistent batch kernel's global
+    // levels (tuned value).
+    int batch_step_cells = 1024;
     // The last captured cycle (+ the rz kernel) of the grid path and what it
     // was captured for: the workspace address, n, precision, options and
     // tail level.  Reused while all of them match.

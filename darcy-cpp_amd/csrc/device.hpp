@@ -394,8 +394,9 @@ DARCY_INLINE void cycle_levels(int l0, int L, bool fresh0, const CycleOpts& o, E
 // Rows per pipeline step: about `cells` cells, even, at least 2.  A step's
 // rows are read two or three times (red, black, residual), so they should
 // stay in L2 between those reads: the persistent batch kernel, 128
-// workgroups to each 4 MB L2, takes 1024-cell steps (1000 x 256^2: FP32
-// cycle 100.1 -> 99.2 ms, FP64 141.1 -> 135.4 ms against 2048); the tail
+// workgroups to each 4 MB L2, takes kStepCellsShared-cell steps unless its
+// launch sets another size (a tuning value; 1000 x 256^2: FP32 cycle
+// 100.1 -> 99.2 ms, FP64 141.1 -> 135.4 ms at 1024 against 2048); the tail
 // workgroup of the grid path, alone on its CU, takes 4096 (4096^2 FP64
 // 352 -> 345 ms).  The step size does not change any result.
 constexpr int kStepCellsShared = 1024;

@@ -9,15 +9,20 @@ none of them changes a result, only how the work is launched:
   grid_tail_side  on the grid path, levels with side <= this run inside one
                   workgroup (the rest phase by phase over the whole GPU)
   grid_graph      replay the grid path's cycle from a captured HIP graph
+  batch_step_cells  cells per row-pipeline step of the persistent kernel's
+                  levels in global memory (its tile of the smoothing and
+                  restriction sweeps)
 
 The tuner times candidate settings on synthetic two-phase media and picks the
 fastest: first grid_tail_side x grid_graph over a set of grid-path sizes
 (total time), then grid_min_n as the largest size at which the persistent
-kernel still beats the grid path.  Results go to a JSON table per device
+kernel still beats the grid path, then batch_step_cells on a batch of
+persistent-kernel solves.  Results go to a JSON table per device
 architecture (darcy/tuning/<arch>.json), which capi.Context and the
 process-wide context of darcy._core apply when they are created.
 
     python -m darcy.autotune [--sizes 128 256 512 1024 2048] [--repeats 3]
+                             [--batch-n 256] [--batch-count 1000]
 """
 
 from __future__ import annotations
@@ -35,7 +40,8 @@ from . import capi
 from .fields import gaussian_random_field, two_phase_permeability
 
 TABLE_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning")
-KEYS = ("grid_min_n", "grid_tail_side", "grid_graph")
+KEYS = ("grid_min_n", "grid_tail_side", "grid_graph", "batch_step_cells")
+STEP_CELLS = (512, 1024, 2048, 4096)
 
 
 def table_path(arch: str) -> str:
@@ -71,8 +77,17 @@ def _time_solve(ctx: capi.Context, n: int, a_ptr: int, u_ptr: int, precision: st
     return best * 1e3
 
 
+def _time_batch(ctx: capi.Context, n: int, a_ptrs: list, u_ptrs: list, precision: str, repeats: int) -> float:
+    ctx.solve_batch_device([n] * len(a_ptrs), a_ptrs, u_ptrs, precision=precision)  # warm-up
+    best = float("inf")
+    for _ in range(repeats):
+        ctx.solve_batch_device([n] * len(a_ptrs), a_ptrs, u_ptrs, precision=precision)
+        best = min(best, ctx.last_kernel_ms())
+    return best
+
+
 def tune(ctx: capi.Context, sizes: Iterable[int], small: Iterable[int], repeats: int = 3,
-         precision: str = "mixed", log=print) -> dict:
+         precision: str = "mixed", log=print, batch_n: int = 256, batch_count: int = 1000) -> dict:
     sizes, small = list(sizes), list(small)
     media = {n: two_phase_permeability(gaussian_random_field(n, seed=n)) for n in set(sizes) | set(small)}
     bufs = {}
@@ -81,7 +96,13 @@ def tune(ctx: capi.Context, sizes: Iterable[int], small: Iterable[int], repeats:
         ctx.upload(pa, a)
         bufs[n] = (pa, pu)
     saved = {k: ctx.get_tuning(k) for k in KEYS}
-    results: dict = {"grid": [], "switch": []}
+    results: dict = {"grid": [], "switch": [], "batch": []}
+    batch_a = np.stack([two_phase_permeability(gaussian_random_field(batch_n, seed=i)) for i in range(batch_count)])
+    batch_base, batch_u = ctx.alloc(batch_a.nbytes), ctx.alloc(batch_a.nbytes)
+    ctx.upload(batch_base, batch_a)
+    cells = batch_n * batch_n
+    batch_ptrs = [batch_base + 8 * cells * i for i in range(batch_count)]
+    batch_out = [batch_u + 8 * cells * i for i in range(batch_count)]
     try:
         # 1. Grid-path shape: every size forced onto the grid path.
         ctx.set_tuning("grid_min_n", 1)
@@ -110,13 +131,25 @@ def tune(ctx: capi.Context, sizes: Iterable[int], small: Iterable[int], repeats:
             log(json.dumps(results["switch"][-1]))
             if t_batch <= t_grid:
                 switch = n
-        chosen = {"grid_min_n": switch, "grid_tail_side": tail, "grid_graph": graph}
+        # 3. The persistent kernel's pipeline step, on a batch of its solves.
+        step = None
+        for cells_per_step in STEP_CELLS:
+            ctx.set_tuning("batch_step_cells", cells_per_step)
+            ms = _time_batch(ctx, batch_n, batch_ptrs, batch_out, precision, repeats)
+            results["batch"].append({"batch_step_cells": cells_per_step, "n": batch_n, "count": batch_count,
+                                     "kernel_ms": ms})
+            log(json.dumps(results["batch"][-1]))
+            if step is None or ms < step[0]:
+                step = (ms, cells_per_step)
+        chosen = {"grid_min_n": switch, "grid_tail_side": tail, "grid_graph": graph, "batch_step_cells": step[1]}
     finally:
         for k, v in saved.items():
             ctx.set_tuning(k, v)
         for pa, pu in bufs.values():
             ctx.free(pa)
             ctx.free(pu)
+        ctx.free(batch_base)
+        ctx.free(batch_u)
     return {"device": ctx.info(), "precision": precision, "repeats": repeats, "chosen": chosen, "results": results}
 
 
@@ -125,10 +158,13 @@ def main(argv=None) -> int:
     ap.add_argument("--sizes", type=int, nargs="+", default=[256, 512, 1024, 2048])
     ap.add_argument("--small", type=int, nargs="+", default=[64, 96, 128, 192, 256])
     ap.add_argument("--repeats", type=int, default=3)
+    ap.add_argument("--batch-n", type=int, default=256)
+    ap.add_argument("--batch-count", type=int, default=1000)
     ap.add_argument("--out", default=None, help="table path (default: darcy/tuning/<arch>.json)")
     args = ap.parse_args(argv)
     with capi.Context(apply_tuning=False) as ctx:
-        table = tune(ctx, args.sizes, args.small, args.repeats, log=lambda s: print(s, flush=True))
+        table = tune(ctx, args.sizes, args.small, args.repeats, log=lambda s: print(s, flush=True),
+                     batch_n=args.batch_n, batch_count=args.batch_count)
         out = args.out or table_path(ctx.info().split(",")[0])
     os.makedirs(os.path.dirname(os.path.abspath(out)), exist_ok=True)
     with open(out, "w") as fh:

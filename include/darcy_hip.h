@@ -97,6 +97,9 @@ void darcy_ctx_destroy(darcy_ctx* ctx);
  *                     0: launches the cycle's kernels one by one
  *   "grid_cg_per_cu"  workgroups per CU of the grid path's streaming CG
  *                     kernels (1..8, default 4; not autotuned)
+ *   "batch_step_cells" cells per row-pipeline step on the levels the
+ *                     persistent batch kernel keeps in global memory
+ *                     (128..16384, default 1024)
  * None changes any result: only the launch structure. */
 int darcy_ctx_set_tuning(darcy_ctx* ctx, const char* key, int32_t value);
 int darcy_ctx_get_tuning(const darcy_ctx* ctx, const char* key, int32_t* value);

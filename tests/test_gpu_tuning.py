@@ -54,13 +54,16 @@ def test_autotuner_picks_valid_shapes_and_restores_the_context():
     with capi.Context(apply_tuning=False) as ctx:
         before = {k: ctx.get_tuning(k) for k in autotune.KEYS}
         print("autotune: small run", flush=True)
-        table = autotune.tune(ctx, sizes=[128, 200], small=[64, 96], repeats=1, log=lambda s: None)
+        table = autotune.tune(ctx, sizes=[128, 200], small=[64, 96], repeats=1, log=lambda s: None, batch_n=64,
+                              batch_count=32)
         after = {k: ctx.get_tuning(k) for k in autotune.KEYS}
         assert after == before
         chosen = table["chosen"]
         assert chosen["grid_tail_side"] in (32, 64, 128) and chosen["grid_graph"] in (0, 1)
         assert chosen["grid_min_n"] in (1, 64, 96)
+        assert chosen["batch_step_cells"] in autotune.STEP_CELLS
         assert len(table["results"]["grid"]) == 6 and len(table["results"]["switch"]) == 2
+        assert len(table["results"]["batch"]) == len(autotune.STEP_CELLS)
         autotune.apply(ctx, table)
         assert {k: ctx.get_tuning(k) for k in autotune.KEYS} == chosen
 
@@ -98,3 +101,38 @@ def test_tail_side_equals_persistent_kernel(precision):
                 u, it = _solve(ctx, a, precision)
                 assert it == it_ref
                 assert np.array_equal(u, u_ref)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_step_cells_do_not_change_results(precision):
+    """The persistent kernel's pipeline step (its sweep tile) over an uneven
+    batch whose sides put 1 to 32 rows in a step: the same bits for every
+    step size."""
+    sizes = [37, 64, 100, 129, 256]
+    media = [two_phase(n, seed=60 + i) for i, n in enumerate(sizes)]
+    with capi.Context(apply_tuning=False) as ctx:
+        bufs = [(ctx.alloc(a.nbytes), ctx.alloc(a.nbytes)) for a in media]
+        try:
+            for (pa, _), a in zip(bufs, media):
+                ctx.upload(pa, a)
+            ref = None
+            for cells in (128, 1024, 4096, 16384):
+                print(f"batch step {cells} {precision}", flush=True)
+                ctx.set_tuning("batch_step_cells", cells)
+                st = ctx.solve_batch_device(sizes, [b[0] for b in bufs], [b[1] for b in bufs], precision=precision)
+                us = []
+                for (_, pu), a in zip(bufs, media):
+                    u = np.empty_like(a)
+                    ctx.download(u, pu)
+                    us.append(u)
+                if ref is None:
+                    ref = (st["iterations"].copy(), us)
+                    assert st["converged"].all()
+                else:
+                    assert np.array_equal(st["iterations"], ref[0])
+                    for u, u_ref in zip(us, ref[1]):
+                        assert np.array_equal(u, u_ref)
+        finally:
+            for pa, pu in bufs:
+                ctx.free(pa)
+                ctx.free(pu)
