@@ -110,7 +110,7 @@ DARCY_INLINE T apply_at(int n, const T* ex, const T* ny, int i, int j, X&& x) {
 // is symmetric bit for bit); in FP32 it is the FP32 cycle's definition, which
 // its stored copy follows too.  Forming it saves the kernels its load.
 template <class T>
-DARCY_INLINE T inv_from_faces(T e, T w, T n, T s) {
+DARCY_HD __forceinline__ T inv_from_faces(T e, T w, T n, T s) {
     return T(1) / (((e + w) + n) + s);
 }
 
@@ -127,6 +127,20 @@ struct Ident {
     }
 };
 
+// The GS update of one cell (level.hpp:153-158) from its faces, b and the
+// neighbour values: ((((b + e x_E) + w x_W) + n x_N) + s x_S) * inv_d, an
+// off-grid neighbour's product selected away.
+template <class T>
+DARCY_HD __forceinline__ T gs_combine(T e, T w, T nn, T s, T bk, T xE, T xW, T xN, T xS, bool hE, bool hW, bool hN, bool hS) {
+    const T inv = inv_from_faces(e, w, nn, s);
+    T v = bk;
+    v += hE ? e * xE : T(0);
+    v += hW ? w * xW : T(0);
+    v += hN ? nn * xN : T(0);
+    v += hS ? s * xS : T(0);
+    return v * inv;
+}
+
 // Gauss-Seidel value of cell (i, j) from its neighbours (level.hpp:153-158).
 // `ld(i, j)` reads what x at grid coordinates (i, j) is formed from and
 // `cv` forms it (by default x is read as stored).  All of the cell's loads
@@ -141,14 +155,7 @@ DARCY_INLINE T gs_at(const LvP<T>& L, int i, int j, T bk, Ld&& ld, Cv&& cv = Cv{
     const auto rE = ld(i, hE ? j + 1 : j), rW = ld(i, hW ? j - 1 : j), rN = ld(hN ? i + 1 : i, j),
                rS = ld(hS ? i - 1 : i, j);
     __builtin_amdgcn_sched_barrier(0);
-    const T inv = inv_from_faces(e, w, nn, s);
-    const T xE = cv(rE), xW = cv(rW), xN = cv(rN), xS = cv(rS);
-    T v = bk;
-    v += hE ? e * xE : T(0);
-    v += hW ? w * xW : T(0);
-    v += hN ? nn * xN : T(0);
-    v += hS ? s * xS : T(0);
-    return v * inv;
+    return gs_combine(e, w, nn, s, bk, cv(rE), cv(rW), cv(rN), cv(rS), hE, hW, hN, hS);
 }
 
 // ------------------------------------------------------------------ phases
@@ -226,6 +233,122 @@ DARCY_INLINE void presmooth_from_zero(const LvP<T>& L, Exec e) {
             if (kr[u] >= 0) x[kr[u]] = vr[u];
             if (kb[u] >= 0) x[kb[u]] = vb[u];
         }
+    }
+}
+
+// Pre-smoothing red + black from x = 0 as a vertical walk, for levels that
+// have the whole GPU (the grid path).  Work item (k, rows i0 .. i1-1) owns
+// the column pair (2k, 2k+1): row a's red cell is column 2k + (a & 1), its
+// black cell the other one.  Each row's red value b * inv_d is formed once
+// and kept in registers for the black cells of the rows below and above; the
+// red neighbour in the adjacent pair (east on even rows, west on odd rows) is
+// formed from its own loads.  The loads of the next row are issued before
+// the arithmetic of the current one.  inv_d is formed from the faces (the stored
+// value bit for bit), so the walk reads b and the faces and writes x, and
+// every value is the one presmooth_from_zero stores, in the same operations.
+constexpr int kWalkRows = 16;  // most rows per work item
+
+template <class T>
+struct WalkRow {
+    T b0, b1;  // b of the pair's two cells
+    T w0, m, e1;  // ex: west of cell 0, between the cells, east of cell 1
+    T s0, s1;  // ny: south faces of the two cells
+};
+
+template <class T>
+struct WalkCell {
+    T b, e, w, nn, s;  // b and the four faces of one cell
+};
+
+template <class T>
+DARCY_HD __forceinline__ void presmooth_walk_item(const LvP<T>& L, int k, int i0, int i1) {
+    const int n = L.n, c0 = 2 * k;
+    const bool two = c0 + 1 < n;     // the pair's second cell is on the grid
+    const int c1 = two ? c0 + 1 : c0;  // its column, clamped for the loads
+    const T* ex = L.ex;
+    const T* ny = L.ny;
+    const T* b = L.b;
+    T* x = L.x;
+    auto row = [&](int a) {
+        const int f = a * (n + 1) + c0, r = a * n;
+        return WalkRow<T>{b[r + c0], b[r + c1], ex[f], ex[f + 1], ex[f + 1 + (c1 - c0)], ny[r + c0], ny[r + c1]};
+    };
+    // Red value of row a from its row data and north faces (n0, n1).
+    auto red_of = [&](int a, const WalkRow<T>& R, T n0, T n1) {
+        return (a & 1) ? R.b1 * inv_from_faces(R.e1, R.m, n1, R.s1) : R.b0 * inv_from_faces(R.m, R.w0, n0, R.s0);
+    };
+    // b and faces of cell (a, c), and the adjacent pair's red cell on row a
+    // (east on even rows, west on odd rows; clamped to c0 where it is off the
+    // grid, its product then unused).
+    auto cell = [&](int a, int c) {
+        const int f = a * (n + 1) + c;
+        return WalkCell<T>{b[a * n + c], ex[f + 1], ex[f], ny[(a + 1) * n + c], ny[a * n + c]};
+    };
+    auto cross_on = [&](int a) { return (a & 1) ? c0 > 0 : c0 + 2 < n; };
+    auto cross_col = [&](int a) { return cross_on(a) ? ((a & 1) ? c0 - 1 : c0 + 2) : c0; };
+
+    WalkRow<T> cur = row(i0);
+    T cn0 = ny[(i0 + 1) * n + c0], cn1 = ny[(i0 + 1) * n + c1];  // north faces of row i0
+    T rcur = red_of(i0, cur, cn0, cn1), rprev = T(0);
+    if (i0 > 0) rprev = red_of(i0 - 1, row(i0 - 1), cur.s0, cur.s1);
+    // One row ahead: row i0 + 1 with its north faces, and row i0's cross cell.
+    WalkRow<T> nx = cur;
+    T nn0 = T(0), nn1 = T(0);
+    if (i0 + 1 < n) {
+        nx = row(i0 + 1);
+        nn0 = ny[(i0 + 2) * n + c0];
+        nn1 = ny[(i0 + 2) * n + c1];
+    }
+    WalkCell<T> cx = cell(i0, cross_col(i0));
+    for (int a = i0; a < i1; ++a) {
+        const bool hN = a + 1 < n, hS = a > 0, odd = (a & 1) != 0, more = a + 1 < i1;
+        // Loads for the next row, issued before this row's arithmetic: row
+        // a + 2 with its north faces and row a + 1's cross cell.
+        WalkRow<T> nx2 = nx;
+        T mm0 = T(0), mm1 = T(0);
+        if (more && a + 2 < n) {
+            nx2 = row(a + 2);
+            mm0 = ny[(a + 3) * n + c0];
+            mm1 = ny[(a + 3) * n + c1];
+        }
+        WalkCell<T> cx2 = cx;
+        if (more) cx2 = cell(a + 1, cross_col(a + 1));
+        const T rnext = hN ? red_of(a + 1, nx, nn0, nn1) : T(0);
+        const bool hX = cross_on(a);
+        const T cross = cx.b * inv_from_faces(cx.e, cx.w, cx.nn, cx.s);
+        const T blk = odd ? gs_combine(cur.m, cur.w0, cn0, cur.s0, cur.b0, rcur, cross, rnext, rprev, two, hX, hN, hS)
+                          : gs_combine(cur.e1, cur.m, cn1, cur.s1, cur.b1, cross, rcur, rnext, rprev, hX, true, hN, hS);
+        T* xr = x + a * n + c0;
+        if (odd) {
+            xr[0] = blk;
+            if (two) xr[1] = rcur;
+        } else {
+            xr[0] = rcur;
+            if (two) xr[1] = blk;
+        }
+        rprev = rcur;
+        rcur = rnext;
+        cur = nx;
+        cn0 = nn0;
+        cn1 = nn1;
+        nx = nx2;
+        nn0 = mm0;
+        nn1 = mm1;
+        cx = cx2;
+    }
+}
+
+// The walk over a level: rows per item chosen so that the items about fill
+// the launch (at most kWalkRows; each item re-forms one red row above it).
+template <class T>
+DARCY_INLINE void presmooth_walk(const LvP<T>& L, Exec e) {
+    const int n = L.n, h = (n + 1) >> 1;
+    long long per = (static_cast<long long>(n) * h) / e.nthr;
+    const int rows = per < 1 ? 1 : (per > kWalkRows ? kWalkRows : static_cast<int>(per));
+    const int strips = (n + rows - 1) / rows, total = strips * h;
+    for (int w = e.tid; w < total; w += e.nthr) {
+        const int st = w / h, k = w - st * h, i0 = st * rows;
+        presmooth_walk_item(L, k, i0, i0 + rows < n ? i0 + rows : n);
     }
 }
 

@@ -59,7 +59,7 @@ __global__ __launch_bounds__(kBlock) void kg_presmooth(const Hier* hd, int l, in
     const LvP<T> L = global_level<T>(*hd, l);
     const Exec e = grid_exec();
     if (kind == 0)
-        presmooth_from_zero(L, e);  // red + black from x = 0
+        presmooth_walk(L, e);  // red + black from x = 0
     else
         gs_colour(L, kind == 1 ? 0 : 1, e);  // 1: red, 2: black
 }
