@@ -260,7 +260,15 @@ struct WalkCell {
     T b, e, w, nn, s;  // b and the four faces of one cell
 };
 
+// Two consecutive values at an even offset, loaded or stored as one access.
 template <class T>
+struct alignas(2 * sizeof(T)) Pair2 {
+    T lo, hi;
+};
+
+// kPairs: n is even, so every pair starts at an even offset and both of its
+// cells are on the grid; b, ny and x move as pairs.
+template <class T, bool kPairs>
 DARCY_HD __forceinline__ void presmooth_walk_item(const LvP<T>& L, int k, int i0, int i1) {
     const int n = L.n, c0 = 2 * k;
     const bool two = c0 + 1 < n;     // the pair's second cell is on the grid
@@ -271,7 +279,31 @@ DARCY_HD __forceinline__ void presmooth_walk_item(const LvP<T>& L, int k, int i0
     T* x = L.x;
     auto row = [&](int a) {
         const int f = a * (n + 1) + c0, r = a * n;
-        return WalkRow<T>{b[r + c0], b[r + c1], ex[f], ex[f + 1], ex[f + 1 + (c1 - c0)], ny[r + c0], ny[r + c1]};
+        if constexpr (kPairs) {
+            // b and ny pairs start at even offsets; the ex row (n + 1 long)
+            // holds an aligned pair at f on even rows and at f + 1 on odd ones.
+            const Pair2<T> bb = *reinterpret_cast<const Pair2<T>*>(b + r + c0);
+            const Pair2<T> ss = *reinterpret_cast<const Pair2<T>*>(ny + r + c0);
+            if (a & 1) {
+                const Pair2<T> q = *reinterpret_cast<const Pair2<T>*>(ex + f + 1);
+                return WalkRow<T>{bb.lo, bb.hi, ex[f], q.lo, q.hi, ss.lo, ss.hi};
+            }
+            const Pair2<T> q = *reinterpret_cast<const Pair2<T>*>(ex + f);
+            return WalkRow<T>{bb.lo, bb.hi, q.lo, q.hi, ex[f + 2], ss.lo, ss.hi};
+        } else {
+            return WalkRow<T>{b[r + c0], b[r + c1], ex[f], ex[f + 1], ex[f + 1 + (c1 - c0)], ny[r + c0], ny[r + c1]};
+        }
+    };
+    // North faces of row a - 1 (the south faces of row a), into (s0, s1).
+    auto south = [&](int a, T& s0, T& s1) {
+        if constexpr (kPairs) {
+            const Pair2<T> q = *reinterpret_cast<const Pair2<T>*>(ny + a * n + c0);
+            s0 = q.lo;
+            s1 = q.hi;
+        } else {
+            s0 = ny[a * n + c0];
+            s1 = ny[a * n + c1];
+        }
     };
     // Red value of row a from its row data and north faces (n0, n1).
     auto red_of = [&](int a, const WalkRow<T>& R, T n0, T n1) {
@@ -288,7 +320,8 @@ DARCY_HD __forceinline__ void presmooth_walk_item(const LvP<T>& L, int k, int i0
     auto cross_col = [&](int a) { return cross_on(a) ? ((a & 1) ? c0 - 1 : c0 + 2) : c0; };
 
     WalkRow<T> cur = row(i0);
-    T cn0 = ny[(i0 + 1) * n + c0], cn1 = ny[(i0 + 1) * n + c1];  // north faces of row i0
+    T cn0, cn1;  // north faces of row i0
+    south(i0 + 1, cn0, cn1);
     T rcur = red_of(i0, cur, cn0, cn1), rprev = T(0);
     if (i0 > 0) rprev = red_of(i0 - 1, row(i0 - 1), cur.s0, cur.s1);
     // One row ahead: row i0 + 1 with its north faces, and row i0's cross cell.
@@ -296,8 +329,7 @@ DARCY_HD __forceinline__ void presmooth_walk_item(const LvP<T>& L, int k, int i0
     T nn0 = T(0), nn1 = T(0);
     if (i0 + 1 < n) {
         nx = row(i0 + 1);
-        nn0 = ny[(i0 + 2) * n + c0];
-        nn1 = ny[(i0 + 2) * n + c1];
+        south(i0 + 2, nn0, nn1);
     }
     WalkCell<T> cx = cell(i0, cross_col(i0));
     for (int a = i0; a < i1; ++a) {
@@ -308,8 +340,7 @@ DARCY_HD __forceinline__ void presmooth_walk_item(const LvP<T>& L, int k, int i0
         T mm0 = T(0), mm1 = T(0);
         if (more && a + 2 < n) {
             nx2 = row(a + 2);
-            mm0 = ny[(a + 3) * n + c0];
-            mm1 = ny[(a + 3) * n + c1];
+            south(a + 3, mm0, mm1);
         }
         WalkCell<T> cx2 = cx;
         if (more) cx2 = cell(a + 1, cross_col(a + 1));
@@ -319,7 +350,9 @@ DARCY_HD __forceinline__ void presmooth_walk_item(const LvP<T>& L, int k, int i0
         const T blk = odd ? gs_combine(cur.m, cur.w0, cn0, cur.s0, cur.b0, rcur, cross, rnext, rprev, two, hX, hN, hS)
                           : gs_combine(cur.e1, cur.m, cn1, cur.s1, cur.b1, cross, rcur, rnext, rprev, hX, true, hN, hS);
         T* xr = x + a * n + c0;
-        if (odd) {
+        if constexpr (kPairs) {
+            *reinterpret_cast<Pair2<T>*>(xr) = odd ? Pair2<T>{blk, rcur} : Pair2<T>{rcur, blk};
+        } else if (odd) {
             xr[0] = blk;
             if (two) xr[1] = rcur;
         } else {
@@ -347,8 +380,11 @@ DARCY_INLINE void presmooth_walk(const LvP<T>& L, Exec e) {
     const int rows = per < 1 ? 1 : (per > kWalkRows ? kWalkRows : static_cast<int>(per));
     const int strips = (n + rows - 1) / rows, total = strips * h;
     for (int w = e.tid; w < total; w += e.nthr) {
-        const int st = w / h, k = w - st * h, i0 = st * rows;
-        presmooth_walk_item(L, k, i0, i0 + rows < n ? i0 + rows : n);
+        const int st = w / h, k = w - st * h, i0 = st * rows, i1 = i0 + rows < n ? i0 + rows : n;
+        if (n & 1)
+            presmooth_walk_item<T, false>(L, k, i0, i1);
+        else
+            presmooth_walk_item<T, true>(L, k, i0, i1);
     }
 }
 
