@@ -268,8 +268,24 @@ struct alignas(2 * sizeof(T)) Pair2 {
 
 // kPairs: n is even, so every pair starts at an even offset and both of its
 // cells are on the grid; b, ny and x move as pairs.
-template <class T, bool kPairs>
-DARCY_HD __forceinline__ void presmooth_walk_item(const LvP<T>& L, int k, int i0, int i1) {
+// Where the walk takes the adjacent pair's red value from.  NoShare: from
+// its own loads.  The device walk passes LaneShare: lane l + 1 (l - 1) of the
+// wave walks pair k + 1 (k - 1) of the same rows whenever `east` (`west`)
+// holds, so that value is its current red value, taken by a lane shuffle.
+struct NoShare {
+    bool east = false, west = false;
+    template <class T>
+    DARCY_HD T from_east(T) const {
+        return T(0);
+    }
+    template <class T>
+    DARCY_HD T from_west(T) const {
+        return T(0);
+    }
+};
+
+template <class T, bool kPairs, class Share = NoShare>
+DARCY_HD __forceinline__ void presmooth_walk_item(const LvP<T>& L, int k, int i0, int i1, Share sh = Share{}) {
     const int n = L.n, c0 = 2 * k;
     const bool two = c0 + 1 < n;     // the pair's second cell is on the grid
     const int c1 = two ? c0 + 1 : c0;  // its column, clamped for the loads
@@ -317,6 +333,7 @@ DARCY_HD __forceinline__ void presmooth_walk_item(const LvP<T>& L, int k, int i0
         return WalkCell<T>{b[a * n + c], ex[f + 1], ex[f], ny[(a + 1) * n + c], ny[a * n + c]};
     };
     auto cross_on = [&](int a) { return (a & 1) ? c0 > 0 : c0 + 2 < n; };
+    auto shared = [&](int a) { return (a & 1) ? sh.west : sh.east; };
     auto cross_col = [&](int a) { return cross_on(a) ? ((a & 1) ? c0 - 1 : c0 + 2) : c0; };
 
     WalkRow<T> cur = row(i0);
@@ -331,7 +348,8 @@ DARCY_HD __forceinline__ void presmooth_walk_item(const LvP<T>& L, int k, int i0
         nx = row(i0 + 1);
         south(i0 + 2, nn0, nn1);
     }
-    WalkCell<T> cx = cell(i0, cross_col(i0));
+    WalkCell<T> cx{};
+    if (!shared(i0)) cx = cell(i0, cross_col(i0));
     for (int a = i0; a < i1; ++a) {
         const bool hN = a + 1 < n, hS = a > 0, odd = (a & 1) != 0, more = a + 1 < i1;
         // Loads for the next row, issued before this row's arithmetic: row
@@ -343,10 +361,12 @@ DARCY_HD __forceinline__ void presmooth_walk_item(const LvP<T>& L, int k, int i0
             south(a + 3, mm0, mm1);
         }
         WalkCell<T> cx2 = cx;
-        if (more) cx2 = cell(a + 1, cross_col(a + 1));
+        if (more && !shared(a + 1)) cx2 = cell(a + 1, cross_col(a + 1));
         const T rnext = hN ? red_of(a + 1, nx, nn0, nn1) : T(0);
         const bool hX = cross_on(a);
-        const T cross = cx.b * inv_from_faces(cx.e, cx.w, cx.nn, cx.s);
+        // Every lane takes part in both shuffles.
+        const T ve = sh.from_east(rcur), vw = sh.from_west(rcur);
+        const T cross = shared(a) ? (odd ? vw : ve) : cx.b * inv_from_faces(cx.e, cx.w, cx.nn, cx.s);
         const T blk = odd ? gs_combine(cur.m, cur.w0, cn0, cur.s0, cur.b0, rcur, cross, rnext, rprev, two, hX, hN, hS)
                           : gs_combine(cur.e1, cur.m, cn1, cur.s1, cur.b1, cross, rcur, rnext, rprev, hX, true, hN, hS);
         T* xr = x + a * n + c0;
@@ -371,6 +391,18 @@ DARCY_HD __forceinline__ void presmooth_walk_item(const LvP<T>& L, int k, int i0
     }
 }
 
+struct LaneShare {
+    bool east, west;
+    template <class T>
+    DARCY_INLINE T from_east(T v) const {
+        return __shfl_down(v, 1, kWave);
+    }
+    template <class T>
+    DARCY_INLINE T from_west(T v) const {
+        return __shfl_up(v, 1, kWave);
+    }
+};
+
 // The walk over a level: rows per item chosen so that the items about fill
 // the launch (at most kWalkRows; each item re-forms one red row above it).
 template <class T>
@@ -379,12 +411,16 @@ DARCY_INLINE void presmooth_walk(const LvP<T>& L, Exec e) {
     long long per = (static_cast<long long>(n) * h) / e.nthr;
     const int rows = per < 1 ? 1 : (per > kWalkRows ? kWalkRows : static_cast<int>(per));
     const int strips = (n + rows - 1) / rows, total = strips * h;
+    const int lane = threadIdx.x & (kWave - 1);
     for (int w = e.tid; w < total; w += e.nthr) {
         const int st = w / h, k = w - st * h, i0 = st * rows, i1 = i0 + rows < n ? i0 + rows : n;
+        // Consecutive lanes hold consecutive items, so lane l + 1 walks pair
+        // k + 1 of the same strip when k + 1 < h (and l - 1 pair k - 1 when k > 0).
+        const LaneShare sh{lane + 1 < kWave && k + 1 < h, lane > 0 && k > 0};
         if (n & 1)
-            presmooth_walk_item<T, false>(L, k, i0, i1);
+            presmooth_walk_item<T, false>(L, k, i0, i1, sh);
         else
-            presmooth_walk_item<T, true>(L, k, i0, i1);
+            presmooth_walk_item<T, true>(L, k, i0, i1, sh);
     }
 }
 
