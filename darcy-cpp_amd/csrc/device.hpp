@@ -87,6 +87,21 @@ struct Exec {
 // differ at most in the sign of a zero).  The product, not the coefficient,
 // is selected: the value read in place of a missing neighbour may be stale
 // or uninitialised.
+// (A x)_k from the cell's faces and x values: d x_k - e x_E - w x_W - n x_N
+// - s x_S with d = ((e + w) + n) + s, an off-grid neighbour's product
+// selected away.
+template <class T>
+DARCY_HD __forceinline__ T apply_combine(T e, T w, T nn, T s, T xk, T xE, T xW, T xN, T xS, bool hE, bool hW,
+                                         bool hN, bool hS) {
+    const T d = ((e + w) + nn) + s;
+    T v = d * xk;
+    v -= hE ? e * xE : T(0);
+    v -= hW ? w * xW : T(0);
+    v -= hN ? nn * xN : T(0);
+    v -= hS ? s * xS : T(0);
+    return v;
+}
+
 template <class T, class X>
 DARCY_INLINE T apply_at(int n, const T* ex, const T* ny, int i, int j, X&& x) {
     const int k = i * n + j, f = i * (n + 1) + j;
@@ -95,13 +110,7 @@ DARCY_INLINE T apply_at(int n, const T* ex, const T* ny, int i, int j, X&& x) {
     const T xk = x(k), xE = x(hE ? k + 1 : k), xW = x(hW ? k - 1 : k), xN = x(hN ? k + n : k),
             xS = x(hS ? k - n : k);
     __builtin_amdgcn_sched_barrier(0);  // all loads in flight together, as in gs_at
-    const T d = ((e + w) + nn) + s;
-    T v = d * xk;
-    v -= hE ? e * xE : T(0);
-    v -= hW ? w * xW : T(0);
-    v -= hN ? nn * xN : T(0);
-    v -= hS ? s * xS : T(0);
-    return v;
+    return apply_combine(e, w, nn, s, xk, xE, xW, xN, xS, hE, hW, hN, hS);
 }
 
 // The reciprocal diagonal from the cell's four faces: ((e + w) + n) + s
@@ -268,6 +277,37 @@ struct alignas(2 * sizeof(T)) Pair2 {
 
 // kPairs: n is even, so every pair starts at an even offset and both of its
 // cells are on the grid; b, ny and x move as pairs.
+// Values at (a, c0) and (a, c1) of an n-wide array (c1 = c0 + 1, or c0 when
+// the pair's second cell is off the grid).
+template <class T, bool kPairs>
+DARCY_HD __forceinline__ Pair2<T> walk_pair(const T* p, int n, int c0, int c1, int a) {
+    if constexpr (kPairs) {
+        return *reinterpret_cast<const Pair2<T>*>(p + a * n + c0);
+    } else {
+        return Pair2<T>{p[a * n + c0], p[a * n + c1]};
+    }
+}
+
+// b, the ex faces and the south faces of the pair's cells on row a.  b and
+// ny pairs start at even offsets; the ex row (n + 1 long) holds an aligned
+// pair at f on even rows and at f + 1 on odd ones.
+template <class T, bool kPairs>
+DARCY_HD __forceinline__ WalkRow<T> walk_row(const LvP<T>& L, int c0, int c1, int a) {
+    const int n = L.n, f = a * (n + 1) + c0;
+    const T* ex = L.ex;
+    const Pair2<T> bb = walk_pair<T, kPairs>(L.b, n, c0, c1, a), ss = walk_pair<T, kPairs>(L.ny, n, c0, c1, a);
+    if constexpr (kPairs) {
+        if (a & 1) {
+            const Pair2<T> q = *reinterpret_cast<const Pair2<T>*>(ex + f + 1);
+            return WalkRow<T>{bb.lo, bb.hi, ex[f], q.lo, q.hi, ss.lo, ss.hi};
+        }
+        const Pair2<T> q = *reinterpret_cast<const Pair2<T>*>(ex + f);
+        return WalkRow<T>{bb.lo, bb.hi, q.lo, q.hi, ex[f + 2], ss.lo, ss.hi};
+    } else {
+        return WalkRow<T>{bb.lo, bb.hi, ex[f], ex[f + 1], ex[f + 1 + (c1 - c0)], ss.lo, ss.hi};
+    }
+}
+
 // Where the walk takes the adjacent pair's red value from.  NoShare: from
 // its own loads.  The device walk passes LaneShare: lane l + 1 (l - 1) of the
 // wave walks pair k + 1 (k - 1) of the same rows whenever `east` (`west`)
@@ -293,33 +333,12 @@ DARCY_HD __forceinline__ void presmooth_walk_item(const LvP<T>& L, int k, int i0
     const T* ny = L.ny;
     const T* b = L.b;
     T* x = L.x;
-    auto row = [&](int a) {
-        const int f = a * (n + 1) + c0, r = a * n;
-        if constexpr (kPairs) {
-            // b and ny pairs start at even offsets; the ex row (n + 1 long)
-            // holds an aligned pair at f on even rows and at f + 1 on odd ones.
-            const Pair2<T> bb = *reinterpret_cast<const Pair2<T>*>(b + r + c0);
-            const Pair2<T> ss = *reinterpret_cast<const Pair2<T>*>(ny + r + c0);
-            if (a & 1) {
-                const Pair2<T> q = *reinterpret_cast<const Pair2<T>*>(ex + f + 1);
-                return WalkRow<T>{bb.lo, bb.hi, ex[f], q.lo, q.hi, ss.lo, ss.hi};
-            }
-            const Pair2<T> q = *reinterpret_cast<const Pair2<T>*>(ex + f);
-            return WalkRow<T>{bb.lo, bb.hi, q.lo, q.hi, ex[f + 2], ss.lo, ss.hi};
-        } else {
-            return WalkRow<T>{b[r + c0], b[r + c1], ex[f], ex[f + 1], ex[f + 1 + (c1 - c0)], ny[r + c0], ny[r + c1]};
-        }
-    };
+    auto row = [&](int a) { return walk_row<T, kPairs>(L, c0, c1, a); };
     // North faces of row a - 1 (the south faces of row a), into (s0, s1).
     auto south = [&](int a, T& s0, T& s1) {
-        if constexpr (kPairs) {
-            const Pair2<T> q = *reinterpret_cast<const Pair2<T>*>(ny + a * n + c0);
-            s0 = q.lo;
-            s1 = q.hi;
-        } else {
-            s0 = ny[a * n + c0];
-            s1 = ny[a * n + c1];
-        }
+        const Pair2<T> q = walk_pair<T, kPairs>(ny, n, c0, c1, a);
+        s0 = q.lo;
+        s1 = q.hi;
     };
     // Red value of row a from its row data and north faces (n0, n1).
     auto red_of = [&](int a, const WalkRow<T>& R, T n0, T n1) {
@@ -391,6 +410,66 @@ DARCY_HD __forceinline__ void presmooth_walk_item(const LvP<T>& L, int k, int i0
     }
 }
 
+// Residual and restriction (level.hpp:122-137, :214-227) as a vertical walk
+// for the grid path.  Work item (J, coarse rows I0 .. I1-1) owns the fine
+// column pair (2J, 2J+1) and walks its fine rows, holding x of the rows
+// below, at and above in registers; x of the columns beside the pair comes
+// from the neighbouring lanes' current rows through `sh` (from loads at wave
+// and strip edges).  The next row's loads are issued before the current
+// row's arithmetic.  Each residual is apply_at's expression and each coarse
+// value sums its aggregate from zero in the order (i0,j0),(i0,j1),(i1,j0),
+// (i1,j1), as restricted_residual does: the same bits.
+template <class T, bool kPairs, class Share = NoShare>
+DARCY_HD __forceinline__ void restrict_walk_item(const LvP<T>& L, T* bc, int J, int I0, int I1, Share sh = Share{}) {
+    const int n = L.n, m = (n + 1) >> 1, c0 = 2 * J;
+    const bool two = c0 + 1 < n, west_on = c0 > 0, east_on = c0 + 2 < n;
+    const int c1 = two ? c0 + 1 : c0;
+    const T* x = L.x;
+    const T* ny = L.ny;
+    const int a0 = 2 * I0, a1 = 2 * I1 < n ? 2 * I1 : n;
+    auto xpair = [&](int a) { return walk_pair<T, kPairs>(x, n, c0, c1, a); };
+    Pair2<T> xp{T(0), T(0)}, xc = xpair(a0), xn{T(0), T(0)};
+    if (a0 > 0) xp = xpair(a0 - 1);
+    if (a0 + 1 < n) xn = xpair(a0 + 1);
+    WalkRow<T> cur = walk_row<T, kPairs>(L, c0, c1, a0);
+    Pair2<T> north = walk_pair<T, kPairs>(ny, n, c0, c1, a0 + 1);
+    T xw = T(0), xe = T(0);  // x beside the pair, where no lane shares it
+    if (!sh.west && west_on) xw = x[a0 * n + c0 - 1];
+    if (!sh.east && east_on) xe = x[a0 * n + c0 + 2];
+    T acc = T(0);
+    for (int a = a0; a < a1; ++a) {
+        const bool hN = a + 1 < n, hS = a > 0, more = a + 1 < a1;
+        Pair2<T> xn2 = xn, north2 = north;
+        WalkRow<T> nx = cur;
+        T xw2 = T(0), xe2 = T(0);
+        if (more) {
+            nx = walk_row<T, kPairs>(L, c0, c1, a + 1);
+            north2 = walk_pair<T, kPairs>(ny, n, c0, c1, a + 2);
+            if (a + 2 < n) xn2 = xpair(a + 2);
+            if (!sh.west && west_on) xw2 = x[(a + 1) * n + c0 - 1];
+            if (!sh.east && east_on) xe2 = x[(a + 1) * n + c0 + 2];
+        }
+        // Every lane takes part in both shuffles.
+        const T vw = sh.from_west(xc.hi), ve = sh.from_east(xc.lo);
+        const T XW = sh.west ? vw : xw, XE = sh.east ? ve : xe;
+        const T r0 =
+            cur.b0 - apply_combine(cur.m, cur.w0, north.lo, cur.s0, xc.lo, xc.hi, XW, xn.lo, xp.lo, two, west_on, hN, hS);
+        const T r1 =
+            cur.b1 - apply_combine(cur.e1, cur.m, north.hi, cur.s1, xc.hi, XE, xc.lo, xn.hi, xp.hi, east_on, true, hN, hS);
+        if (!(a & 1)) acc = T(0);
+        acc += r0;
+        if (two) acc += r1;
+        if ((a & 1) || !hN) bc[(a >> 1) * m + J] = acc;
+        xp = xc;
+        xc = xn;
+        xn = xn2;
+        cur = nx;
+        north = north2;
+        xw = xw2;
+        xe = xe2;
+    }
+}
+
 struct LaneShare {
     bool east, west;
     template <class T>
@@ -421,6 +500,25 @@ DARCY_INLINE void presmooth_walk(const LvP<T>& L, Exec e) {
             presmooth_walk_item<T, false>(L, k, i0, i1, sh);
         else
             presmooth_walk_item<T, true>(L, k, i0, i1, sh);
+    }
+}
+
+// The restriction walk over a level, coarse rows per item chosen as in
+// presmooth_walk (at most kWalkRows / 2).
+template <class T>
+DARCY_INLINE void restrict_walk(const LvP<T>& L, T* bc, Exec e) {
+    const int m = (L.n + 1) >> 1;
+    long long per = (static_cast<long long>(m) * m) / e.nthr;
+    const int rows = per < 1 ? 1 : (per > kWalkRows / 2 ? kWalkRows / 2 : static_cast<int>(per));
+    const int strips = (m + rows - 1) / rows, total = strips * m;
+    const int lane = threadIdx.x & (kWave - 1);
+    for (int w = e.tid; w < total; w += e.nthr) {
+        const int st = w / m, J = w - st * m, I0 = st * rows, I1 = I0 + rows < m ? I0 + rows : m;
+        const LaneShare sh{lane + 1 < kWave && J + 1 < m, lane > 0 && J > 0};
+        if (L.n & 1)
+            restrict_walk_item<T, false>(L, bc, J, I0, I1, sh);
+        else
+            restrict_walk_item<T, true>(L, bc, J, I0, I1, sh);
     }
 }
 

@@ -67,7 +67,7 @@ __global__ __launch_bounds__(kBlock) void kg_presmooth(const Hier* hd, int l, in
 template <class T>
 __global__ __launch_bounds__(kBlock) void kg_restrict(const Hier* hd, int l) {
     const LvP<T> L = global_level<T>(*hd, l);
-    residual_restrict(L, static_cast<T*>(hd->b[l + 1]), grid_exec());
+    restrict_walk(L, static_cast<T*>(hd->b[l + 1]), grid_exec());
 }
 
 template <class T>
