@@ -143,10 +143,15 @@ __global__ __launch_bounds__(kTailThreads) void kg_tail(const Hier* hd, int l0, 
     __shared__ __align__(16) T pool[LdsPool<T, NL>::kElems];
     __shared__ __align__(16) float tail[sizeof(T) == 4 ? kTailLds : 1];
     const Exec e{static_cast<int>(threadIdx.x), kTailThreads};
-    if (threadIdx.x == 0) {
-        s.h = *hd;
-        plan_lds<T, NL>(s.h, s.t, l0);
-    }
+    // The hierarchy (about 2.2 KB) is copied by the whole workgroup, 8 bytes
+    // per thread, rather than by one thread: this prologue runs on every one
+    // of the tail's calls (64 per CG iteration at 4096^2 with the W-cycle).
+    static_assert(sizeof(Hier) % 8 == 0 && sizeof(Hier) / 8 <= kTailThreads, "hierarchy copy");
+    if (threadIdx.x < sizeof(Hier) / 8)
+        reinterpret_cast<unsigned long long*>(&s.h)[threadIdx.x] =
+            reinterpret_cast<const unsigned long long*>(hd)[threadIdx.x];
+    __syncthreads();
+    if (threadIdx.x == 0) plan_lds<T, NL>(s.h, s.t, l0);
     __syncthreads();
     const Hier& h = s.h;
     for (int l = s.t.first; l < h.L; ++l) {
