@@ -89,12 +89,27 @@ DARCY_INLINE RegCell reg_cell(const RegLevel& L) {
 
 DARCY_INLINE double lane_value(double v, int src) { return __shfl(v, src, kWave); }
 
+// Lane exchange within a row of 16 lanes (DPP row shift, no LDS): lane l
+// reads lane l + k (row_shl:k) or l - k (row_shr:k); 0 from outside the row.
+template <int CTRL>
+DARCY_INLINE double row_value(double v) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, true);
+    return __hiloint2double(hi, lo);
+}
+constexpr int kRowShl1 = 0x101, kRowShr1 = 0x111, kRowShl4 = 0x104, kRowShr4 = 0x114;
+
 struct RegNeighbours {
     double xE, xW, xN, xS;
 };
 
 DARCY_INLINE RegNeighbours reg_neighbours(const RegLevel& L, const RegCell& c, double v) {
     const int q = static_cast<int>(threadIdx.x & (kWave - 1)), n = L.A.n;
+    // The 4 x 4 level fills lanes 0..15, one DPP row: east / west are lanes
+    // q +- 1 and north / south q +- 4, read by row shifts.  Where a neighbour
+    // is off the grid the lane read is another cell (or 0), and reg_gs /
+    // reg_residual select its product away, as for the clamped permute.
+    if (n == 4) return {row_value<kRowShl1>(v), row_value<kRowShr1>(v), row_value<kRowShl4>(v), row_value<kRowShr4>(v)};
     return {lane_value(v, c.hE ? q + 1 : q), lane_value(v, c.hW ? q - 1 : q), lane_value(v, c.hN ? q + n : q),
             lane_value(v, c.hS ? q - n : q)};
 }
