@@ -259,15 +259,30 @@ DARCY_INLINE void reg_visit2(const RegLevel& L, const RegLevel& C, bool fresh, c
     WaveSync{}();
 }
 
-// Calls f(lv[k], lv[k + 1]) for a wave-uniform k < kRegLevels - 1.
+// Calls f(lv[k], lv[k + 1]) for a wave-uniform k < kRegLevels - 1.  The pair
+// is selected first and f is called once, so its body is compiled once: the
+// kernels that run this cycle are large, and one copy per level pair made
+// the code the single wave steps through three times longer.
+// Field by field, so that lv is only ever indexed by constants and stays in
+// registers.
+DARCY_INLINE RegLevel reg_pick(int k, const RegLevel& a, const RegLevel& b, const RegLevel& c) {
+    auto sel = [k](auto x, auto y, auto z) { return k == 0 ? x : (k == 1 ? y : z); };
+    RegLevel r;
+    r.A.n = sel(a.A.n, b.A.n, c.A.n);
+    r.A.ex = sel(a.A.ex, b.A.ex, c.A.ex);
+    r.A.ny = sel(a.A.ny, b.A.ny, c.A.ny);
+    r.A.id = sel(a.A.id, b.A.id, c.A.id);
+    r.A.x = sel(a.A.x, b.A.x, c.A.x);
+    r.A.b = sel(a.A.b, b.A.b, c.A.b);
+    r.i = sel(a.i, b.i, c.i);
+    r.j = sel(a.j, b.j, c.j);
+    r.on = sel(a.on, b.on, c.on);
+    return r;
+}
+
 template <class F>
 DARCY_INLINE void reg_pair(const RegLevel* lv, int k, F&& f) {
-    if (k == 0)
-        f(lv[0], lv[1]);
-    else if (k == 1)
-        f(lv[1], lv[2]);
-    else
-        f(lv[2], lv[3]);
+    f(reg_pick(k, lv[0], lv[1], lv[2]), reg_pick(k, lv[1], lv[2], lv[3]));
 }
 
 // The cycle over levels 0..nl-1 of lv, the last of side 1 (cycle_levels
