@@ -370,10 +370,14 @@ int run_grid(darcy_ctx* ctx, const Job& job) {
         }
         const int64_t nl = static_cast<int64_t>(h.n[l]) * h.n[l];
         const int bl = blocks_for(ctx, nl, 2048), bc = blocks_for(ctx, nl / 4 + 1, 256);
+        // The pre-smoothing walk takes as many rows per lane as the launch
+        // leaves it (each a memory round trip in turn): 512 cells per
+        // workgroup give the levels below 2048^2 one row per lane.
+        const int bw = blocks_for(ctx, nl, 512);
         hipError_t e = hipSuccess;
         for (int sw = 0; sw < o.pre_sweeps && e == hipSuccess; ++sw) {
             if (sw == 0 && fresh) {
-                e = darcy_grid_presmooth(mixed, hd, l, 0, bl, s);
+                e = darcy_grid_presmooth(mixed, hd, l, 0, bw, s);
             } else {
                 e = darcy_grid_presmooth(mixed, hd, l, 1, bl, s);
                 if (e == hipSuccess) e = darcy_grid_presmooth(mixed, hd, l, 2, bl, s);
