@@ -369,11 +369,11 @@ int run_grid(darcy_ctx* ctx, const Job& job) {
             return;
         }
         const int64_t nl = static_cast<int64_t>(h.n[l]) * h.n[l];
-        const int bl = blocks_for(ctx, nl, 2048), bc = blocks_for(ctx, nl / 4 + 1, 256);
-        // The pre-smoothing walk takes as many rows per lane as the launch
-        // leaves it (each a memory round trip in turn): 512 cells per
-        // workgroup give the levels below 2048^2 one row per lane.
-        const int bw = blocks_for(ctx, nl, 512);
+        // 512 cells per workgroup: the small levels are latency-bound, and
+        // the pre-smoothing walk takes as many rows per lane as the launch
+        // leaves it (each a memory round trip in turn), so the levels below
+        // 2048^2 get one row per lane and more CUs; the largest are capped.
+        const int bl = blocks_for(ctx, nl, 512), bc = blocks_for(ctx, nl / 4 + 1, 256), bw = bl;
         hipError_t e = hipSuccess;
         for (int sw = 0; sw < o.pre_sweeps && e == hipSuccess; ++sw) {
             if (sw == 0 && fresh) {
