@@ -110,7 +110,13 @@ DARCY_INLINE T apply_at(int n, const T* ex, const T* ny, int i, int j, X&& x) {
     const T xk = x(k), xE = x(hE ? k + 1 : k), xW = x(hW ? k - 1 : k), xN = x(hN ? k + n : k),
             xS = x(hS ? k - n : k);
     __builtin_amdgcn_sched_barrier(0);  // all loads in flight together, as in gs_at
-    return apply_combine(e, w, nn, s, xk, xE, xW, xN, xS, hE, hW, hN, hS);
+    const T d = ((e + w) + nn) + s;
+    T v = d * xk;
+    v -= hE ? e * xE : T(0);
+    v -= hW ? w * xW : T(0);
+    v -= hN ? nn * xN : T(0);
+    v -= hS ? s * xS : T(0);
+    return v;
 }
 
 // The reciprocal diagonal from the cell's four faces: ((e + w) + n) + s
@@ -138,7 +144,12 @@ struct Ident {
 
 // The GS update of one cell (level.hpp:153-158) from its faces, b and the
 // neighbour values: ((((b + e x_E) + w x_W) + n x_N) + s x_S) * inv_d, an
-// off-grid neighbour's product selected away.
+// off-grid neighbour's product selected away.  gs_at keeps its own copy of
+// this expression (and apply_at of apply_combine's): routing them through
+// these functions changed the persistent kernel's schedule and made the
+// 2000-problem uneven batch 5-13% slower.  The copies must stay identical;
+// the grid path's walks, which use these, are checked against them bit for
+// bit.
 template <class T>
 DARCY_HD __forceinline__ T gs_combine(T e, T w, T nn, T s, T bk, T xE, T xW, T xN, T xS, bool hE, bool hW, bool hN, bool hS) {
     const T inv = inv_from_faces(e, w, nn, s);
@@ -164,7 +175,14 @@ DARCY_INLINE T gs_at(const LvP<T>& L, int i, int j, T bk, Ld&& ld, Cv&& cv = Cv{
     const auto rE = ld(i, hE ? j + 1 : j), rW = ld(i, hW ? j - 1 : j), rN = ld(hN ? i + 1 : i, j),
                rS = ld(hS ? i - 1 : i, j);
     __builtin_amdgcn_sched_barrier(0);
-    return gs_combine(e, w, nn, s, bk, cv(rE), cv(rW), cv(rN), cv(rS), hE, hW, hN, hS);
+    const T inv = inv_from_faces(e, w, nn, s);
+    const T xE = cv(rE), xW = cv(rW), xN = cv(rN), xS = cv(rS);
+    T v = bk;
+    v += hE ? e * xE : T(0);
+    v += hW ? w * xW : T(0);
+    v += hN ? nn * xN : T(0);
+    v += hS ? s * xS : T(0);
+    return v * inv;
 }
 
 // ------------------------------------------------------------------ phases
