@@ -99,15 +99,36 @@ def run_gpu(ctx, a_ptrs, u_ptrs, n, precision, steps, warmup, cycle, warm=None, 
     return res, wall / steps * 1e3, float(np.mean(kernel_ms)), float(np.mean(inf_ms))
 
 
+def host_cores() -> tuple[int, int, str]:
+    """(threads to use, cores visible, CPU model).  Threads: every core this
+    process may run on, bounded by OMP_NUM_THREADS where it is set -- on the
+    GPU boxes it is the CPU share that comes with one GPU (16), which the
+    box's rules say pools must not exceed."""
+    try:
+        visible = len(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover
+        visible = os.cpu_count() or 1
+    threads = visible
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        threads = min(threads, int(env))
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:  # pragma: no cover
+        pass
+    return max(1, threads), visible, model
+
+
 def cpu_baseline(n: int, sample: int, cycle: int) -> dict:
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
 
-    try:
-        threads = len(os.sched_getaffinity(0))
-    except AttributeError:  # pragma: no cover
-        threads = os.cpu_count() or 1
-    threads = max(1, min(threads, 16))
+    threads, visible, model = host_cores()
     a = media(n, sample, SEED0)
     if pyoracle.reference_available():
         solver, kind = pyoracle.Reference(), "reference"
@@ -123,7 +144,10 @@ def cpu_baseline(n: int, sample: int, cycle: int) -> dict:
         "cores": threads,
         "kind": kind,
         "sample": f"{sample} of the {n}x{n} media (seeds {SEED0}-{SEED0 + sample - 1}), FP64 W-cycle MG-CG, "
-                  f"OpenMP dynamic across samples; mean {iters.mean():.1f} iterations, {dt:.1f} s wall",
+                  f"OpenMP dynamic across samples on {threads} threads; mean {iters.mean():.1f} iterations, "
+                  f"{dt:.1f} s wall",
+        "cores_visible": visible,
+        "cpu_model": model,
     }
 
 
@@ -216,7 +240,7 @@ def suite_uneven(args) -> int:
         import pyoracle
 
         solver = pyoracle.Reference() if pyoracle.reference_available() else pyoracle.Oracle()
-        threads = max(1, min(len(os.sched_getaffinity(0)), 16))
+        threads, visible, model = host_cores()
         t_total, solved = 0.0, 0
         for s in np.unique(sizes):  # the reference batch needs one n per call
             idx = np.nonzero(sizes == s)[0][: args.cpu_sample // 7 + 1]
@@ -226,7 +250,8 @@ def suite_uneven(args) -> int:
             t_total += time.perf_counter() - t0
             solved += len(idx)
         line["cpu_baseline_sample"] = {"solves_per_s_on_sample": round(solved / t_total, 1), "cores": threads,
-                                       "problems": solved, "note": "per-size-group calls, sample of each group"}
+                                       "cores_visible": visible, "cpu_model": model, "problems": solved,
+                                       "note": "per-size-group calls, sample of each group"}
     print(json.dumps(line), flush=True)
     return 0
 
@@ -366,6 +391,9 @@ def main() -> int:
             "unit": "GB/s",
             "frac": round(best["achieved_GBs"] / HBM_PEAK_GBS, 4),
             "traffic": TRAFFIC_DEFAULT_BYTES if (batch, n, args.cycle) == (1000, 256, 2) else None,
+            "traffic_source": "rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE, scaled on the copy kernel: bytes between "
+                              "the L2 caches and the memory fabric (Infinity Cache hits included), an upper "
+                              "bound on HBM bytes",
             "launch_ms": round(best["kernel_ms"], 3),
             "alg_bytes_per_launch": best["alg_bytes"],
         },

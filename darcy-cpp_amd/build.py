@@ -28,7 +28,7 @@ INCLUDE = os.path.join(ROOT, "include")
 ARCH = os.environ.get("DARCY_OFFLOAD_ARCH", "gfx950")
 VERSION = "1.0.0"
 
-HIP_SOURCES = ["batch.hip", "grid.hip", "conv.hip", "capi.cpp", "net.cpp"]
+HIP_SOURCES = ["batch.hip", "grid.hip", "conv.hip", "fields.hip", "capi.cpp", "net.cpp"]
 HOST_SOURCES = ["host/api.cpp", "bindings.cpp"]
 
 HIPCC_FLAGS = [
@@ -84,9 +84,15 @@ def build(verbose: bool = False) -> dict[str, str]:
     ext = os.path.join(OUT, "_core" + sysconfig.get_config_var("EXT_SUFFIX"))
     _run(["g++", "-shared", "-fPIC", "-fopenmp", *host_objs, "-o", ext, f"-L{OUT}", "-ldarcy_hip",
           "-Wl,-rpath,$ORIGIN"])
+    # The native test of the C++ API: the API's object linked into a program
+    # against the library (the way a C++ caller uses it).
+    cpp_test = os.path.join(ROOT, "tests", "cpp", "test_api_gpu")
+    _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{INCLUDE}", os.path.join(ROOT, "tests", "cpp", "test_api_gpu.cpp"),
+          host_objs[0], "-o", cpp_test, "-fopenmp", f"-L{OUT}", "-ldarcy_hip",
+          "-Wl,-rpath,$ORIGIN/../../darcy-cpp_amd/darcy"])
     if verbose:
-        print(f"built {lib}\nbuilt {ext}")
-    return {"lib": lib, "ext": ext}
+        print(f"built {lib}\nbuilt {ext}\nbuilt {cpp_test}")
+    return {"lib": lib, "ext": ext, "cpp_test": cpp_test}
 
 
 if __name__ == "__main__":

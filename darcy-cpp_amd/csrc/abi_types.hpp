@@ -47,7 +47,11 @@ struct SolveParams {
     int32_t mixed;  // FP32 cycle inside FP64 flexible CG
     int32_t mode;
     int32_t record_history;
-    int32_t step_cells;  // cells per row-pipeline step of the global levels
+    int32_t history_cap;  // entries the history buffer holds
+    int32_t step_cells;   // cells per row-pipeline step of the global levels
+    int32_t min_coarse;   // hierarchy options (multigrid.hpp:43-46)
+    int32_t max_levels;
+    int32_t breakdown_at;  // testing: iteration (1-based) whose <p, Ap> reads as 0; 0 = off
 };
 
 struct BatchArgs {
@@ -80,6 +84,8 @@ struct GridCg {
     double tol;
     int max_iter;
     int mixed;
+    int history_cap;   // entries `history` holds
+    int breakdown_at;  // testing: iteration (1-based) whose <p, Ap> reads as 0; 0 = off
 };
 
 }  // namespace darcy_hip

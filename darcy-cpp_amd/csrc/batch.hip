@@ -35,7 +35,7 @@ DARCY_INLINE void run_problem(Shared& s, T* pool, float* tail, const Problem& pb
 
     const bool mg = P.mode == kModeApplyPreconditioner || (P.mode == kModeSolve && P.precond == kPcMultigrid);
     if (threadIdx.x == 0) {
-        plan_slot(slot, n, mixed, &s.h);
+        plan_slot(slot, n, mixed, &s.h, P.min_coarse, P.max_levels);
         plan_lds<T, LdsSide<T>::value>(s.h, s.t);
     }
     __syncthreads();
@@ -143,8 +143,10 @@ DARCY_INLINE void run_problem(Shared& s, T* pool, float* tail, const Problem& pb
         return;
     }
     double res = pb.in ? sqrt(s.red[1]) / bnorm : 1.0;
+    // History entries past history_cap are not stored (the host reports at
+    // most that many).
     const bool hist = P.record_history && pb.history;
-    if (hist && threadIdx.x == 0) pb.history[0] = res;
+    if (hist && threadIdx.x == 0 && P.history_cap > 0) pb.history[0] = res;
     if (pb.in && res < P.tol) {
         if (threadIdx.x == 0) st = ProblemStats{0, kConverged, levels, hist ? 1 : 0, res, -1};
         return;
@@ -189,7 +191,8 @@ DARCY_INLINE void run_problem(Shared& s, T* pool, float* tail, const Problem& pb
         pending = false;
         reduce_parts(s, 1, C);
         const double pap = s.red[0];
-        if (!(fabs(pap) > 0.0) || !isfinite(pap)) {  // solver.hpp:149-155
+        // solver.hpp:149-155; breakdown_at forces the exit for tests.
+        if (!(fabs(pap) > 0.0) || !isfinite(pap) || it + 1 == P.breakdown_at) {
             status = kBreakdown;
             iters = it;
             break;
@@ -210,7 +213,7 @@ DARCY_INLINE void run_problem(Shared& s, T* pool, float* tail, const Problem& pb
         reduce_parts(s, 1, C);
         res = sqrt(s.red[0]) / bnorm;
         iters = it + 1;
-        if (hist && threadIdx.x == 0) pb.history[iters] = res;
+        if (hist && threadIdx.x == 0 && iters < P.history_cap) pb.history[iters] = res;
         if (res < P.tol) {
             status = kConverged;
             break;

@@ -7,6 +7,8 @@
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
 
+#include <iomanip>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -70,10 +72,18 @@ Forcing make_rhs(const py::object& f, index_t n) {
     return r;
 }
 
+// A new (n, n) array owning a copy of `data` (pybind11 copies a pointer it
+// is given without a base object), so callers may write to it freely.
 py::array_t<double> to_numpy(const std::vector<double>& data, index_t n) {
-    py::array_t<double> out(std::vector<py::ssize_t>{static_cast<py::ssize_t>(n), static_cast<py::ssize_t>(n)});
-    std::copy(data.begin(), data.end(), static_cast<double*>(out.request().ptr));
-    return out;
+    const py::ssize_t side = static_cast<py::ssize_t>(n);
+    return py::array_t<double>({side, side}, data.data());
+}
+
+// The compiler of this host module (the kernels are built by hipcc).
+std::string host_compiler() {
+    std::string v = __VERSION__;
+    if (v.find("clang") == std::string::npos && v.find("Clang") == std::string::npos) v = "gcc " + v;
+    return v + "; kernels: hipcc for gfx950";
 }
 
 index_t require_square_2d(const py::buffer_info& buf, const char* what) {
@@ -91,27 +101,31 @@ struct PyResult {
     std::vector<double> history;
 };
 
+std::string describe(const PyResult& r) {
+    std::ostringstream os;
+    os << "SolveResult(n=" << r.u.shape(0) << ", iterations=" << r.iterations << ", residual=" << std::scientific
+       << std::setprecision(3) << r.residual << ", converged=" << (r.converged ? "True" : "False")
+       << ", levels=" << r.levels << ")";
+    return os.str();
+}
+
 }  // namespace
 
 PYBIND11_MODULE(_core, m) {
     m.doc() =
-        "Matrix-free multigrid-preconditioned finite-volume solver for 2-D Darcy flow on AMD Instinct MI355X.\n"
-        "Low-level extension module; prefer the `darcy` package API.";
+        "darcy._core: the compiled boundary of the GPU solve path (C++17 API over the HIP kernels).\n"
+        "The `darcy` package wraps these functions with keyword-only arguments and dataclass results.";
 
     py::register_exception<InvalidInput>(m, "InvalidInput", PyExc_ValueError);
 
-    py::class_<PyResult>(m, "SolveResult", "Outcome of a single Darcy solve.")
+    py::class_<PyResult>(m, "SolveResult", "Solution u and the convergence record of one solve.")
         .def_readonly("u", &PyResult::u)
         .def_readonly("iterations", &PyResult::iterations)
         .def_readonly("residual", &PyResult::residual)
         .def_readonly("converged", &PyResult::converged)
         .def_readonly("levels", &PyResult::levels)
         .def_readonly("history", &PyResult::history)
-        .def("__repr__", [](const PyResult& r) {
-            return "SolveResult(iterations=" + std::to_string(r.iterations) + ", residual=" +
-                   std::to_string(r.residual) + ", converged=" + (r.converged ? "True" : "False") +
-                   ", levels=" + std::to_string(r.levels) + ")";
-        });
+        .def("__repr__", &describe);
 
     m.def(
         "solve",
@@ -150,7 +164,7 @@ PYBIND11_MODULE(_core, m) {
         py::arg("preconditioner") = "multigrid", py::arg("pre_sweeps") = 1, py::arg("post_sweeps") = 1,
         py::arg("cycle_index") = 2, py::arg("record_history") = false, py::arg("u0") = py::none(),
         py::arg("precision") = "fp64",
-        "Solve -div(a grad u) = f with homogeneous Dirichlet boundary conditions.");
+        "u with -div(a grad u) = f in the unit square and u = 0 on its boundary, on the GPU.");
 
     m.def(
         "solve_batch",
@@ -180,7 +194,7 @@ PYBIND11_MODULE(_core, m) {
         py::arg("a"), py::arg("f") = 1.0, py::arg("tol") = 1e-10, py::arg("max_iter") = 10000,
         py::arg("preconditioner") = "multigrid", py::arg("pre_sweeps") = 1, py::arg("post_sweeps") = 1,
         py::arg("cycle_index") = 2, py::arg("threads") = 0, py::arg("precision") = "fp64",
-        "Solve a batch of independent Darcy problems in one persistent GPU launch.");
+        "(u, iterations, residuals, converged) of a (batch, n, n) stack, one persistent GPU launch.");
 
     m.def(
         "solve_batch_uneven",
@@ -213,7 +227,7 @@ PYBIND11_MODULE(_core, m) {
         py::arg("a"), py::arg("f") = 1.0, py::arg("tol") = 1e-10, py::arg("max_iter") = 10000,
         py::arg("preconditioner") = "multigrid", py::arg("pre_sweeps") = 1, py::arg("post_sweeps") = 1,
         py::arg("cycle_index") = 2, py::arg("precision") = "fp64",
-        "Solve problems of different grid sizes in one persistent GPU launch.");
+        "A list of square media of any sides, solved together in one persistent GPU launch.");
 
     m.def(
         "apply_operator",
@@ -231,7 +245,7 @@ PYBIND11_MODULE(_core, m) {
             }
             return to_numpy(y, n);
         },
-        py::arg("a"), py::arg("x"), "Apply the finite-volume operator A to x (matrix-free).");
+        py::arg("a"), py::arg("x"), "y = A x for the harmonic-face finite-volume operator of a, computed on the GPU.");
 
     m.def(
         "apply_preconditioner",
@@ -258,7 +272,7 @@ PYBIND11_MODULE(_core, m) {
         },
         py::arg("a"), py::arg("r"), py::arg("pre_sweeps") = 1, py::arg("post_sweeps") = 1,
         py::arg("cycle_index") = 2, py::arg("precision") = "fp64",
-        "Apply one multigrid cycle to r from a zero initial guess.");
+        "z = M r: one multigrid cycle from z = 0 on the GPU (the CG preconditioner).");
 
     m.def("num_threads", &max_threads, "Host threads available to the library.");
     m.def("has_openmp", &has_openmp, "Whether the host layer was compiled with OpenMP.");
@@ -278,12 +292,5 @@ PYBIND11_MODULE(_core, m) {
     });
 
     m.attr("__version__") = DARCY_VERSION;
-    m.attr("__compiler__") =
-#if defined(__clang__)
-        "clang " __clang_version__;
-#elif defined(__GNUC__)
-        "gcc " __VERSION__;
-#else
-        "unknown";
-#endif
+    m.attr("__compiler__") = host_compiler();
 }

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <functional>
@@ -34,6 +35,9 @@ extern "C" hipError_t darcy_grid_cg(int which, const GridCg* g, const double* po
                                     hipStream_t s);
 extern "C" hipError_t darcy_hip_batch_occupancy(int mixed, int* blocks_per_cu);
 extern "C" hipError_t darcy_hip_launch_copy(const void* src, void* dst, size_t bytes, int grid, hipStream_t stream);
+extern "C" hipError_t darcy_hip_launch_grf(int n, int count, double alpha, double tau, const double* xi,
+                                           double* cmat, double* amp, double* t, double* g, double* a, double hi,
+                                           double lo, hipStream_t s);
 
 namespace {
 
@@ -66,12 +70,29 @@ int check_opts(darcy_ctx* ctx, const darcy_opts& o) {
         return fail(ctx, DARCY_INVALID_INPUT, "cycle_index must be 1 (V-cycle) or 2 (W-cycle)");
     if (o.precision != DARCY_FP64 && o.precision != DARCY_MIXED)
         return fail(ctx, DARCY_INVALID_INPUT, "precision must be 0 (fp64) or 1 (mixed)");
+    if (o.breakdown_at < 0) return fail(ctx, DARCY_INVALID_INPUT, "breakdown_at must be 0 (off) or positive");
     return DARCY_OK;
 }
 
 // Largest grid side one call may use: flat indices and face counts are
 // 32-bit inside the kernels.
 constexpr int64_t kMaxSide = 32768;
+
+// Checks every entry point makes before any device work: the side, and the
+// arrays every solve reads and writes.
+int check_problem(darcy_ctx* ctx, int64_t n, const double* a, const double* out) {
+    if (n < 1) return fail(ctx, DARCY_INVALID_INPUT, "grid size must be at least 1");
+    if (n > kMaxSide) return fail(ctx, DARCY_INVALID_INPUT, "grid size exceeds 32768");
+    if (!a || !out) return fail(ctx, DARCY_INVALID_INPUT, "null array argument");
+    return DARCY_OK;
+}
+
+// Entries of a history buffer: max_iter + 1 are all a solve can write, and
+// the caller's capacity bounds them.
+size_t history_entries(const darcy_opts& o, int32_t cap) {
+    const size_t want = static_cast<size_t>(o.max_iter) + 1;
+    return std::min<size_t>(want, static_cast<size_t>(std::max<int32_t>(cap, 0)));
+}
 
 struct Job {
     int mode;
@@ -89,6 +110,59 @@ struct Job {
     bool device_ptrs;
 };
 
+// Per-problem stats for the caller, and the reference's message for the
+// lowest problem with an invalid permeability (solver.hpp:68-77).
+int report(darcy_ctx* ctx, const std::vector<ProblemStats>& st, darcy_stats* out, int32_t history_cap) {
+    const int64_t count = static_cast<int64_t>(st.size());
+    int64_t first_bad_problem = -1;
+    for (int64_t i = 0; i < count; ++i) {
+        const ProblemStats& s = st[i];
+        if (out) {
+            darcy_stats& d = out[i];
+            d.iterations = s.iterations;
+            d.converged = s.status == kConverged ? 1 : 0;
+            d.levels = s.levels;
+            d.status = s.status;
+            d.residual = s.residual;
+            d.bad_index = s.status == kInvalid ? s.bad_index : -1;
+            d.history_len = std::min<int32_t>(s.history_len, history_cap);
+            d.reserved = 0;
+        }
+        if (s.status == kInvalid && first_bad_problem < 0) first_bad_problem = i;
+    }
+    if (first_bad_problem >= 0) {
+        char msg[256];
+        std::snprintf(msg, sizeof msg,
+                      "permeability must be finite and strictly positive everywhere; offending value at flat index "
+                      "%lld%s",
+                      static_cast<long long>(st[first_bad_problem].bad_index),
+                      count > 1 ? (" of problem " + std::to_string(first_bad_problem)).c_str() : "");
+        return fail(ctx, DARCY_INVALID_INPUT, msg);
+    }
+    ctx->error.clear();
+    return DARCY_OK;
+}
+
+// Solve parameters of the persistent kernel from the options.
+SolveParams solve_params(const darcy_ctx* ctx, const darcy_opts& o, int mode, bool mixed, int32_t history_cap) {
+    SolveParams p{};
+    p.tol = o.tol;
+    p.max_iter = o.max_iter;
+    p.precond = o.preconditioner;
+    p.pre = o.pre_sweeps;
+    p.post = o.post_sweeps;
+    p.gamma = o.cycle_index;
+    p.mixed = mixed ? 1 : 0;
+    p.mode = mode;
+    p.record_history = history_cap > 0 ? 1 : 0;
+    p.history_cap = history_cap;
+    p.step_cells = ctx->batch_step_cells;
+    p.min_coarse = o.min_coarse_n;
+    p.max_levels = o.max_levels;
+    p.breakdown_at = o.breakdown_at;
+    return p;
+}
+
 int run(darcy_ctx* ctx, const Job& job) {
     const darcy_opts& o = *job.opts;
     const bool mg = job.mode == kModeApplyPreconditioner || o.preconditioner == DARCY_PC_MULTIGRID;
@@ -97,36 +171,46 @@ int run(darcy_ctx* ctx, const Job& job) {
     if (count == 0) return DARCY_OK;
     int nmax = 0;
     for (int64_t i = 0; i < count; ++i) {
-        if (job.n[i] < 1) return fail(ctx, DARCY_INVALID_INPUT, "grid size must be at least 1");
-        if (job.n[i] > kMaxSide) return fail(ctx, DARCY_INVALID_INPUT, "grid size exceeds 32768");
-        if (!job.a[i] || !job.out[i]) return fail(ctx, DARCY_INVALID_INPUT, "null array argument");
+        if (const int rc = check_problem(ctx, job.n[i], job.a[i], job.out[i])) return rc;
         nmax = std::max<int>(nmax, static_cast<int>(job.n[i]));
     }
     HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
 
     // Device copies of host inputs; identical host pointers (a shared rhs)
-    // are uploaded once.
+    // are uploaded once, with the largest size any problem reads through
+    // them (a pointer shared by problems of different sides).
     std::vector<Problem> probs(static_cast<size_t>(count));
     const bool hist = job.mode == kModeSolve && o.record_history && job.history && count == 1;
+    const size_t hist_n = hist ? history_entries(o, job.history_cap) : 0;
     size_t stage_bytes = 0;
+    std::unordered_map<const void*, size_t> want;  // host ptr -> bytes
+    auto note = [&](const double* p, size_t bytes) {
+        size_t& w = want[p];
+        w = std::max(w, bytes);
+    };
     std::unordered_map<const void*, size_t> staged;
     std::vector<std::pair<const void*, size_t>> uploads;  // host ptr, bytes
-    auto plan_upload = [&](const double* p, size_t cells) -> size_t {
+    auto plan_upload = [&](const double* p) -> size_t {
         auto it = staged.find(p);
         if (it != staged.end()) return it->second;
-        const size_t off = stage_bytes;
-        stage_bytes += align_up(cells * 8);
+        const size_t off = stage_bytes, bytes = want.at(p);
+        stage_bytes += align_up(bytes);
         staged.emplace(p, off);
-        uploads.emplace_back(p, cells * 8);
+        uploads.emplace_back(p, bytes);
         return off;
     };
     std::vector<size_t> off_a(count), off_rhs(count, SIZE_MAX), off_in(count, SIZE_MAX), off_out(count);
     if (!job.device_ptrs) {
         for (int64_t i = 0; i < count; ++i) {
-            const size_t cells = static_cast<size_t>(job.n[i] * job.n[i]);
-            off_a[i] = plan_upload(job.a[i], cells);
-            if (job.rhs && job.rhs[i]) off_rhs[i] = plan_upload(job.rhs[i], cells);
-            if (job.in && job.in[i]) off_in[i] = plan_upload(job.in[i], cells);
+            const size_t bytes = static_cast<size_t>(job.n[i] * job.n[i]) * 8;
+            note(job.a[i], bytes);
+            if (job.rhs && job.rhs[i]) note(job.rhs[i], bytes);
+            if (job.in && job.in[i]) note(job.in[i], bytes);
+        }
+        for (int64_t i = 0; i < count; ++i) {
+            off_a[i] = plan_upload(job.a[i]);
+            if (job.rhs && job.rhs[i]) off_rhs[i] = plan_upload(job.rhs[i]);
+            if (job.in && job.in[i]) off_in[i] = plan_upload(job.in[i]);
         }
         for (int64_t i = 0; i < count; ++i) {
             off_out[i] = stage_bytes;
@@ -134,7 +218,7 @@ int run(darcy_ctx* ctx, const Job& job) {
         }
     }
     size_t hist_off = stage_bytes;
-    if (hist && !job.device_ptrs) stage_bytes += align_up(static_cast<size_t>(o.max_iter + 1) * 8);
+    if (hist && !job.device_ptrs) stage_bytes += align_up(hist_n * 8);
     HIP_TRY(ctx->staging.reserve(std::max<size_t>(stage_bytes, 256)), "staging allocation");
     char* stage = ctx->staging.as<char>();
     for (const auto& up : uploads)
@@ -170,7 +254,7 @@ int run(darcy_ctx* ctx, const Job& job) {
     HIP_TRY(darcy_hip_batch_occupancy(mixed, &per_cu), "occupancy query");
     per_cu = std::max(per_cu, 1);
     Hier scratch_h;
-    const size_t slot_bytes = plan_slot(nullptr, nmax, mixed, &scratch_h);
+    const size_t slot_bytes = plan_slot(nullptr, nmax, mixed, &scratch_h, o.min_coarse_n, o.max_levels);
     int64_t grid = std::min<int64_t>(count, static_cast<int64_t>(ctx->cus) * per_cu);
     // Keep the workspace within a quarter of free device memory.
     size_t free_b = 0, total_b = 0;
@@ -197,16 +281,7 @@ int run(darcy_ctx* ctx, const Job& job) {
     args.slots = ctx->slots.as<char>();
     args.slot_bytes = slot_bytes;
     args.count = static_cast<int32_t>(count);
-    args.prm.tol = o.tol;
-    args.prm.max_iter = o.max_iter;
-    args.prm.precond = o.preconditioner;
-    args.prm.pre = o.pre_sweeps;
-    args.prm.post = o.post_sweeps;
-    args.prm.gamma = o.cycle_index;
-    args.prm.mixed = mixed ? 1 : 0;
-    args.prm.mode = job.mode;
-    args.prm.record_history = hist ? 1 : 0;
-    args.prm.step_cells = ctx->batch_step_cells;
+    args.prm = solve_params(ctx, o, job.mode, mixed, static_cast<int32_t>(hist_n));
 
     HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream), "event record");
     HIP_TRY(darcy_hip_launch_batch(&args, static_cast<int>(grid), mixed, ctx->stream), "solve kernel launch");
@@ -232,33 +307,7 @@ int run(darcy_ctx* ctx, const Job& job) {
                     "history readback");
     }
 
-    int64_t first_bad_problem = -1;
-    for (int64_t i = 0; i < count; ++i) {
-        const ProblemStats& s = st[i];
-        if (job.stats) {
-            darcy_stats& d = job.stats[i];
-            d.iterations = s.iterations;
-            d.converged = s.status == kConverged ? 1 : 0;
-            d.levels = s.levels;
-            d.status = s.status;
-            d.residual = s.residual;
-            d.bad_index = s.status == kInvalid ? s.bad_index : -1;
-            d.history_len = std::min<int32_t>(s.history_len, job.history_cap);
-            d.reserved = 0;
-        }
-        if (s.status == kInvalid && first_bad_problem < 0) first_bad_problem = i;
-    }
-    if (first_bad_problem >= 0) {
-        char msg[256];
-        std::snprintf(msg, sizeof msg,
-                      "permeability must be finite and strictly positive everywhere; offending value at flat index "
-                      "%lld%s",
-                      static_cast<long long>(st[first_bad_problem].bad_index),
-                      count > 1 ? (" of problem " + std::to_string(first_bad_problem)).c_str() : "");
-        return fail(ctx, DARCY_INVALID_INPUT, msg);
-    }
-    ctx->error.clear();
-    return DARCY_OK;
+    return report(ctx, st, job.stats, job.history_cap);
 }
 
 // ------------------------------------------------------------ grid path
@@ -285,9 +334,9 @@ int run_grid(darcy_ctx* ctx, const Job& job) {
     // Workspace: slot, device Hier, state, bad-index word, tail matrix,
     // history, staging of host inputs/outputs.
     Hier h;
-    const size_t slot_bytes = plan_slot(nullptr, n, mixed, &h);
-    const size_t hist_n = static_cast<size_t>(o.max_iter) + 1;
+    const size_t slot_bytes = plan_slot(nullptr, n, mixed, &h, o.min_coarse_n, o.max_levels);
     const bool hist = o.record_history && job.history;
+    const size_t hist_n = hist ? history_entries(o, job.history_cap) : 0;
     size_t off = 0;
     auto take = [&](size_t b) {
         const size_t at = off;
@@ -301,7 +350,7 @@ int run_grid(darcy_ctx* ctx, const Job& job) {
                  o_u0 = host && job.in && job.in[0] ? take(cells) : 0, o_u = host ? take(cells) : 0;
     HIP_TRY(ctx->grid.reserve(off), "workspace allocation");
     char* base = ctx->grid.as<char>();
-    plan_slot(base + o_slot, n, mixed, &h);
+    plan_slot(base + o_slot, n, mixed, &h, o.min_coarse_n, o.max_levels);
     Hier* hd = reinterpret_cast<Hier*>(base + o_h);
     GridState* st = reinterpret_cast<GridState*>(base + o_st);
     auto* bad = reinterpret_cast<unsigned long long*>(base + o_bad);
@@ -346,7 +395,8 @@ int run_grid(darcy_ctx* ctx, const Job& job) {
     const CycleOptsC co{o.pre_sweeps, o.post_sweeps, o.cycle_index};
     if (mixed && h.L > 1) HIP_TRY(darcy_grid_tail_matrix(hd, co, tail, s), "tail matrix");
 
-    GridCg g{hd, st, a, rhs, job.f ? job.f[0] : 1.0, u0, u, history, o.tol, o.max_iter, mixed ? 1 : 0};
+    GridCg g{hd,    st,         a,          rhs,          job.f ? job.f[0] : 1.0,           u0, u, history,
+             o.tol, o.max_iter, mixed ? 1 : 0, static_cast<int>(hist_n), o.breakdown_at};
     // CG kernels: the stencil apply on up to 8 blocks per CU; the streaming
     // rz and update kernels on grid_cg_per_cu (each wave takes several rounds
     // of two chunks).
@@ -479,7 +529,7 @@ int run_grid(darcy_ctx* ctx, const Job& job) {
     ctx->last_ms = ms;
     // A breakdown reports the iterations completed before it (solver.hpp:149-155).
     const int iters = hs.it;
-    const int32_t hist_len = hist && !zero_rhs ? std::min<int32_t>(iters + 1, job.history_cap) : 0;
+    const int32_t hist_len = hist && !zero_rhs ? std::min<int32_t>(iters + 1, static_cast<int32_t>(hist_n)) : 0;
     if (hist && host && hist_len > 0)
         HIP_TRY(hipMemcpy(job.history, history, static_cast<size_t>(hist_len) * 8, hipMemcpyDeviceToHost),
                 "history readback");
@@ -511,6 +561,9 @@ extern "C" void darcy_default_opts(darcy_opts* o) {
     o->record_history = 0;
     o->precision = DARCY_FP64;
     o->device_pointers = 0;
+    o->min_coarse_n = 1;
+    o->max_levels = 32;
+    o->breakdown_at = 0;
 }
 
 extern "C" int darcy_ctx_create(int device, darcy_ctx** out) {
@@ -540,7 +593,8 @@ extern "C" void darcy_ctx_destroy(darcy_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->cycle_exec) (void)hipGraphExecDestroy(c->cycle_exec);
     if (c->cycle_graph) (void)hipGraphDestroy(c->cycle_graph);
-    for (DevBuf* b : {&c->slots, &c->probs, &c->stats, &c->order, &c->counter, &c->staging, &c->grid}) b->release();
+    for (DevBuf* b : {&c->slots, &c->probs, &c->stats, &c->order, &c->counter, &c->staging, &c->grid, &c->fields})
+        b->release();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -554,7 +608,7 @@ extern "C" int darcy_solve(darcy_ctx* ctx, const double* a, const double* rhs, d
                            double* history, int32_t history_cap) {
     if (!ctx) return DARCY_INVALID_INPUT;
     std::lock_guard<std::mutex> lock(ctx->mu);
-    if (n < 1) return fail(ctx, DARCY_INVALID_INPUT, "grid size must be at least 1");
+    if (const int rc = check_problem(ctx, n, a, u_out)) return rc;
     darcy_opts o;
     darcy_default_opts(&o);
     if (opts) o = *opts;
@@ -611,6 +665,21 @@ extern "C" int darcy_apply_preconditioner(darcy_ctx* ctx, const double* a, int64
     if (const int rc = check_opts(ctx, o)) return rc;
     Job job{kModeApplyPreconditioner, &o, 1, &n, &a, nullptr, nullptr, &r, &z, nullptr, nullptr, 0,
             device_pointers != 0};
+    return run(ctx, job);
+}
+
+extern "C" int darcy_apply_preconditioner_opts(darcy_ctx* ctx, const double* a, int64_t n, const double* r,
+                                               double* z, const darcy_opts* opts) {
+    if (!ctx) return DARCY_INVALID_INPUT;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (!r) return fail(ctx, DARCY_INVALID_INPUT, "r must have the same shape as a");
+    darcy_opts o;
+    darcy_default_opts(&o);
+    if (opts) o = *opts;
+    o.preconditioner = DARCY_PC_MULTIGRID;
+    if (const int rc = check_opts(ctx, o)) return rc;
+    Job job{kModeApplyPreconditioner, &o, 1, &n, &a, nullptr, nullptr, &r, &z, nullptr, nullptr, 0,
+            o.device_pointers != 0};
     return run(ctx, job);
 }
 
@@ -725,5 +794,68 @@ extern "C" int darcy_ctx_get_tuning(const darcy_ctx* ctx, const char* key, int32
         *value = ctx->batch_step_cells;
     else
         return DARCY_INVALID_INPUT;
+    return DARCY_OK;
+}
+
+extern "C" int darcy_grf_fields(darcy_ctx* ctx, int64_t count, int64_t n, double alpha, double tau, const double* xi,
+                                double hi, double lo, double* grf_out, double* a_out, int32_t device_pointers) {
+    if (!ctx) return DARCY_INVALID_INPUT;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (n < 1) return fail(ctx, DARCY_INVALID_INPUT, "n must be positive");
+    if (n > kMaxSide) return fail(ctx, DARCY_INVALID_INPUT, "grid size exceeds 32768");
+    if (count < 0) return fail(ctx, DARCY_INVALID_INPUT, "count must not be negative");
+    if (!(std::isfinite(alpha) && std::isfinite(tau) && tau > 0.0))
+        return fail(ctx, DARCY_INVALID_INPUT, "alpha must be finite and tau finite and positive");
+    if (a_out && !(hi > 0.0 && lo > 0.0))
+        return fail(ctx, DARCY_INVALID_INPUT, "permeability phases must be strictly positive");
+    if (count > 0 && (!xi || (!grf_out && !a_out))) return fail(ctx, DARCY_INVALID_INPUT, "null array argument");
+    if (count == 0) return DARCY_OK;
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    hipStream_t s = ctx->stream;
+    const size_t plane = static_cast<size_t>(n) * static_cast<size_t>(n), pb = plane * 8;
+    // Samples per launch: the whole call with device pointers; otherwise
+    // chunks of about 2^27 cells (1 GiB per staged array), at most 4096.
+    const bool dev = device_pointers != 0;
+    int64_t chunk = dev ? count : std::max<int64_t>(1, static_cast<int64_t>((size_t{1} << 27) / plane));
+    chunk = std::min<int64_t>(std::min<int64_t>(chunk, count), 4096);
+    const size_t per = static_cast<size_t>(chunk) * pb;
+    const int nout = (grf_out ? 1 : 0) + (a_out ? 1 : 0);
+    const size_t o_c = 0, o_amp = align_up(pb), o_t = o_amp + align_up(pb), o_x = o_t + align_up(per),
+                 o_g = o_x + (dev ? 0 : align_up(per)), total = o_g + (dev ? 0 : nout * align_up(per));
+    HIP_TRY(ctx->fields.reserve(total), "workspace allocation");
+    char* base = ctx->fields.as<char>();
+    double* cmat = reinterpret_cast<double*>(base + o_c);
+    double* amp = reinterpret_cast<double*>(base + o_amp);
+    double* t = reinterpret_cast<double*>(base + o_t);
+    HIP_TRY(hipEventRecord(ctx->ev0, s), "event record");
+    for (int64_t b0 = 0; b0 < count; b0 += chunk) {
+        const int64_t cnt = std::min<int64_t>(chunk, count - b0);
+        const size_t off = static_cast<size_t>(b0) * plane, bytes = static_cast<size_t>(cnt) * pb;
+        const double* x = xi + off;
+        double* g = grf_out ? grf_out + off : nullptr;
+        double* a = a_out ? a_out + off : nullptr;
+        if (!dev) {
+            double* xs = reinterpret_cast<double*>(base + o_x);
+            HIP_TRY(hipMemcpyAsync(xs, x, bytes, hipMemcpyHostToDevice, s), "upload");
+            x = xs;
+            double* out = reinterpret_cast<double*>(base + o_g);
+            g = grf_out ? out : nullptr;
+            a = a_out ? out + (grf_out ? per / 8 : 0) : nullptr;
+        }
+        HIP_TRY(darcy_hip_launch_grf(static_cast<int>(n), static_cast<int>(cnt), alpha, tau, x, cmat, amp, t, g, a,
+                                     hi, lo, s),
+                "field kernels");
+        if (!dev) {
+            if (grf_out) HIP_TRY(hipMemcpyAsync(grf_out + off, g, bytes, hipMemcpyDeviceToHost, s), "readback");
+            if (a_out) HIP_TRY(hipMemcpyAsync(a_out + off, a, bytes, hipMemcpyDeviceToHost, s), "readback");
+            HIP_TRY(hipStreamSynchronize(s), "fields");
+        }
+    }
+    HIP_TRY(hipEventRecord(ctx->ev1, s), "event record");
+    HIP_TRY(hipStreamSynchronize(s), "fields");
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "event timing");
+    ctx->last_ms = ms;
+    ctx->error.clear();
     return DARCY_OK;
 }

@@ -46,7 +46,7 @@ struct darcy_ctx {
     double last_ms = 0.0;
     std::string error;
     std::mutex mu;
-    darcy_hip::DevBuf slots, probs, stats, order, counter, staging, grid;
+    darcy_hip::DevBuf slots, probs, stats, order, counter, staging, grid, fields;
     // Single solves with n > grid_min_n run on the grid path, with
     // levels of side <= grid_tail_side in one workgroup (tuned values);
     // grid_graph != 0 replays the cycle from a captured HIP graph.

@@ -253,21 +253,22 @@ __global__ __launch_bounds__(kFinalThreads) void kg_final(GridCg g) {
             s.status = kConverged;
         else if (g.u0 && s.res < g.tol)
             s.status = kConverged;
-        if (g.history) g.history[0] = s.res;
+        if (g.history && g.history_cap > 0) g.history[0] = s.res;
     } else if constexpr (KIND == 1) {
         if (!s.first) s.beta = g.mixed ? (-s.alpha * s1) / s.rz : s0 / s.rz;
         s.rz = s0;
     } else if constexpr (KIND == 2) {
         s.pap = s0;
-        if (!(fabs(s0) > 0.0) || !isfinite(s0))
+        if (!(fabs(s0) > 0.0) || !isfinite(s0) || s.it + 1 == g.breakdown_at)
             s.status = kBreakdown;
         else
             s.alpha = s.rz / s0;
         s.first = 0;
     } else {
+        if (s.status != kRunning) return;  // a breakdown in the apply step ended the solve
         s.res = sqrt(s0) / s.bnorm;
         s.it += 1;
-        if (g.history) g.history[s.it] = s.res;
+        if (g.history && s.it < g.history_cap) g.history[s.it] = s.res;
         if (s.res < g.tol)
             s.status = kConverged;
         else if (s.it >= g.max_iter)
@@ -371,8 +372,11 @@ __global__ __launch_bounds__(kBlock) void kg_diag(GridCg g, int jacobi) {
         h.z[k] = jacobi ? h.r[k] * id[k] : h.r[k];
 }
 
-// u += alpha p, r -= alpha ap, chunk sums of <r, r>.
+// u += alpha p, r -= alpha ap, chunk sums of <r, r>.  Nothing after a
+// breakdown (solver.hpp:149-155 returns before the update): the host
+// launches this kernel before it reads the state of the apply step.
 __global__ __launch_bounds__(kBlock) void kg_update(GridCg g, const double* p) {
+    if (g.st->status != kRunning) return;
     const Hier& h = *g.hd;
     const int N = h.n[0] * h.n[0];
     const double alpha = g.st->alpha;

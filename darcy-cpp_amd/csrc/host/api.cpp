@@ -3,6 +3,7 @@
 // Maps the reference's argument checks and messages (solver.hpp:85-87,
 // :182-187) and its exceptions: InvalidInput for bad input, runtime_error for
 // device failures.  One process-wide context serves all calls.
+#include <algorithm>
 #include <cmath>
 #include <memory>
 #include <mutex>
@@ -38,6 +39,13 @@ void check(int rc, darcy_ctx* ctx) {
     throw std::runtime_error(msg);
 }
 
+// The hierarchy options, clamped to what 32-bit fields hold: a side limit
+// beyond any grid side, or a depth beyond 32 levels, acts like the limit.
+void set_hierarchy(darcy_opts& c, const MultigridOptions& m) {
+    c.min_coarse_n = static_cast<int32_t>(std::min<index_t>(std::max<index_t>(m.min_coarse_n, 0), 1 << 30));
+    c.max_levels = m.max_levels;
+}
+
 darcy_opts to_c(const SolveOptions& o) {
     darcy_opts c;
     darcy_default_opts(&c);
@@ -51,12 +59,8 @@ darcy_opts to_c(const SolveOptions& o) {
     c.cycle_index = o.multigrid.cycle_index;
     c.record_history = o.record_history ? 1 : 0;
     c.precision = o.precision == Precision::Mixed ? DARCY_MIXED : DARCY_FP64;
+    set_hierarchy(c, o.multigrid);
     return c;
-}
-
-void check_supported(const MultigridOptions& m) {
-    if (m.min_coarse_n != 1 || m.max_levels != 32)
-        throw InvalidInput("the GPU path supports min_coarse_n = 1 and max_levels = 32 only");
 }
 
 // solver.hpp:85-87
@@ -64,7 +68,14 @@ void check_scalars(index_t n, const SolveOptions& o) {
     if (n < 1) throw InvalidInput("grid size must be at least 1");
     if (o.tol <= 0.0) throw InvalidInput("tol must be positive");
     if (o.max_iter < 1) throw InvalidInput("max_iter must be at least 1");
-    check_supported(o.multigrid);
+}
+
+// History buffer: one entry per iteration plus the initial residual, capped
+// so that a huge max_iter does not allocate up front (a solve that runs
+// longer keeps the first kHistoryCap entries).
+constexpr std::size_t kHistoryCap = std::size_t{1} << 20;
+std::size_t history_size(const SolveOptions& o) {
+    return std::min<std::size_t>(static_cast<std::size_t>(o.max_iter) + 1, kHistoryCap);
 }
 
 int g_threads = 0;
@@ -109,7 +120,7 @@ SolveResult solve(const double* a, const double* rhs, index_t n, const SolveOpti
     darcy_opts c = to_c(opts);
     darcy_stats st{};
     std::vector<double> hist;
-    if (opts.record_history) hist.resize(static_cast<std::size_t>(opts.max_iter) + 1);
+    if (opts.record_history) hist.resize(history_size(opts));
     check(darcy_solve(ctx, a, rhs, 1.0, n, &c, u0, out.u.data(), &st, hist.empty() ? nullptr : hist.data(),
                       static_cast<int32_t>(hist.size())),
           ctx);
@@ -133,7 +144,7 @@ SolveResult solve_constant_rhs(const double* a, index_t n, double f, const Solve
     darcy_opts c = to_c(opts);
     darcy_stats st{};
     std::vector<double> hist;
-    if (opts.record_history) hist.resize(static_cast<std::size_t>(opts.max_iter) + 1);
+    if (opts.record_history) hist.resize(history_size(opts));
     check(darcy_solve(ctx, a, nullptr, f, n, &c, nullptr, out.u.data(), &st, hist.empty() ? nullptr : hist.data(),
                       static_cast<int32_t>(hist.size())),
           ctx);
@@ -205,8 +216,19 @@ void apply_preconditioner(const double* a, index_t n, const double* r, double* z
     if (n < 1) throw InvalidInput("a must be a square 2-D array with at least one cell");
     if (mg.pre_sweeps < 1 || mg.post_sweeps < 1) throw InvalidInput("sweeps must be at least 1");
     if (mg.cycle_index < 1 || mg.cycle_index > 2) throw InvalidInput("cycle_index must be 1 or 2");
-    check_supported(mg);
     darcy_ctx* ctx = context();
+    if (mg.min_coarse_n != 1 || mg.max_levels != 32) {
+        // Non-default hierarchies go through the options form of the call.
+        darcy_opts c;
+        darcy_default_opts(&c);
+        c.pre_sweeps = mg.pre_sweeps;
+        c.post_sweeps = mg.post_sweeps;
+        c.cycle_index = mg.cycle_index;
+        c.precision = precision == Precision::Mixed ? DARCY_MIXED : DARCY_FP64;
+        set_hierarchy(c, mg);
+        check(darcy_apply_preconditioner_opts(ctx, a, n, r, z, &c), ctx);
+        return;
+    }
     check(darcy_apply_preconditioner(ctx, a, n, r, z, mg.pre_sweeps, mg.post_sweeps, mg.cycle_index,
                                      precision == Precision::Mixed ? DARCY_MIXED : DARCY_FP64, 0),
           ctx);

@@ -45,11 +45,14 @@ struct CycleOpts {
 
 DARCY_HD inline size_t align_up(size_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
 
-// Level count for side n: coarsen with (n + 1) / 2 until 1x1 or 32 levels
-// (multigrid.hpp:55-71, level.hpp:164).
-DARCY_HD inline int level_count(int n) {
+// Level count for side n: coarsen with (n + 1) / 2 while the level is wider
+// than min_coarse (and than 1) and fewer than max_levels exist
+// (multigrid.hpp:55-71, level.hpp:164).  max_levels is capped at kMaxLevels,
+// which no side below 2^31 reaches.
+DARCY_HD inline int level_count(int n, int min_coarse = 1, int max_levels = kMaxLevels) {
+    const int cap = max_levels < kMaxLevels ? max_levels : kMaxLevels;
     int l = 1;
-    while (n > 1 && l < kMaxLevels) {
+    while (l < cap && n > min_coarse && n > 1) {
         n = (n + 1) / 2;
         ++l;
     }
@@ -84,7 +87,9 @@ struct Hier {
 
 // Fill `*h` for a slot starting at `base` (nullptr to size only); returns the
 // slot's size in bytes.  `h` must not be null (device code passes LDS).
-DARCY_HD inline size_t plan_slot(char* base, int n0, bool mixed, Hier* h) {
+// min_coarse / max_levels are the hierarchy options of level_count.
+DARCY_HD inline size_t plan_slot(char* base, int n0, bool mixed, Hier* h, int min_coarse = 1,
+                                 int max_levels = kMaxLevels) {
     size_t off = 0;
     auto take = [&](size_t bytes) -> char* {
         char* p = base ? base + off : nullptr;
@@ -93,7 +98,7 @@ DARCY_HD inline size_t plan_slot(char* base, int n0, bool mixed, Hier* h) {
     };
     const size_t N0 = static_cast<size_t>(n0) * n0;
     Hier& t = *h;
-    t.L = level_count(n0);
+    t.L = level_count(n0, min_coarse, max_levels);
     t.u = reinterpret_cast<double*>(take(N0 * 8));
     t.r = reinterpret_cast<double*>(take(N0 * 8));
     t.p = reinterpret_cast<double*>(take(N0 * 8));

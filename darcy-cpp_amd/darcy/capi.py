@@ -32,6 +32,9 @@ class Opts(C.Structure):
         ("record_history", C.c_int32),
         ("precision", C.c_int32),
         ("device_pointers", C.c_int32),
+        ("min_coarse_n", C.c_int32),
+        ("max_levels", C.c_int32),
+        ("breakdown_at", C.c_int32),
     ]
 
 
@@ -68,6 +71,8 @@ def _load() -> C.CDLL:
     lib.darcy_device_info.argtypes = [vp, C.c_char_p, i32]
     lib.darcy_bandwidth_probe.argtypes = [vp, C.c_size_t, i32, dp]
     lib.darcy_solve.argtypes = [vp, vp, vp, C.c_double, i64, C.POINTER(Opts), vp, vp, C.POINTER(Stats), dp, i32]
+    lib.darcy_apply_preconditioner_opts.argtypes = [vp, vp, i64, vp, vp, C.POINTER(Opts)]
+    lib.darcy_grf_fields.argtypes = [vp, i64, i64, C.c_double, C.c_double, vp, C.c_double, C.c_double, vp, vp, i32]
     lib.darcy_ctx_set_tuning.argtypes = [vp, C.c_char_p, i32]
     lib.darcy_ctx_get_tuning.argtypes = [vp, C.c_char_p, C.POINTER(i32)]
     # Learned warm start.
@@ -167,6 +172,74 @@ class Context:
         v = C.c_int32()
         self._check(self.lib.darcy_ctx_get_tuning(self.ptr, key.encode(), C.byref(v)))
         return v.value
+
+    def _opts(self, **kw) -> Opts:
+        o = Opts()
+        self.lib.darcy_default_opts(C.byref(o))
+        for key, value in kw.items():
+            if key == "preconditioner":
+                value = PC[value]
+            elif key == "precision":
+                value = PRECISION[value]
+            setattr(o, key, int(value) if isinstance(value, bool) else value)
+        return o
+
+    def solve(self, a: np.ndarray, f: float = 1.0, *, rhs: np.ndarray | None = None, u0: np.ndarray | None = None,
+              history_cap: int | None = None, **options) -> dict:
+        """One solve of host arrays with any darcy_opts field as a keyword
+        (tol, max_iter, preconditioner, pre_sweeps, post_sweeps, cycle_index,
+        precision, record_history, min_coarse_n, max_levels, breakdown_at)."""
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        n = a.shape[0]
+        o = self._opts(**options)
+        u = np.empty((n, n))
+        st = Stats()
+        if rhs is not None:
+            rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        if u0 is not None:
+            u0 = np.ascontiguousarray(u0, dtype=np.float64)
+        cap = (o.max_iter + 1) if history_cap is None else history_cap
+        hist = np.zeros(max(cap, 1)) if o.record_history else None
+        self._check(self.lib.darcy_solve(
+            self.ptr, a.ctypes.data, None if rhs is None else rhs.ctypes.data, C.c_double(f), C.c_int64(n),
+            C.byref(o), None if u0 is None else u0.ctypes.data, u.ctypes.data, C.byref(st),
+            None if hist is None else _ptr(hist, C.c_double), cap if hist is not None else 0))
+        return {"u": u, "iterations": st.iterations, "converged": bool(st.converged), "levels": st.levels,
+                "status": st.status, "residual": st.residual,
+                "history": hist[: st.history_len].copy() if hist is not None else None}
+
+    def apply_preconditioner(self, a: np.ndarray, r: np.ndarray, **options) -> np.ndarray:
+        """z = M r with the cycle options of darcy_opts as keywords."""
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        z = np.empty_like(r)
+        o = self._opts(**options)
+        self._check(self.lib.darcy_apply_preconditioner_opts(self.ptr, a.ctypes.data, C.c_int64(a.shape[0]),
+                                                             r.ctypes.data, z.ctypes.data, C.byref(o)))
+        return z
+
+    def grf(self, xi: np.ndarray, *, alpha: float = 2.0, tau: float = 3.0, hi: float = 12.0, lo: float = 3.0,
+            field: bool = True, media: bool = True) -> tuple[np.ndarray | None, np.ndarray | None]:
+        """Gaussian random fields and two-phase media of the normals xi
+        (shape (count, n, n)), computed on the GPU: (fields, media)."""
+        xi = np.ascontiguousarray(xi, dtype=np.float64)
+        if xi.ndim == 2:
+            xi = xi[None]
+        count, n = xi.shape[0], xi.shape[1]
+        g = np.empty_like(xi) if field else None
+        a = np.empty_like(xi) if media else None
+        self._check(self.lib.darcy_grf_fields(self.ptr, count, n, C.c_double(alpha), C.c_double(tau), xi.ctypes.data,
+                                              C.c_double(hi), C.c_double(lo), g.ctypes.data if field else None,
+                                              a.ctypes.data if media else None, 0))
+        return g, a
+
+    def grf_device(self, count: int, n: int, xi_ptr: int, a_ptr: int, *, alpha: float = 2.0, tau: float = 3.0,
+                   hi: float = 12.0, lo: float = 3.0, field_ptr: int | None = None) -> None:
+        """Media (and optionally fields) of count * n * n normals already in
+        device memory, written to device memory."""
+        self._check(self.lib.darcy_grf_fields(self.ptr, count, n, C.c_double(alpha), C.c_double(tau),
+                                              C.c_void_p(xi_ptr), C.c_double(hi), C.c_double(lo),
+                                              C.c_void_p(field_ptr) if field_ptr else None, C.c_void_p(a_ptr), 1))
 
     def solve_device(self, n: int, a_ptr: int, u_ptr: int, *, u0_ptr: int | None = None, f: float = 1.0,
                      tol: float = 1e-10, max_iter: int = 10000, cycle_index: int = 2,

@@ -1,12 +1,22 @@
-"""Command line: ``python -m darcy solve | dataset | bench | info``.
+"""``python -m darcy`` -- the command line of the GPU solve path.
 
-The commands, flags, defaults, printed lines, output files and exit codes
-are those of the reference's ``darcy`` command (python/darcy/cli.py:16-200,
-benchmark.py:54-140); the solves run on the GPU through this package.  One
-option is added: ``--precision`` (``fp64``, the reference's arithmetic, or
-``mixed``, the FP32 cycle inside FP64 flexible CG).  The headline benchmark
-(batched 256^2 solves with the learned warm start) is ``bench.py`` at the
-repository root.
+Four subcommands, with the reference command's names, flags, defaults,
+report lines, ``.npz`` keys and exit codes (python/darcy/cli.py:16-205 and
+benchmark.py:54-140 of the reference), so scripts written against it keep
+working:
+
+  solve    one medium (generated, or loaded from .npy/.npz) -> report, .npz
+  dataset  (a, u) training pairs; the media are generated on the GPU
+           (darcy_grf_fields) from the reference's normal draws and solved
+           in one persistent launch
+  bench    the README table: wall time per grid and preconditioner against
+           a sparse direct solve
+  info     build and device description (JSON)
+
+Exit codes: 0 success, 1 a solve did not converge, 2 invalid input, 130
+interrupted.  ``--precision mixed`` (FP32 cycle inside FP64 flexible CG) is
+this package's addition; the batched 256^2 benchmark with the learned warm
+start is ``bench.py`` at the repository root.
 """
 
 from __future__ import annotations
@@ -15,218 +25,290 @@ import argparse
 import json
 import sys
 import time
-from collections.abc import Sequence
+from collections.abc import Callable, Sequence
+from dataclasses import dataclass
 from pathlib import Path
+from typing import Any
 
 import numpy as np
 
 import darcy
 
 
-def _solver_flags(p: argparse.ArgumentParser) -> None:
-    """cli.py:18-29."""
-    p.add_argument("--tol", type=float, default=1e-10, help="relative residual tolerance")
-    p.add_argument("--max-iter", type=int, default=10_000, help="iteration cap")
-    p.add_argument("--preconditioner", choices=["multigrid", "jacobi", "none"], default="multigrid",
-                   help="preconditioner (default: multigrid)")
-    p.add_argument("--pre-sweeps", type=int, default=1, help="pre-smoothing sweeps per level")
-    p.add_argument("--post-sweeps", type=int, default=1, help="post-smoothing sweeps per level")
-    p.add_argument("--cycle-index", type=int, default=2, choices=[1, 2], help="1 = V-cycle, 2 = W-cycle")
-    p.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
-                   help="fp64, or the FP32 cycle inside FP64 flexible CG")
+# ----------------------------------------------------------------- options
+
+@dataclass(frozen=True)
+class Flag:
+    """One command-line option: argparse keywords kept as data."""
+
+    name: str
+    kind: Any = None
+    default: Any = None
+    text: str | None = None
+    choices: tuple | None = None
+    action: str | None = None
+    nargs: str | None = None
+
+    def add_to(self, parser: argparse.ArgumentParser) -> None:
+        kw: dict[str, Any] = {}
+        for key, value in (("type", self.kind), ("default", self.default), ("help", self.text),
+                           ("choices", self.choices), ("action", self.action), ("nargs", self.nargs)):
+            if value is not None:
+                kw[key] = list(value) if key == "choices" else value
+        parser.add_argument(self.name, **kw)
 
 
-def _medium(grid: int, seed, alpha: float, tau: float) -> np.ndarray:
-    return darcy.two_phase_permeability(darcy.gaussian_random_field(grid, alpha=alpha, tau=tau, seed=seed))
+# What reaches darcy.solve / solve_batch.  The reference's five solver flags,
+# then --precision.
+SOLVER = (
+    Flag("--tol", float, 1e-10, "stop at this relative residual ||r|| / ||b||"),
+    Flag("--max-iter", int, 10_000, "largest number of CG iterations"),
+    Flag("--preconditioner", str, "multigrid", "multigrid, jacobi or none", ("multigrid", "jacobi", "none")),
+    Flag("--pre-sweeps", int, 1, "red-black sweeps before each restriction"),
+    Flag("--post-sweeps", int, 1, "red-black sweeps after each prolongation"),
+    Flag("--cycle-index", int, 2, "coarse visits per level: 1 V-cycle, 2 W-cycle", (1, 2)),
+    Flag("--precision", str, "fp64", "fp64, or mixed: FP32 cycle inside FP64 flexible CG", ("fp64", "mixed")),
+)
+
+# The random medium: a thresholded Gaussian random field.
+MEDIUM = (
+    Flag("--alpha", float, 2.0, "spectral decay of the random field (larger: smoother)"),
+    Flag("--tau", float, 3.0, "inverse correlation length of the random field"),
+    Flag("--f", float, 1.0, "the constant forcing"),
+)
 
 
-def cmd_solve(args: argparse.Namespace) -> int:
-    """One problem (cli.py:36-75): a generated or loaded field, a report, and
-    optionally (a, u) in an .npz.  Exit 0 when converged, 1 otherwise."""
-    if args.input is not None:
-        loaded = np.load(args.input)
-        a = loaded[args.key] if isinstance(loaded, np.lib.npyio.NpzFile) else loaded
-        a = np.asarray(a, dtype=np.float64)
-        if a.ndim != 2:
-            print(f"error: expected a 2-D field in {args.input}, got shape {a.shape}", file=sys.stderr)
-            return 2
+def _solver_kwargs(ns: argparse.Namespace, names: Sequence[str]) -> dict[str, Any]:
+    return {name: getattr(ns, name) for name in names}
+
+
+# ----------------------------------------------------------------- reports
+
+def _report(rows: Sequence[tuple[str, Any]]) -> None:
+    """Label / value lines, labels left-aligned in 13 columns."""
+    for label, value in rows:
+        print(f"{label:<13} {value}")
+
+
+def _medium(n: int, seed: Any, alpha: float, tau: float) -> np.ndarray:
+    grf = darcy.gaussian_random_field(n, alpha=alpha, tau=tau, seed=seed)
+    return darcy.two_phase_permeability(grf)
+
+
+# ---------------------------------------------------------------- commands
+
+COMMANDS: dict[str, tuple[str, tuple[Flag, ...], Callable[[argparse.Namespace], int]]] = {}
+
+
+def command(name: str, summary: str, *flags: Flag):
+    def register(fn: Callable[[argparse.Namespace], int]):
+        COMMANDS[name] = (summary, flags, fn)
+        return fn
+
+    return register
+
+
+def _load_field(path: Path, key: str) -> np.ndarray | str:
+    """The field stored in a .npy, or under `key` in a .npz; a message when
+    it is not two-dimensional."""
+    stored = np.load(path)
+    field = stored[key] if isinstance(stored, np.lib.npyio.NpzFile) else stored
+    field = np.asarray(field, dtype=np.float64)
+    if field.ndim != 2:
+        return f"error: expected a 2-D field in {path}, got shape {field.shape}"
+    return field
+
+
+@command(
+    "solve", "solve one problem and print a report",
+    Flag("--grid", int, 128, "side of the generated field"),
+    Flag("--seed", int, 0, "seed of the generated field"),
+    *MEDIUM,
+    Flag("--input", Path, None, "read the permeability from this .npy / .npz instead"),
+    Flag("--key", str, "a", "array name inside an --input .npz"),
+    Flag("--output", Path, None, "save a and u to this .npz"),
+    *SOLVER,
+)
+def run_solve(ns: argparse.Namespace) -> int:
+    if ns.input is None:
+        a = _medium(ns.grid, ns.seed, ns.alpha, ns.tau)
     else:
-        a = _medium(args.grid, args.seed, args.alpha, args.tau)
-
+        a = _load_field(ns.input, ns.key)
+        if isinstance(a, str):
+            print(a, file=sys.stderr)
+            return 2
     t0 = time.perf_counter()
-    r = darcy.solve(a, args.f, tol=args.tol, max_iter=args.max_iter, preconditioner=args.preconditioner,
-                    pre_sweeps=args.pre_sweeps, post_sweeps=args.post_sweeps, cycle_index=args.cycle_index,
-                    precision=args.precision)
-    elapsed = time.perf_counter() - t0
-
-    print(f"grid          {a.shape[0]} x {a.shape[0]}")
-    print(f"preconditioner {args.preconditioner} ({r.levels} levels)")
-    print(f"iterations    {r.iterations}")
-    print(f"residual      {r.residual:.3e}")
-    print(f"converged     {r.converged}")
-    print(f"wall time     {elapsed:.4f} s")
-    print(f"max |u|       {np.abs(r.u).max():.6e}")
-    if args.output is not None:
-        args.output.parent.mkdir(parents=True, exist_ok=True)
-        np.savez_compressed(args.output, a=a, u=r.u)
-        print(f"wrote         {args.output}")
-    return 0 if r.converged else 1
+    res = darcy.solve(a, ns.f, **_solver_kwargs(ns, ("tol", "max_iter", "preconditioner", "pre_sweeps",
+                                                     "post_sweeps", "cycle_index", "precision")))
+    seconds = time.perf_counter() - t0
+    n = a.shape[0]
+    rows = [("grid", f"{n} x {n}"), ("preconditioner", f"{ns.preconditioner} ({res.levels} levels)"),
+            ("iterations", res.iterations), ("residual", f"{res.residual:.3e}"), ("converged", res.converged),
+            ("wall time", f"{seconds:.4f} s"), ("max |u|", f"{np.abs(res.u).max():.6e}")]
+    if ns.output is not None:
+        ns.output.parent.mkdir(parents=True, exist_ok=True)
+        np.savez_compressed(ns.output, a=a, u=res.u)
+        rows.append(("wrote", ns.output))
+    _report(rows)
+    return 0 if res.converged else 1
 
 
-def cmd_dataset(args: argparse.Namespace) -> int:
-    """(a, u) pairs for operator learning (cli.py:77-115): n_samples media
-    drawn from one default_rng(seed) stream, one batched solve, arrays in
-    ``--dtype``.  As in the reference, only the tolerance, the iteration cap
-    and the preconditioner reach the solver.  Exit 1 if any sample fails."""
-    rng = np.random.default_rng(args.seed)
-    a_all = np.empty((args.n_samples, args.grid, args.grid), dtype=np.float64)
-    for i in range(args.n_samples):
-        a_all[i] = _medium(args.grid, rng, args.alpha, args.tau)
+# Samples whose normals are drawn, turned into media and solved together:
+# bounds host and device memory for large datasets.
+DATASET_CHUNK_CELLS = 1 << 26
 
-    t0 = time.perf_counter()
-    batch = darcy.solve_batch(a_all, args.f, tol=args.tol, max_iter=args.max_iter,
-                              preconditioner=args.preconditioner, threads=args.threads, precision=args.precision)
-    elapsed = time.perf_counter() - t0
-    if not batch.all_converged:
-        failed = int((~batch.converged).sum())
-        print(f"error: {failed}/{args.n_samples} samples did not converge", file=sys.stderr)
+
+@command(
+    "dataset", "generate (a, u) pairs for operator learning",
+    Flag("--n-samples", int, 1000, "number of media"),
+    Flag("--grid", int, 64, "side of every medium"),
+    Flag("--seed", int, 0, "seed of the one generator all media are drawn from"),
+    *MEDIUM,
+    Flag("--threads", int, 0, "host threads (the solves run as one GPU launch; kept for scripts)"),
+    Flag("--dtype", str, "float32", "dtype of the stored arrays", ("float32", "float64")),
+    Flag("--output", Path, Path("data/darcy_train.npz"), "destination .npz"),
+    *SOLVER,
+)
+def run_dataset(ns: argparse.Namespace) -> int:
+    """All media come from one default_rng(seed) stream, sample after sample
+    (the reference draws each sample's n x n normals in turn; a single
+    (count, n, n) draw is the same sequence).  The GPU turns the normals
+    into fields and media and solves them; like the reference, only the
+    tolerance, the iteration cap and the preconditioner are passed to the
+    solver (plus --precision)."""
+    from darcy import capi
+
+    n, count = ns.grid, ns.n_samples
+    if n < 1 or count < 1:
+        raise ValueError("--grid and --n-samples must be positive")
+    rng = np.random.default_rng(ns.seed)
+    media = np.empty((count, n, n), dtype=np.float64)
+    sol = np.empty_like(media)
+    iters = np.empty(count, dtype=np.int64)
+    ok = np.empty(count, dtype=bool)
+    step = max(1, DATASET_CHUNK_CELLS // (n * n))
+    seconds = 0.0
+    with capi.Context() as ctx:
+        for s0 in range(0, count, step):
+            s1 = min(count, s0 + step)
+            xi = rng.standard_normal((s1 - s0, n, n))
+            t0 = time.perf_counter()
+            _, media[s0:s1] = ctx.grf(xi, alpha=ns.alpha, tau=ns.tau, field=False)
+            out = darcy.solve_batch(media[s0:s1], ns.f, tol=ns.tol, max_iter=ns.max_iter,
+                                    preconditioner=ns.preconditioner, threads=ns.threads, precision=ns.precision)
+            seconds += time.perf_counter() - t0
+            sol[s0:s1], iters[s0:s1], ok[s0:s1] = out.u, out.iterations, out.converged
+    if not ok.all():
+        print(f"error: {int((~ok).sum())}/{count} samples did not converge", file=sys.stderr)
         return 1
-
-    args.output.parent.mkdir(parents=True, exist_ok=True)
-    np.savez_compressed(args.output, a=a_all.astype(args.dtype), u=batch.u.astype(args.dtype))
-    per = elapsed / args.n_samples
-    print(f"wrote {args.output}: {args.n_samples} samples at {args.grid}x{args.grid}")
-    print(f"  {elapsed:.2f} s total, {per * 1e3:.1f} ms/sample, {batch.iterations.mean():.1f} mean CG iterations")
-    print(f"  threads: {args.threads or darcy.num_threads()}")
+    ns.output.parent.mkdir(parents=True, exist_ok=True)
+    np.savez_compressed(ns.output, a=media.astype(ns.dtype), u=sol.astype(ns.dtype))
+    print(f"wrote {ns.output}: {count} samples at {n}x{n}")
+    print(f"  {seconds:.2f} s total, {seconds / count * 1e3:.1f} ms/sample, {iters.mean():.1f} mean CG iterations")
+    print(f"  threads: {ns.threads or darcy.num_threads()}")
     return 0
 
 
-def _best_time(fn, repeats: int):
-    best, result = float("inf"), None
-    for _ in range(repeats):
+def _min_time(fn: Callable[[], Any], repeats: int) -> tuple[float, Any]:
+    """Least wall time over `repeats` calls, and the last result."""
+    times, result = [], None
+    for _ in range(max(1, repeats)):
         t0 = time.perf_counter()
         result = fn()
-        best = min(best, time.perf_counter() - t0)
-    return best, result
+        times.append(time.perf_counter() - t0)
+    return min(times), result
 
 
-def _table(rows: list[dict]) -> str:
-    """The Markdown table of benchmark.py:123-140."""
-    lines = ["| grid | preconditioner | CG iters | levels | time (ms) | SciPy direct (ms) | speed-up | rel. err |",
-             "|" + "|".join(["---"] * 8) + "|"]
-    for r in rows:
-        d, t, e = r["scipy_seconds"], r["seconds"], r["relative_error"]
-        direct = "-" if d is None else f"{d * 1e3:.1f}"
-        speed = "-" if d is None else f"{d / t:.2f}x"
-        err = "-" if e is None else f"{e:.1e}"
-        lines.append(f"| {r['grid']} | {r['preconditioner']} | {r['iterations']} | {r['levels']} | {t * 1e3:.1f} | "
-                     f"{direct} | {speed} | {err} |")
-    return "\n".join(lines)
+TABLE_HEAD = ("| grid | preconditioner | CG iters | levels | time (ms) | SciPy direct (ms) | speed-up | rel. err |\n"
+              "|---|---|---|---|---|---|---|---|")
 
 
-def cmd_bench(args: argparse.Namespace) -> int:
-    """The reference's benchmark rows (benchmark.py:54-140) on the GPU: per
-    grid the seed-0 two-phase medium, the minimum wall time over repeats of
+def _table_line(row: dict[str, Any]) -> str:
+    direct, secs, err = row["scipy_seconds"], row["seconds"], row["relative_error"]
+    cells = [row["grid"], row["preconditioner"], row["iterations"], row["levels"], f"{secs * 1e3:.1f}",
+             "-" if direct is None else f"{direct * 1e3:.1f}", "-" if direct is None else f"{direct / secs:.2f}x",
+             "-" if err is None else f"{err:.1e}"]
+    return "| " + " | ".join(str(c) for c in cells) + " |"
+
+
+@command(
+    "bench", "time the solver per grid size against a direct solve",
+    Flag("--grids", int, [64, 128, 256, 512], "grid sides", nargs="+"),
+    Flag("--repeats", int, 3, "timed repeats per row (the minimum is kept)"),
+    Flag("--no-scipy", None, None, "leave out the sparse direct baseline", action="store_true"),
+    Flag("--preconditioners", str, ["multigrid", "jacobi"], "preconditioners to time",
+         ("multigrid", "jacobi", "none"), nargs="+"),
+    Flag("--precision", str, "fp64", "fp64 or mixed", ("fp64", "mixed")),
+    Flag("--json", Path, None, "also write the rows to this JSON file"),
+)
+def run_bench(ns: argparse.Namespace) -> int:
+    """Per grid: the seed-0 two-phase medium, the minimum wall time of
     darcy.solve per preconditioner (iteration cap 1000 for multigrid, 100000
-    otherwise), and a SciPy direct solve of the assembled operator as the
-    baseline and the error reference.  Prints the rows and the Markdown table
-    of the reference's README; --json writes the rows."""
+    otherwise), and SciPy's sparse direct solve of the same operator as the
+    baseline and the error reference."""
     from darcy.direct import solve_direct
 
     info = darcy.build_info()
     print(f"darcy-cpp {info['version']} | {info['compiler']}")
     print(f"device: {darcy.device_info()}")
     print()
-    rows = []
-    for n in args.grids:
+    rows: list[dict[str, Any]] = []
+    for n in ns.grids:
         a = _medium(n, 0, 2.0, 3.0)
-        direct_s, u_ref = (None, None) if args.no_scipy else _best_time(lambda a=a: solve_direct(a), 1)
-        for name in args.preconditioners:
-            cap = 1_000 if name == "multigrid" else 100_000
-            secs, r = _best_time(lambda a=a, name=name, cap=cap: darcy.solve(
-                a, tol=1e-10, preconditioner=name, max_iter=cap, precision=args.precision), args.repeats)
-            err = None if u_ref is None else float(np.linalg.norm(r.u - u_ref) / np.linalg.norm(u_ref))
-            row = {"grid": n, "preconditioner": name, "seconds": secs, "iterations": r.iterations,
-                   "levels": r.levels, "residual": r.residual, "scipy_seconds": direct_s, "relative_error": err}
+        direct_s, u_ref = (None, None) if ns.no_scipy else _min_time(lambda: solve_direct(a), 1)
+        for pc in ns.preconditioners:
+            cap = 1_000 if pc == "multigrid" else 100_000
+            secs, res = _min_time(lambda: darcy.solve(a, tol=1e-10, preconditioner=pc, max_iter=cap,
+                                                      precision=ns.precision), ns.repeats)
+            err = None if u_ref is None else float(np.linalg.norm(res.u - u_ref) / np.linalg.norm(u_ref))
+            row = {"grid": n, "preconditioner": pc, "seconds": secs, "iterations": res.iterations,
+                   "levels": res.levels, "residual": res.residual, "scipy_seconds": direct_s,
+                   "relative_error": err}
             rows.append(row)
-            speed = f"{direct_s / secs:.2f}x" if direct_s else "-"
-            err_s = "-" if err is None else f"{err:.1e}"
-            print(f"  {n:>5} {name:>10}  {secs * 1e3:>9.1f} ms  {r.iterations:>6} iters  speedup {speed:>7}  "
-                  f"rel.err {err_s}")
+            speed = "-" if direct_s is None else f"{direct_s / secs:.2f}x"
+            print(f"  {n:>5} {pc:>10}  {secs * 1e3:>9.1f} ms  {res.iterations:>6} iters  speedup {speed:>7}  "
+                  f"rel.err {'-' if err is None else f'{err:.1e}'}")
     print()
-    print(_table(rows))
-    if args.json is not None:
-        args.json.parent.mkdir(parents=True, exist_ok=True)
-        args.json.write_text(json.dumps(rows, indent=2))
-        print(f"wrote {args.json}")
+    print("\n".join([TABLE_HEAD] + [_table_line(r) for r in rows]))
+    if ns.json is not None:
+        ns.json.parent.mkdir(parents=True, exist_ok=True)
+        ns.json.write_text(json.dumps(rows, indent=2))
+        print(f"wrote {ns.json}")
     return 0
 
 
-def cmd_info(_: argparse.Namespace) -> int:
+@command("info", "print build and device information")
+def run_info(_: argparse.Namespace) -> int:
     info = dict(darcy.build_info())
     try:
         info["device"] = darcy.device_info()
-    except RuntimeError as exc:  # no usable GPU: report it, the build details still print
+    except RuntimeError as exc:  # no usable GPU: the build details still print
         info["device"] = f"unavailable: {exc}"
     print(json.dumps(info, indent=2))
     return 0
 
 
+# -------------------------------------------------------------- entry point
+
 def build_parser() -> argparse.ArgumentParser:
-    """cli.py:138-193."""
-    parser = argparse.ArgumentParser(prog="python -m darcy",
-                                     description="Matrix-free multigrid solver for 2-D Darcy flow (MI355X).",
-                                     formatter_class=argparse.ArgumentDefaultsHelpFormatter)
+    fmt = argparse.ArgumentDefaultsHelpFormatter
+    parser = argparse.ArgumentParser(prog="python -m darcy", formatter_class=fmt,
+                                     description="2-D Darcy flow: multigrid-preconditioned CG on an AMD MI355X.")
     parser.add_argument("--version", action="version", version=f"darcy-cpp {darcy.__version__}")
     sub = parser.add_subparsers(dest="command", required=True)
-    fmt = parser.formatter_class
-
-    solve = sub.add_parser("solve", help="solve one problem", formatter_class=fmt)
-    solve.add_argument("--grid", type=int, default=128, help="grid size when generating a field")
-    solve.add_argument("--seed", type=int, default=0, help="random seed for the generated field")
-    solve.add_argument("--alpha", type=float, default=2.0, help="GRF smoothness")
-    solve.add_argument("--tau", type=float, default=3.0, help="GRF inverse correlation length")
-    solve.add_argument("--f", type=float, default=1.0, help="constant forcing term")
-    solve.add_argument("--input", type=Path, help="load the permeability field from .npy/.npz")
-    solve.add_argument("--key", default="a", help="array key when --input is an .npz")
-    solve.add_argument("--output", type=Path, help="write (a, u) to this .npz")
-    _solver_flags(solve)
-    solve.set_defaults(func=cmd_solve)
-
-    dataset = sub.add_parser("dataset", help="generate a training dataset", formatter_class=fmt)
-    dataset.add_argument("--n-samples", type=int, default=1000)
-    dataset.add_argument("--grid", type=int, default=64)
-    dataset.add_argument("--seed", type=int, default=0)
-    dataset.add_argument("--alpha", type=float, default=2.0)
-    dataset.add_argument("--tau", type=float, default=3.0)
-    dataset.add_argument("--f", type=float, default=1.0)
-    dataset.add_argument("--threads", type=int, default=0, help="accepted for compatibility; the GPU takes the batch")
-    dataset.add_argument("--dtype", default="float32", choices=["float32", "float64"])
-    dataset.add_argument("--output", type=Path, default=Path("data/darcy_train.npz"))
-    _solver_flags(dataset)
-    dataset.set_defaults(func=cmd_dataset)
-
-    bench = sub.add_parser("bench", help="run the benchmark suite", formatter_class=fmt)
-    bench.add_argument("--grids", type=int, nargs="+", default=[64, 128, 256, 512])
-    bench.add_argument("--repeats", type=int, default=3)
-    bench.add_argument("--no-scipy", action="store_true", help="skip the SciPy direct-solver baseline")
-    bench.add_argument("--preconditioners", nargs="+", default=["multigrid", "jacobi"],
-                       choices=["multigrid", "jacobi", "none"])
-    bench.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
-    bench.add_argument("--json", type=Path, help="also write the results as JSON")
-    bench.set_defaults(func=cmd_bench)
-
-    info = sub.add_parser("info", help="print build, threading and device information")
-    info.set_defaults(func=cmd_info)
+    for name, (summary, flags, fn) in COMMANDS.items():
+        p = sub.add_parser(name, help=summary, formatter_class=fmt)
+        for flag in flags:
+            flag.add_to(p)
+        p.set_defaults(func=fn)
     return parser
 
 
 def main(argv: Sequence[str] | None = None) -> int:
-    """cli.py:196-205: input validation errors exit 2 with the message."""
-    args = build_parser().parse_args(argv)
+    ns = build_parser().parse_args(argv)
     try:
-        return int(args.func(args))
-    except ValueError as exc:
+        return int(ns.func(ns))
+    except ValueError as exc:  # input rejected by the solver or the loader
         print(f"error: {exc}", file=sys.stderr)
         return 2
     except KeyboardInterrupt:

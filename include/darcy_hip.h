@@ -52,7 +52,8 @@ extern "C" {
 typedef struct darcy_ctx darcy_ctx;
 
 /* Solver options: SolveOptions + MultigridOptions of the reference
- * (solver.hpp:40-46, multigrid.hpp:29-47) plus two additions. */
+ * (solver.hpp:40-46, multigrid.hpp:29-47) plus three additions.  Fill with
+ * darcy_default_opts before setting fields. */
 typedef struct darcy_opts {
     double tol;              /* relative residual target (default 1e-10) */
     int32_t max_iter;        /* default 10000 */
@@ -61,8 +62,13 @@ typedef struct darcy_opts {
     int32_t post_sweeps;     /* default 1 */
     int32_t cycle_index;     /* 1 = V-cycle, 2 = W-cycle (default) */
     int32_t record_history;  /* keep the relative residual per iteration */
-    int32_t precision;       /* DARCY_FP64 (default) or DARCY_MIXED */
-    int32_t device_pointers; /* 1: array arguments are device pointers */
+    int32_t precision;       /* addition: DARCY_FP64 (default) or DARCY_MIXED */
+    int32_t device_pointers; /* addition: 1: array arguments are device pointers */
+    int32_t min_coarse_n;    /* stop coarsening at a side <= this (default 1, multigrid.hpp:43-44) */
+    int32_t max_levels;      /* hierarchy depth cap (default 32, multigrid.hpp:45-46) */
+    int32_t breakdown_at;    /* addition, for tests of the breakdown exit: 0 (default) = off;
+                                k > 0 makes <p, A p> of CG iteration k read as 0, so the solve
+                                returns as solver.hpp:149-155 does, with k - 1 iterations */
 } darcy_opts;
 
 /* Outcome of one solve: the scalar fields of SolveResult (solver.hpp:48-56). */
@@ -78,7 +84,8 @@ typedef struct darcy_stats {
 } darcy_stats;
 
 /* Defaults of the reference (tol 1e-10, max_iter 10000, multigrid W-cycle,
- * one pre- and one post-sweep, FP64, host pointers). */
+ * one pre- and one post-sweep, coarsening to 1 x 1 within 32 levels, FP64,
+ * host pointers, no fault injection). */
 void darcy_default_opts(darcy_opts* opts);
 
 /* Create a context on HIP device `device`.  No reference counterpart (the
@@ -111,7 +118,8 @@ const char* darcy_last_error(const darcy_ctx* ctx);
  * Replaces darcy::solve (reference include/darcy/solver.hpp:83-174).
  * rhs: n x n forcing, or NULL for the constant f.  u0: initial guess, or
  * NULL for zero (the reference behaviour).  history: NULL, or room for
- * history_cap doubles (max_iter + 1 suffice). */
+ * history_cap doubles (max_iter + 1 suffice); entries beyond history_cap
+ * are neither written nor counted in history_len. */
 int darcy_solve(darcy_ctx* ctx, const double* a, const double* rhs, double f, int64_t n,
                 const darcy_opts* opts, const double* u0, double* u_out, darcy_stats* stats,
                 double* history, int32_t history_cap);
@@ -139,6 +147,14 @@ int darcy_apply_preconditioner(darcy_ctx* ctx, const double* a, int64_t n, const
                                int32_t pre_sweeps, int32_t post_sweeps, int32_t cycle_index,
                                int32_t precision, int32_t device_pointers);
 
+/* darcy_apply_preconditioner with every cycle option in `opts` (sweeps,
+ * cycle_index, precision, device_pointers, min_coarse_n, max_levels; the
+ * preconditioner field is ignored).  Replaces Multigrid(Level, MultigridOptions)
+ * + Multigrid::apply (reference multigrid.hpp:55-71, :88-93) for hierarchies
+ * other than the default. */
+int darcy_apply_preconditioner_opts(darcy_ctx* ctx, const double* a, int64_t n, const double* r, double* z,
+                                    const darcy_opts* opts);
+
 /* Multigrid depth for side n (reference multigrid.hpp:55-71). */
 int32_t darcy_num_levels(int64_t n);
 
@@ -159,6 +175,20 @@ int darcy_bandwidth_probe(darcy_ctx* ctx, size_t bytes, int32_t repeats, double*
 
 /* "gfx950, 256 CUs, HIP 7.2" style description of the context's device. */
 int darcy_device_info(const darcy_ctx* ctx, char* buf, int32_t cap);
+
+/* Gaussian random fields and two-phase media for `count` samples on an
+ * n x n grid.  Replaces gaussian_random_field + two_phase_permeability
+ * (reference python/darcy/fields.py:31-83) for batches: xi holds the
+ * count * n * n standard normals the reference draws (in its order, from
+ * numpy.random.default_rng), and each sample is the orthonormal 2-D DCT-III
+ * of xi scaled by the spectral amplitude
+ * tau^(alpha-1) (pi^2 (k1^2 + k2^2) + tau^2)^(-alpha/2) (mean mode 0).
+ * grf_out (may be NULL) receives the fields, a_out (may be NULL) the media
+ * hi where the field is >= 0 and lo elsewhere.  Host pointers unless
+ * device_pointers != 0.  The DCT is a direct FP64 sum, so values agree with
+ * SciPy's FFT-based idctn to round-off (about 1e-16 of the field's scale). */
+int darcy_grf_fields(darcy_ctx* ctx, int64_t count, int64_t n, double alpha, double tau, const double* xi,
+                     double hi, double lo, double* grf_out, double* a_out, int32_t device_pointers);
 
 /* ------------------------------------------------------------------------
  * Learned warm start.  No counterpart in the reference, whose solve always

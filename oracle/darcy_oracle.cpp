@@ -256,11 +256,13 @@ void vzero(std::vector<T>& x) {
 }
 
 // --- hierarchy and cycle (multigrid.hpp) --------------------------------
-std::vector<Op<double>> build_hierarchy(Op<double> fine) {
+// multigrid.hpp:55-63: coarsen while fewer than max_levels levels exist and
+// the last one is wider than min_coarse_n and than 1.
+std::vector<Op<double>> build_hierarchy(Op<double> fine, int64_t min_coarse_n = 1, int max_levels = kMaxLevels) {
     std::vector<Op<double>> lv;
     lv.push_back(std::move(fine));
-    while (static_cast<int>(lv.size()) < kMaxLevels) {
-        if (lv.back().n <= 1) break;  // min_coarse_n = 1
+    while (static_cast<int>(lv.size()) < max_levels) {
+        if (lv.back().n <= min_coarse_n || lv.back().n <= 1) break;
         lv.push_back(galerkin(lv.back()));
     }
     return lv;
@@ -423,7 +425,7 @@ int solve_one(const double* a, const double* rhs, idx n, const oracle_opts& o, c
     Cycle<float> c32;
     std::vector<Op<double>> hier;
     if (mg) {
-        hier = build_hierarchy(std::move(fine));
+        hier = build_hierarchy(std::move(fine), o.min_coarse_n, o.max_levels);
         out.levels = static_cast<int>(hier.size());
         if (fp32) {
             for (const auto& L : hier) c32.ops.push_back(convert<float>(L));
@@ -573,16 +575,18 @@ extern "C" int oracle_apply_operator(const double* a, int64_t n, const double* x
     return 0;
 }
 
-extern "C" int oracle_apply_preconditioner(const double* a, int64_t n, const double* r, double* z,
-                                           int32_t pre_sweeps, int32_t post_sweeps, int32_t cycle_index,
-                                           int32_t cycle_fp32, int64_t* bad_index) {
+namespace {
+
+int apply_cycle(const double* a, int64_t n, const double* r, double* z, int32_t pre_sweeps, int32_t post_sweeps,
+                int32_t cycle_index, int32_t cycle_fp32, int64_t min_coarse_n, int32_t max_levels,
+                int64_t* bad_index) {
     const idx b_at = first_bad(a, n * n);
     if (b_at >= 0) {
         if (bad_index) *bad_index = b_at;
         return 1;
     }
     if (pre_sweeps < 1 || post_sweeps < 1 || cycle_index < 1 || cycle_index > 2) return 1;
-    auto hier = build_hierarchy(fine_operator(a, static_cast<idx>(n)));
+    auto hier = build_hierarchy(fine_operator(a, static_cast<idx>(n)), min_coarse_n, max_levels);
     if (cycle_fp32) {
         Cycle<float> c;
         for (const auto& L : hier) c.ops.push_back(convert<float>(L));
@@ -596,6 +600,20 @@ extern "C" int oracle_apply_preconditioner(const double* a, int64_t n, const dou
         c.apply(r, z);
     }
     return 0;
+}
+
+}  // namespace
+
+extern "C" int oracle_apply_preconditioner(const double* a, int64_t n, const double* r, double* z,
+                                           int32_t pre_sweeps, int32_t post_sweeps, int32_t cycle_index,
+                                           int32_t cycle_fp32, int64_t* bad_index) {
+    return apply_cycle(a, n, r, z, pre_sweeps, post_sweeps, cycle_index, cycle_fp32, 1, kMaxLevels, bad_index);
+}
+
+extern "C" int oracle_apply_preconditioner_opts(const double* a, int64_t n, const double* r, double* z,
+                                                const oracle_opts* o, int64_t* bad_index) {
+    return apply_cycle(a, n, r, z, o->pre_sweeps, o->post_sweeps, o->cycle_index, o->cycle_fp32, o->min_coarse_n,
+                       o->max_levels, bad_index);
 }
 
 extern "C" int32_t oracle_num_levels(int64_t n) {

@@ -33,6 +33,8 @@ typedef struct oracle_opts {
     int32_t record_history;
     int32_t cycle_fp32;  /* 1: cycle in FP32 inside FP64 flexible CG (no reference counterpart) */
     int32_t threads;     /* OpenMP threads for batch loops; 0 = all */
+    int32_t min_coarse_n; /* multigrid.hpp:43-44 (reference default 1) */
+    int32_t max_levels;   /* multigrid.hpp:45-46 (reference default 32) */
 } oracle_opts;
 
 typedef struct oracle_stats {
@@ -64,6 +66,11 @@ int oracle_apply_operator(const double* a, int64_t n, const double* x, double* y
 int oracle_apply_preconditioner(const double* a, int64_t n, const double* r, double* z,
                                 int32_t pre_sweeps, int32_t post_sweeps, int32_t cycle_index,
                                 int32_t cycle_fp32, int64_t* bad_index);
+
+/* The same with the cycle options of `opts` (pre_sweeps, post_sweeps,
+ * cycle_index, cycle_fp32, min_coarse_n, max_levels). */
+int oracle_apply_preconditioner_opts(const double* a, int64_t n, const double* r, double* z,
+                                     const oracle_opts* opts, int64_t* bad_index);
 
 /* Number of levels for a fine size n (multigrid.hpp:55-71, level.hpp:164). */
 int32_t oracle_num_levels(int64_t n);

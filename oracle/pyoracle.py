@@ -34,6 +34,8 @@ class Opts(C.Structure):
         ("record_history", C.c_int32),
         ("cycle_fp32", C.c_int32),
         ("threads", C.c_int32),
+        ("min_coarse_n", C.c_int32),
+        ("max_levels", C.c_int32),
     ]
 
 
@@ -61,9 +63,10 @@ def _dp(x: np.ndarray):
     return x.ctypes.data_as(C.POINTER(C.c_double))
 
 
-def _opts(tol, max_iter, preconditioner, pre_sweeps, post_sweeps, cycle_index, record_history, fp32, threads):
+def _opts(tol, max_iter, preconditioner, pre_sweeps, post_sweeps, cycle_index, record_history, fp32, threads,
+          min_coarse_n=1, max_levels=32):
     return Opts(tol, max_iter, PC[preconditioner], pre_sweeps, post_sweeps, cycle_index,
-                int(record_history), int(fp32), threads)
+                int(record_history), int(fp32), threads, min_coarse_n, max_levels)
 
 
 def _rhs(f, n):
@@ -92,6 +95,7 @@ class Oracle:
                                          C.POINTER(i64)]
         L.oracle_apply_operator.argtypes = [dp, i64, dp, dp, C.POINTER(i64)]
         L.oracle_apply_preconditioner.argtypes = [dp, i64, dp, dp, i32, i32, i32, i32, C.POINTER(i64)]
+        L.oracle_apply_preconditioner_opts.argtypes = [dp, i64, dp, dp, C.POINTER(Opts), C.POINTER(i64)]
         L.oracle_num_levels.argtypes = [i64]
         L.oracle_num_levels.restype = i32
         L.oracle_hierarchy.argtypes = [dp, i64, dp]
@@ -100,7 +104,8 @@ class Oracle:
         L.oracle_set_num_threads.argtypes = [i32]
 
     def solve(self, a, f=1.0, *, tol=1e-10, max_iter=10000, preconditioner="multigrid", pre_sweeps=1,
-              post_sweeps=1, cycle_index=2, record_history=False, fp32=False, u0=None) -> Result:
+              post_sweeps=1, cycle_index=2, record_history=False, fp32=False, u0=None, min_coarse_n=1,
+              max_levels=32) -> Result:
         a = np.ascontiguousarray(a, dtype=np.float64)
         n = a.shape[0]
         rhs = _rhs(f, n)
@@ -113,7 +118,8 @@ class Oracle:
         if u0 is not None:
             u0 = np.ascontiguousarray(u0, dtype=np.float64)
             u0p = _dp(u0)
-        o = _opts(tol, max_iter, preconditioner, pre_sweeps, post_sweeps, cycle_index, record_history, fp32, 0)
+        o = _opts(tol, max_iter, preconditioner, pre_sweeps, post_sweeps, cycle_index, record_history, fp32, 0,
+                  min_coarse_n, max_levels)
         rc = self.lib.oracle_solve(_dp(a), _dp(rhs), n, C.byref(o), u0p, _dp(u), C.byref(st),
                                    _dp(hist) if record_history else None, cap, C.byref(bad))
         if rc:
@@ -145,13 +151,15 @@ class Oracle:
             raise ValueError(f"invalid permeability at {bad.value}")
         return y
 
-    def apply_preconditioner(self, a, r, *, pre_sweeps=1, post_sweeps=1, cycle_index=2, fp32=False):
+    def apply_preconditioner(self, a, r, *, pre_sweeps=1, post_sweeps=1, cycle_index=2, fp32=False,
+                             min_coarse_n=1, max_levels=32):
         a = np.ascontiguousarray(a, dtype=np.float64)
         r = np.ascontiguousarray(r, dtype=np.float64)
         z = np.zeros_like(r)
         bad = C.c_int64(-1)
-        if self.lib.oracle_apply_preconditioner(_dp(a), a.shape[0], _dp(r), _dp(z), pre_sweeps, post_sweeps,
-                                                cycle_index, int(fp32), C.byref(bad)):
+        o = _opts(1e-10, 1, "multigrid", pre_sweeps, post_sweeps, cycle_index, False, fp32, 0, min_coarse_n,
+                  max_levels)
+        if self.lib.oracle_apply_preconditioner_opts(_dp(a), a.shape[0], _dp(r), _dp(z), C.byref(o), C.byref(bad)):
             raise ValueError("invalid input")
         return z
 
@@ -198,6 +206,7 @@ class Reference:
         L.ref_solve_batch.argtypes = [dp, dp, i64, i64, C.POINTER(Opts), dp, C.POINTER(Stats)]
         L.ref_apply_operator.argtypes = [dp, i64, dp, dp]
         L.ref_apply_preconditioner.argtypes = [dp, i64, dp, dp, i32, i32, i32]
+        L.ref_apply_preconditioner_opts.argtypes = [dp, i64, dp, dp, C.POINTER(Opts)]
         L.ref_hierarchy_diag.argtypes = [dp, i64, dp]
         L.ref_hierarchy_diag.restype = i64
         L.ref_num_levels.argtypes = [i64]
@@ -214,7 +223,7 @@ class Reference:
         self.lib.ref_set_num_threads(threads)
 
     def solve(self, a, f=1.0, *, tol=1e-10, max_iter=10000, preconditioner="multigrid", pre_sweeps=1,
-              post_sweeps=1, cycle_index=2, record_history=False) -> Result:
+              post_sweeps=1, cycle_index=2, record_history=False, min_coarse_n=1, max_levels=32) -> Result:
         a = np.ascontiguousarray(a, dtype=np.float64)
         n = a.shape[0]
         rhs = _rhs(f, n)
@@ -222,7 +231,8 @@ class Reference:
         st = Stats()
         cap = max_iter + 2 if record_history else 0
         hist = np.zeros(max(cap, 1))
-        o = _opts(tol, max_iter, preconditioner, pre_sweeps, post_sweeps, cycle_index, record_history, False, 0)
+        o = _opts(tol, max_iter, preconditioner, pre_sweeps, post_sweeps, cycle_index, record_history, False, 0,
+                  min_coarse_n, max_levels)
         rc = self.lib.ref_solve(_dp(a), _dp(rhs), n, C.byref(o), _dp(u), C.byref(st),
                                 _dp(hist) if record_history else None, cap)
         if rc:
@@ -252,12 +262,14 @@ class Reference:
             raise ValueError("invalid input")
         return y
 
-    def apply_preconditioner(self, a, r, *, pre_sweeps=1, post_sweeps=1, cycle_index=2):
+    def apply_preconditioner(self, a, r, *, pre_sweeps=1, post_sweeps=1, cycle_index=2, min_coarse_n=1,
+                             max_levels=32):
         a = np.ascontiguousarray(a, dtype=np.float64)
         r = np.ascontiguousarray(r, dtype=np.float64)
         z = np.zeros_like(r)
-        if self.lib.ref_apply_preconditioner(_dp(a), a.shape[0], _dp(r), _dp(z), pre_sweeps, post_sweeps,
-                                             cycle_index):
+        o = _opts(1e-10, 1, "multigrid", pre_sweeps, post_sweeps, cycle_index, False, False, 0, min_coarse_n,
+                  max_levels)
+        if self.lib.ref_apply_preconditioner_opts(_dp(a), a.shape[0], _dp(r), _dp(z), C.byref(o)):
             raise ValueError("invalid input")
         return z
 

@@ -27,6 +27,8 @@ darcy::SolveOptions to_options(const oracle_opts& o) {
     s.multigrid.pre_sweeps = o.pre_sweeps;
     s.multigrid.post_sweeps = o.post_sweeps;
     s.multigrid.cycle_index = o.cycle_index;
+    s.multigrid.min_coarse_n = o.min_coarse_n;
+    s.multigrid.max_levels = o.max_levels;
     s.record_history = o.record_history != 0;
     return s;
 }
@@ -112,6 +114,18 @@ extern "C" int ref_apply_preconditioner(const double* a, int64_t n, const double
         mo.post_sweeps = post_sweeps;
         mo.cycle_index = cycle_index;
         const darcy::Multigrid mg(darcy::Level(a, static_cast<darcy::index_t>(n)), mo);
+        mg.apply(r, z);
+        return 0;
+    } catch (...) {
+        return 1;
+    }
+}
+
+extern "C" int ref_apply_preconditioner_opts(const double* a, int64_t n, const double* r, double* z,
+                                             const oracle_opts* opts) {
+    try {
+        darcy::validate_permeability(a, n * n);
+        const darcy::Multigrid mg(darcy::Level(a, static_cast<darcy::index_t>(n)), to_options(*opts).multigrid);
         mg.apply(r, z);
         return 0;
     } catch (...) {

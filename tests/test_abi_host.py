@@ -48,6 +48,7 @@ def test_default_options_are_the_reference_defaults():
     assert (o.tol, o.max_iter, o.preconditioner, o.pre_sweeps, o.post_sweeps, o.cycle_index) == (
         1e-10, 10000, 2, 1, 1, 2)
     assert (o.record_history, o.precision, o.device_pointers) == (0, 0, 0)
+    assert (o.min_coarse_n, o.max_levels, o.breakdown_at) == (1, 32, 0)  # multigrid.hpp:43-46; no injection
 
 
 def test_num_levels_closed_form():
@@ -162,3 +163,13 @@ class TestFields:
             gaussian_random_field(0)
         with pytest.raises(ValueError, match="positive"):
             two_phase_permeability(np.zeros((8, 8)), hi=1.0, lo=-1.0)
+
+
+def test_one_draw_equals_the_per_sample_draws():
+    """`darcy dataset` draws all normals of a chunk at once; numpy's
+    generator yields the same sequence as the reference's per-sample
+    (n, n) draws (python/darcy/cli.py:79-84 of the reference)."""
+    a = np.random.default_rng(7).standard_normal((5, 24, 24))
+    rng = np.random.default_rng(7)
+    b = np.stack([rng.standard_normal((24, 24)) for _ in range(5)])
+    assert np.array_equal(a, b)

@@ -114,6 +114,85 @@ def test_operator_cycle_and_hierarchy_bit_exact_with_reference(n, oracle, rng):
     assert oracle.num_levels(n) == ref.num_levels(n)
 
 
+@needs_ref
+@pytest.mark.parametrize("min_coarse_n,max_levels", [(4, 32), (16, 32), (1, 3), (1, 1), (6, 2), (100, 32), (0, 0)])
+@pytest.mark.parametrize("n", [13, 37, 64])
+def test_hierarchy_options_bit_exact_with_reference(n, min_coarse_n, max_levels, oracle, rng):
+    """MultigridOptions.min_coarse_n / max_levels (multigrid.hpp:43-46,
+    :55-63) and the 8 + 8-sweep coarsest solve (:129-146)."""
+    ref = pyoracle.Reference()
+    a = two_phase(n, seed=n)
+    x = rng.standard_normal((n, n))
+    kw = dict(min_coarse_n=min_coarse_n, max_levels=max_levels)
+    for cyc in (1, 2):
+        assert np.array_equal(oracle.apply_preconditioner(a, x, cycle_index=cyc, **kw),
+                              ref.apply_preconditioner(a, x, cycle_index=cyc, **kw))
+    threads, ref_threads = oracle.max_threads(), ref.max_threads()
+    oracle.set_num_threads(1)
+    ref.set_num_threads(1)
+    try:
+        got = oracle.solve(a, max_iter=3000, **kw)
+        want = ref.solve(a, max_iter=3000, **kw)
+    finally:
+        oracle.set_num_threads(threads)
+        ref.set_num_threads(ref_threads)
+    assert (got.iterations, got.levels, got.converged) == (want.iterations, want.levels, want.converged)
+    assert np.array_equal(got.u, want.u)
+
+
+@pytest.mark.parametrize("n,min_coarse_n,max_levels,levels", [
+    (64, 1, 32, 7), (64, 4, 32, 5), (64, 5, 32, 5), (64, 8, 32, 4), (64, 64, 32, 1), (64, 1, 3, 3),
+    (37, 10, 32, 3), (1, 1, 32, 1), (64, 1, 1, 1), (64, 1, 0, 1)])
+def test_hierarchy_options_level_counts(n, min_coarse_n, max_levels, levels, oracle):
+    """Closed form of the coarsening loop: sides 64 -> 32 -> 16 -> 8 -> 4 ->
+    2 -> 1 (37 -> 19 -> 10 -> 5), stopping at the first side <= min_coarse_n
+    or at max_levels levels."""
+    r = oracle.solve(two_phase(n, seed=1), max_iter=5000, min_coarse_n=min_coarse_n, max_levels=max_levels)
+    assert r.levels == levels and r.converged
+
+
+@pytest.mark.parametrize("n", [2, 3, 4, 5])
+def test_coarsest_sweeps_as_block_gauss_seidel(n, oracle, rng):
+    """With one level the cycle is the coarsest solve alone
+    (multigrid.hpp:129-146): from x = 0, 8 sweeps red then black, then 8
+    black then red.  Restated on the dense matrix A (columns from unit
+    vectors): a colour sweep is x_C = D_C^-1 (b_C - A_{C,other} x_other),
+    since cells of one colour do not touch each other."""
+    a = two_phase(8, seed=3)[:n, :n].copy()
+    r = rng.standard_normal((n, n))
+    z = oracle.apply_preconditioner(a, r, max_levels=1)
+    A = np.stack([oracle.apply_operator(a, e.reshape(n, n)).ravel() for e in np.eye(n * n)], axis=1)
+    i, j = np.divmod(np.arange(n * n), n)
+    red = (i + j) % 2 == 0
+    b, x = r.ravel(), np.zeros(n * n)
+
+    def sweep(c):
+        m = red if c == 0 else ~red
+        x[m] = (b[m] - A[np.ix_(m, ~m)] @ x[~m]) / np.diag(A)[m]
+
+    for _ in range(8):
+        sweep(0)
+        sweep(1)
+    for _ in range(8):
+        sweep(1)
+        sweep(0)
+    np.testing.assert_allclose(z.ravel(), x, rtol=1e-12, atol=1e-14 * np.abs(x).max())
+    # and it approaches the exact solve A^-1 r
+    exact = np.linalg.solve(A, b)
+    assert np.linalg.norm(x - exact) <= 1e-2 * np.linalg.norm(exact)
+
+
+@pytest.mark.parametrize("min_coarse_n,max_levels", [(4, 32), (1, 2), (16, 32)])
+def test_hierarchy_options_preconditioner_symmetric(min_coarse_n, max_levels, oracle, rng):
+    """The cycle stays a symmetric operator with a swept coarsest level."""
+    n = 20
+    a = two_phase(n, seed=5)
+    x, y = rng.standard_normal((2, n, n))
+    mx = oracle.apply_preconditioner(a, x, min_coarse_n=min_coarse_n, max_levels=max_levels)
+    my = oracle.apply_preconditioner(a, y, min_coarse_n=min_coarse_n, max_levels=max_levels)
+    assert abs(np.vdot(y, mx) - np.vdot(x, my)) <= 1e-10 * abs(np.vdot(x, mx))
+
+
 # -------------------------------------------------- published known answers
 
 @pytest.mark.parametrize("n,iters,levels", [(32, 18, 6), (64, 21, 7), (128, 24, 8), (256, 26, 9), (512, 28, 10),
