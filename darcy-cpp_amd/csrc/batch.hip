@@ -10,6 +10,8 @@
 // Two additions have no reference counterpart: a warm start u0, and the FP32
 // cycle inside an FP64 flexible CG (beta = -alpha <z_new, A p> / <z, r>,
 // Notay's FCG(1), since r_new - r_old = -alpha A p).
+#include <mutex>
+
 #include "block.hpp"
 
 namespace darcy_hip {
@@ -262,6 +264,20 @@ __global__ __launch_bounds__(kBlock, 4) void k_batch(BatchArgs A) {
 template __global__ void k_batch<double>(BatchArgs);
 template __global__ void k_batch<float>(BatchArgs);
 
+// Scratch of the allocation-free entry point (darcy_solve_batch_noalloc):
+// its problem tables, work-queue counter and solve slots.  Each device the
+// library runs on holds one copy until the process ends; calls on one device
+// share it.
+constexpr size_t kScratchBytes = size_t{64} << 20;
+static __device__ __align__(256) char scratch[kScratchBytes];
+
+// Zeroes `words` 32-bit words (the work-queue counter): a kernel rather than
+// a memset, so that the sequence stays valid inside a captured graph.
+__global__ void k_zero_words(uint32_t* p, int words) {
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < words) p[i] = 0u;
+}
+
 // Streaming copy of `n` doubles: the reference kernel for achievable HBM
 // bandwidth and for scaling the byte counters of the profiler.
 __global__ __launch_bounds__(256) void k_stream_copy(const double2* __restrict__ src, double2* __restrict__ dst,
@@ -289,6 +305,34 @@ extern "C" hipError_t darcy_hip_batch_occupancy(int mixed, int* blocks_per_cu) {
                                                             darcy_hip::kBlock, 0);
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, darcy_hip::k_batch<double>,
                                                         darcy_hip::kBlock, 0);
+}
+
+// Device address of `scratch` on the current device.  The array's name is a
+// host-side stand-in, so every host use of its memory goes through the
+// address looked up here, once per device.
+extern "C" hipError_t darcy_hip_scratch(void** ptr, size_t* bytes) {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static void* addr[kMaxDevices] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(darcy_hip::scratch));
+        if (e != hipSuccess) return e;
+        addr[dev] = p;
+    }
+    *ptr = addr[dev];
+    *bytes = darcy_hip::kScratchBytes;
+    return hipSuccess;
+}
+
+extern "C" hipError_t darcy_hip_zero_words(uint32_t* p, int words, hipStream_t stream) {
+    hipLaunchKernelGGL(darcy_hip::k_zero_words, dim3((words + 255) / 256), dim3(256), 0, stream, p, words);
+    return hipGetLastError();
 }
 
 extern "C" hipError_t darcy_hip_launch_copy(const void* src, void* dst, size_t bytes, int grid, hipStream_t stream) {

@@ -35,6 +35,8 @@ extern "C" hipError_t darcy_grid_cg(int which, const GridCg* g, const double* po
                                     hipStream_t s);
 extern "C" hipError_t darcy_hip_batch_occupancy(int mixed, int* blocks_per_cu);
 extern "C" hipError_t darcy_hip_launch_copy(const void* src, void* dst, size_t bytes, int grid, hipStream_t stream);
+extern "C" hipError_t darcy_hip_scratch(void** ptr, size_t* bytes);
+extern "C" hipError_t darcy_hip_zero_words(uint32_t* p, int words, hipStream_t stream);
 extern "C" hipError_t darcy_hip_launch_grf(int n, int count, double alpha, double tau, const double* xi,
                                            double* cmat, double* amp, double* t, double* g, double* a, double hi,
                                            double lo, hipStream_t s);
@@ -583,6 +585,10 @@ extern "C" int darcy_ctx_create(int device, darcy_ctx** out) {
     }
     c->cus = prop.multiProcessorCount;
     c->arch = prop.gcnArchName;
+    // Once per context, with its device current: the scratch array's device
+    // address (an error is kept and reported by the entry point that needs it).
+    c->scratch_status = darcy_hip_scratch(&c->scratch, &c->scratch_bytes);
+    if (c->scratch_status != hipSuccess) c->scratch = nullptr;
     *out = c;
     return DARCY_OK;
 }
@@ -857,5 +863,101 @@ extern "C" int darcy_grf_fields(darcy_ctx* ctx, int64_t count, int64_t n, double
     HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "event timing");
     ctx->last_ms = ms;
     ctx->error.clear();
+    return DARCY_OK;
+}
+
+// darcy_solve_batch on the library's static scratch array: no device
+// allocation, no free.  The caller's arrays are device pointers.  The tables
+// (problems, stats, order, counter) and then as many solve slots as fit take
+// the scratch in that order; a batch whose first slot does not fit is
+// rejected.
+extern "C" int darcy_solve_batch_noalloc(darcy_ctx* ctx, int64_t count, const int64_t* n, const double* const* a,
+                                         const double* const* rhs, const double* f, const darcy_opts* opts,
+                                         const double* const* u0, double* const* u_out, darcy_stats* stats) {
+    if (!ctx) return DARCY_INVALID_INPUT;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (count < 1 || !n || !a || !u_out) return fail(ctx, DARCY_INVALID_INPUT, "a must have shape (batch, n, n)");
+    if (count > INT32_MAX) return fail(ctx, DARCY_INVALID_INPUT, "batch too large");
+    darcy_opts o;
+    darcy_default_opts(&o);
+    if (opts) o = *opts;
+    if (!o.device_pointers)
+        return fail(ctx, DARCY_INVALID_INPUT, "the allocation-free entry point takes device pointers only");
+    o.record_history = 0;
+    if (const int rc = check_opts(ctx, o)) return rc;
+    int nmax = 0;
+    for (int64_t i = 0; i < count; ++i) {
+        if (const int rc = check_problem(ctx, n[i], a[i], u_out[i])) return rc;
+        nmax = std::max<int>(nmax, static_cast<int>(n[i]));
+    }
+    if (!ctx->scratch) return hip_fail(ctx, ctx->scratch_status, "scratch address lookup");
+    const bool mixed = o.preconditioner == DARCY_PC_MULTIGRID && o.precision == DARCY_MIXED;
+    Hier sizing;
+    const size_t slot_bytes = plan_slot(nullptr, nmax, mixed, &sizing, o.min_coarse_n, o.max_levels);
+    const size_t c = static_cast<size_t>(count);
+    const size_t o_probs = 0, o_stats = align_up(sizeof(Problem) * c), o_order = o_stats + align_up(sizeof(ProblemStats) * c),
+                 o_counter = o_order + align_up(sizeof(int32_t) * c), o_slots = o_counter + kAlign;
+    if (o_slots + slot_bytes > ctx->scratch_bytes) {
+        char msg[200];
+        std::snprintf(msg, sizeof msg,
+                      "batch exceeds the allocation-free scratch (%zu bytes): %zu bytes of tables and %zu bytes per "
+                      "solve slot",
+                      ctx->scratch_bytes, o_slots, slot_bytes);
+        return fail(ctx, DARCY_INVALID_INPUT, msg);
+    }
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    char* base = static_cast<char*>(ctx->scratch);
+    std::vector<Problem> probs(c);
+    for (size_t i = 0; i < c; ++i) {
+        Problem& p = probs[i];
+        p.n = static_cast<int32_t>(n[i]);
+        p.pad = 0;
+        p.a = a[i];
+        p.rhs = rhs ? rhs[i] : nullptr;
+        p.f = f ? f[i] : 1.0;
+        p.in = u0 ? u0[i] : nullptr;
+        p.out = u_out[i];
+        p.history = nullptr;
+    }
+    std::vector<int32_t> order(c);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return n[x] > n[y]; });
+    int per_cu = 1;
+    HIP_TRY(darcy_hip_batch_occupancy(mixed, &per_cu), "occupancy query");
+    const int64_t fit = static_cast<int64_t>((ctx->scratch_bytes - o_slots) / slot_bytes);
+    const int64_t grid = std::max<int64_t>(
+        1, std::min<int64_t>({count, static_cast<int64_t>(ctx->cus) * std::max(per_cu, 1), fit}));
+
+    hipStream_t s = ctx->stream;
+    HIP_TRY(hipMemcpyAsync(base + o_probs, probs.data(), sizeof(Problem) * c, hipMemcpyHostToDevice, s),
+            "problem upload");
+    HIP_TRY(hipMemcpyAsync(base + o_order, order.data(), sizeof(int32_t) * c, hipMemcpyHostToDevice, s),
+            "order upload");
+    HIP_TRY(darcy_hip_zero_words(reinterpret_cast<uint32_t*>(base + o_counter), 1, s), "counter reset");
+    BatchArgs args{};
+    args.probs = reinterpret_cast<const Problem*>(base + o_probs);
+    args.stats = reinterpret_cast<ProblemStats*>(base + o_stats);
+    args.order = reinterpret_cast<const int32_t*>(base + o_order);
+    args.next = reinterpret_cast<uint32_t*>(base + o_counter);
+    args.slots = base + o_slots;
+    args.slot_bytes = slot_bytes;
+    args.count = static_cast<int32_t>(count);
+    args.prm = solve_params(ctx, o, kModeSolve, mixed, 0);
+    HIP_TRY(hipEventRecord(ctx->ev0, s), "event record");
+    HIP_TRY(darcy_hip_launch_batch(&args, static_cast<int>(grid), mixed, s), "solve kernel launch");
+    HIP_TRY(hipEventRecord(ctx->ev1, s), "event record");
+    std::vector<ProblemStats> st(c);
+    HIP_TRY(hipMemcpyAsync(st.data(), base + o_stats, sizeof(ProblemStats) * c, hipMemcpyDeviceToHost, s),
+            "stats readback");
+    HIP_TRY(hipStreamSynchronize(s), "solve");
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "event timing");
+    ctx->last_ms = ms;
+    return report(ctx, st, stats, 0);
+}
+
+extern "C" int darcy_scratch_bytes(const darcy_ctx* ctx, size_t* bytes) {
+    if (!ctx || !bytes) return DARCY_INVALID_INPUT;
+    *bytes = ctx->scratch ? ctx->scratch_bytes : 0;
     return DARCY_OK;
 }

@@ -58,6 +58,11 @@ struct darcy_ctx {
     // Cells per row-pipeline step of the persistent batch kernel's global
     // levels (tuned value).
     int batch_step_cells = 1024;
+    // Device address and size of the library's static scratch array (the
+    // allocation-free entry point), looked up when the context is created.
+    void* scratch = nullptr;
+    size_t scratch_bytes = 0;
+    hipError_t scratch_status = hipSuccess;
     // The last captured cycle (+ the rz kernel) of the grid path and what it
     // was captured for: the workspace address, n, precision, options and
     // tail level.  Reused while all of them match.

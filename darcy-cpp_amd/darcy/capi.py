@@ -72,6 +72,8 @@ def _load() -> C.CDLL:
     lib.darcy_bandwidth_probe.argtypes = [vp, C.c_size_t, i32, dp]
     lib.darcy_solve.argtypes = [vp, vp, vp, C.c_double, i64, C.POINTER(Opts), vp, vp, C.POINTER(Stats), dp, i32]
     lib.darcy_apply_preconditioner_opts.argtypes = [vp, vp, i64, vp, vp, C.POINTER(Opts)]
+    lib.darcy_solve_batch_noalloc.argtypes = lib.darcy_solve_batch.argtypes
+    lib.darcy_scratch_bytes.argtypes = [vp, C.POINTER(C.c_size_t)]
     lib.darcy_grf_fields.argtypes = [vp, i64, i64, C.c_double, C.c_double, vp, C.c_double, C.c_double, vp, vp, i32]
     lib.darcy_ctx_set_tuning.argtypes = [vp, C.c_char_p, i32]
     lib.darcy_ctx_get_tuning.argtypes = [vp, C.c_char_p, C.POINTER(i32)]
@@ -256,6 +258,11 @@ class Context:
         return {"iterations": st.iterations, "converged": bool(st.converged), "residual": st.residual,
                 "levels": st.levels}
 
+    def scratch_bytes(self) -> int:
+        b = C.c_size_t()
+        self._check(self.lib.darcy_scratch_bytes(self.ptr, C.byref(b)))
+        return b.value
+
     def solve_batch_device(
         self,
         sizes: Sequence[int],
@@ -271,8 +278,11 @@ class Context:
         post_sweeps: int = 1,
         cycle_index: int = 2,
         precision: str = "fp64",
+        no_alloc: bool = False,
     ) -> dict[str, np.ndarray]:
-        """Solve problems whose a / u (and u0) live in device memory."""
+        """Solve problems whose a / u (and u0) live in device memory;
+        no_alloc uses darcy_solve_batch_noalloc (the library's static
+        scratch, no device allocation)."""
         count = len(sizes)
         o = Opts()
         self.lib.darcy_default_opts(C.byref(o))
@@ -285,7 +295,8 @@ class Context:
         u0 = (C.c_void_p * count)(*u0_ptrs) if u0_ptrs is not None else None
         fs = (C.c_double * count)(*([f] * count))
         st = (Stats * count)()
-        self._check(self.lib.darcy_solve_batch(self.ptr, count, n, a, None, fs, C.byref(o), u0, u, st))
+        entry = self.lib.darcy_solve_batch_noalloc if no_alloc else self.lib.darcy_solve_batch
+        self._check(entry(self.ptr, count, n, a, None, fs, C.byref(o), u0, u, st))
         return {
             "iterations": np.array([s.iterations for s in st], dtype=np.int64),
             "converged": np.array([bool(s.converged) for s in st]),

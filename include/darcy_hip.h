@@ -135,6 +135,25 @@ int darcy_solve_batch(darcy_ctx* ctx, int64_t count, const int64_t* n, const dou
                       const double* const* rhs, const double* f, const darcy_opts* opts,
                       const double* const* u0, double* const* u_out, darcy_stats* stats);
 
+/* darcy_solve_batch without any device allocation or free.  No counterpart
+ * in the reference; for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Every array is a device pointer (opts->device_pointers must
+ * be 1); history is not recorded.  The solve tables and slots live in one
+ * static device array of the library, 64 MiB per device
+ * (darcy_scratch_bytes): a batch whose tables plus one solve slot of its
+ * largest problem exceed it is rejected with DARCY_INVALID_INPUT (one
+ * 1024^2 FP64 problem fits; the slots that fit run concurrently).  Calls on
+ * one device share that array: calls that overlap in time on one device --
+ * from two contexts, or two threads with their own contexts -- race. */
+int darcy_solve_batch_noalloc(darcy_ctx* ctx, int64_t count, const int64_t* n, const double* const* a,
+                              const double* const* rhs, const double* f, const darcy_opts* opts,
+                              const double* const* u0, double* const* u_out, darcy_stats* stats);
+
+/* Bytes of the static scratch array of the context's device (0 if its
+ * address could not be looked up). */
+int darcy_scratch_bytes(const darcy_ctx* ctx, size_t* bytes);
+
 /* y = A x for the finite-volume operator of a.  Replaces _core.apply_operator
  * (reference src/bindings.cpp:199-216, Level::apply level.hpp:105-119). */
 int darcy_apply_operator(darcy_ctx* ctx, const double* a, int64_t n, const double* x, double* y,
