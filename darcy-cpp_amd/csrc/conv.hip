@@ -387,17 +387,21 @@ __global__ __launch_bounds__(kT) void k_conv(ConvArgs c, Epi epi) {
 
     // This thread stages chunk kc of A rows (tid >> 2) + 64 s and of B rows
     // (tid >> 2) + 64 s; the pixel of each A row is decoded once.
+    // Offsets within one image are 32-bit (an NHWC image is far below 2^31
+    // elements); only the image base is a 64-bit pointer.
     const int kc = tid & 3, row0 = tid >> 2;
-    int iy0[A_PER], ix0[A_PER];
-    long long base[A_PER];
+    int iy0[A_PER], ix0[A_PER], pix[A_PER];
+    const bf16* img[A_PER];
 #pragma unroll
     for (int s = 0; s < A_PER; ++s) {
         const int m = m_wg + row0 + 64 * s;
         const int ox = m % c.ow, q = m / c.ow, oy = q % c.oh, b = q / c.oh;
         iy0[s] = m < c.M ? oy * c.stride - 1 : -1000000;
         ix0[s] = ox * c.stride - 1;
-        base[s] = ((static_cast<long long>(b) * c.in.h + iy0[s]) * c.in.w + ix0[s]) * cin;
+        img[s] = c.x + static_cast<size_t>(m < c.M ? b : 0) * c.in.h * c.in.w * cin;
+        pix[s] = m < c.M ? (iy0[s] * c.in.w + ix0[s]) * cin : 0;
     }
+    const bf16* wrow = c.w + static_cast<size_t>(n_wg + row0) * K;
     struct Regs {
         v8bf a[A_PER], b[B_PER];
     };
@@ -405,20 +409,21 @@ __global__ __launch_bounds__(kT) void k_conv(ConvArgs c, Epi epi) {
         const int k = k0 + 8 * kc;
         const int tap = k >> c.lg_cin, ci = k & (cin - 1);
         const int ky = (tap * 11) >> 5, kx = tap - 3 * ky;
+        const int toff = (ky * c.in.w + kx) * cin + ci;
 #pragma unroll
         for (int s = 0; s < A_PER; ++s) {
             const int iy = iy0[s] + ky, ix = ix0[s] + kx;
             v8bf v = {};
             if (tap < 9 && static_cast<unsigned>(iy) < static_cast<unsigned>(c.in.h) &&
                 static_cast<unsigned>(ix) < static_cast<unsigned>(c.in.w))
-                v = *reinterpret_cast<const v8bf*>(c.x + base[s] + (static_cast<long long>(ky) * c.in.w + kx) * cin + ci);
+                v = *reinterpret_cast<const v8bf*>(img[s] + (pix[s] + toff));
             rg.a[s] = v;
         }
 #pragma unroll
         for (int s = 0; s < B_PER; ++s) {
             const int col = row0 + 64 * s, n = n_wg + col;
             v8bf v = {};
-            if (col < BN && n < c.cout && k < K) v = *reinterpret_cast<const v8bf*>(c.w + static_cast<size_t>(n) * K + k);
+            if (col < BN && n < c.cout && k < K) v = *reinterpret_cast<const v8bf*>(wrow + (64 * s * K + k));
             rg.b[s] = v;
         }
     };
