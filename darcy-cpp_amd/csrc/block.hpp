@@ -494,6 +494,10 @@ struct Cycle {
     // kStepCellsShared where many workgroups share each L2, kStepCellsAlone
     // for a workgroup that runs alone.
     int step_cells;
+    // FP64 cycle only: the tail as an FP64 matrix (then a 64-entry residual),
+    // or null for the recursive register-resident cycle (the reference's
+    // operations, bit for bit).
+    double* tail64 = nullptr;
 
     DARCY_INLINE LvP<T> G(int l) const { return global_level<T>(h, l); }
     DARCY_INLINE LvP<T> S(int l) const {
@@ -528,14 +532,16 @@ struct Cycle {
         reg_cycle_levels(lv, nl, fresh0, o);
     }
 
-    // --- FP32 cycle: the tail as its matrix.  Fresh: x = M b; repeated
-    // W-cycle visit: x += M (b - A x).  Lane i owns row i; the row sum runs
-    // over k in order.
-    DARCY_INLINE void dense_tail(int l, bool fresh, Exec e) const {
+    // --- The tail as its matrix M (by columns, then a residual vector):
+    // the FP32 cycle's always, the FP64 cycle's when tail64 is set.  Fresh:
+    // x = M b; repeated W-cycle visit: x += M (b - A x).  Lane i owns row i;
+    // the row sum runs over k in order.
+    template <class MT>
+    DARCY_INLINE void dense_tail(MT* mat, int l, bool fresh, Exec e) const {
         const LvP<T> A = S(l);
         const int n = A.n, dim = n * n, i = e.tid;
         const WaveSync sync;
-        float* res = tail + kTailMat;
+        MT* res = mat + kTailMat;
         const T* rhs = A.b;
         if (!fresh) {
             if (i < dim) {
@@ -546,8 +552,8 @@ struct Cycle {
             rhs = res;
         }
         if (i < dim) {
-            float s = 0.0f;
-            for (int k = 0; k < dim; ++k) s += tail[k * dim + i] * rhs[k];
+            MT s = MT(0);
+            for (int k = 0; k < dim; ++k) s += mat[k * dim + i] * rhs[k];
             A.x[i] = fresh ? s : A.x[i] + s;
         }
         sync();
@@ -596,7 +602,9 @@ struct Cycle {
                 if (threadIdx.x < kWave) {
                     const Exec we{static_cast<int>(threadIdx.x), kWave};
                     if constexpr (sizeof(T) == 4)
-                        dense_tail(l, fresh, we);
+                        dense_tail(tail, l, fresh, we);
+                    else if (tail64)
+                        dense_tail(tail64, l, fresh, we);
                     else
                         wave_cycle(l, fresh, we);
                 }
@@ -621,13 +629,15 @@ struct Cycle {
     }
 };
 
-// Tail matrix of the FP32 cycle.  Column k is the FP64 recursive cycle from
-// level tl -- the reference's arithmetic, bit-exact with the oracle -- applied
-// to the unit vector e_k, rounded to FP32 and stored by columns.  The four
-// waves take columns in turn; `scratch` (the LDS pool, before it receives the
-// FP32 operators) holds the FP64 operators of the tail levels and each wave's
-// x and b.
-DARCY_INLINE void build_tail_matrix(const Hier& h, int tl, const CycleOpts& o, void* scratch, float* tail) {
+// Tail matrix of the cycle from level tl.  Column k is the FP64 recursive
+// cycle from level tl -- the reference's arithmetic, bit-exact with the
+// oracle -- applied to the unit vector e_k, rounded to OT (FP32 for the FP32
+// cycle, FP64 for the FP64 cycle's dense tail) and stored by columns.  The
+// four waves take columns in turn; `scratch` (the LDS pool, before it
+// receives the operators) holds the FP64 operators of the tail levels and
+// each wave's x and b.
+template <class OT>
+DARCY_INLINE void build_tail_matrix(const Hier& h, int tl, const CycleOpts& o, void* scratch, OT* tail) {
     double* dp = static_cast<double*>(scratch);
     const int L = h.L, lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
     auto op_off = [&](int l) {
@@ -660,7 +670,7 @@ DARCY_INLINE void build_tail_matrix(const Hier& h, int tl, const CycleOpts& o, v
         for (int q = lane; q < dim; q += kWave) top.b[q] = q == k ? 1.0 : 0.0;
         sync();
         cycle_levels<double>(tl, L, true, o, Exec{lane, kWave}, sync, view);
-        for (int q = lane; q < dim; q += kWave) tail[k * dim + q] = static_cast<float>(top.x[q]);
+        for (int q = lane; q < dim; q += kWave) tail[k * dim + q] = static_cast<OT>(top.x[q]);
         sync();
     }
     __syncthreads();

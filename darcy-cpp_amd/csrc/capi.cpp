@@ -26,14 +26,16 @@ extern "C" hipError_t darcy_grid_fine(const double* a, const Hier* hd, unsigned 
 extern "C" hipError_t darcy_grid_galerkin(const Hier* hd, int l, int blocks, hipStream_t s);
 extern "C" hipError_t darcy_grid_fp32(const Hier* hd, int l, int blocks, hipStream_t s);
 extern "C" hipError_t darcy_grid_tail_matrix(const Hier* hd, CycleOptsC o, float* out, hipStream_t s);
+extern "C" hipError_t darcy_grid_tail_matrix64(const Hier* hd, CycleOptsC o, double* out, hipStream_t s);
 extern "C" hipError_t darcy_grid_presmooth(int mixed, const Hier* hd, int l, int kind, int blocks, hipStream_t s);
 extern "C" hipError_t darcy_grid_restrict(int mixed, const Hier* hd, int l, int blocks, hipStream_t s);
 extern "C" hipError_t darcy_grid_postsmooth(int mixed, const Hier* hd, int l, int kind, int blocks, hipStream_t s);
 extern "C" hipError_t darcy_grid_tail(int mixed, const Hier* hd, int l0, int fresh, CycleOptsC o, const float* tail,
-                                      hipStream_t s);
+                                      const double* tail64, hipStream_t s);
 extern "C" hipError_t darcy_grid_cg(int which, const GridCg* g, const double* pold, double* pnew, int blocks,
                                     hipStream_t s);
 extern "C" hipError_t darcy_hip_batch_occupancy(int mixed, int* blocks_per_cu);
+extern "C" hipError_t darcy_grid_check_kernels(int device, const char** what);
 extern "C" hipError_t darcy_hip_launch_copy(const void* src, void* dst, size_t bytes, int grid, hipStream_t stream);
 extern "C" hipError_t darcy_hip_scratch(void** ptr, size_t* bytes);
 extern "C" hipError_t darcy_hip_zero_words(uint32_t* p, int words, hipStream_t stream);
@@ -346,7 +348,7 @@ int run_grid(darcy_ctx* ctx, const Job& job) {
         return at;
     };
     const size_t o_slot = take(slot_bytes), o_h = take(sizeof(Hier)), o_st = take(sizeof(GridState)),
-                 o_bad = take(8), o_tail = take(64 * 64 * 4), o_hist = take(hist ? hist_n * 8 : 8);
+                 o_bad = take(8), o_tail = take(64 * 64 * 8), o_hist = take(hist ? hist_n * 8 : 8);
     const bool host = !job.device_ptrs;
     const size_t o_a = host ? take(cells) : 0, o_rhs = host && job.rhs && job.rhs[0] ? take(cells) : 0,
                  o_u0 = host && job.in && job.in[0] ? take(cells) : 0, o_u = host ? take(cells) : 0;
@@ -357,6 +359,9 @@ int run_grid(darcy_ctx* ctx, const Job& job) {
     GridState* st = reinterpret_cast<GridState*>(base + o_st);
     auto* bad = reinterpret_cast<unsigned long long*>(base + o_bad);
     float* tail = reinterpret_cast<float*>(base + o_tail);
+    // The FP64 cycle's tail as a dense FP64 matrix (grid_fp64_dense_tail),
+    // in the same workspace.
+    double* tail64 = reinterpret_cast<double*>(base + o_tail);
     double* history = hist ? (host ? reinterpret_cast<double*>(base + o_hist) : job.history) : nullptr;
 
     const double* a = job.a[0];
@@ -396,6 +401,8 @@ int run_grid(darcy_ctx* ctx, const Job& job) {
     while (lT < h.L - 1 && h.n[lT] > T) ++lT;
     const CycleOptsC co{o.pre_sweeps, o.post_sweeps, o.cycle_index};
     if (mixed && h.L > 1) HIP_TRY(darcy_grid_tail_matrix(hd, co, tail, s), "tail matrix");
+    const bool dense64 = mg && !mixed && h.L > 1 && ctx->grid_fp64_dense_tail;
+    if (dense64) HIP_TRY(darcy_grid_tail_matrix64(hd, co, tail64, s), "tail matrix");
 
     GridCg g{hd,    st,         a,          rhs,          job.f ? job.f[0] : 1.0,           u0, u, history,
              o.tol, o.max_iter, mixed ? 1 : 0, static_cast<int>(hist_n), o.breakdown_at};
@@ -416,7 +423,7 @@ int run_grid(darcy_ctx* ctx, const Job& job) {
     std::function<void(int, bool)> cycle = [&](int l, bool fresh) {
         if (rc) return;
         if (l >= lT) {
-            const hipError_t e = darcy_grid_tail(mixed, hd, l, fresh ? 1 : 0, co, tail, s);
+            const hipError_t e = darcy_grid_tail(mixed, hd, l, fresh ? 1 : 0, co, tail, dense64 ? tail64 : nullptr, s);
             if (e != hipSuccess) rc = hip_fail(ctx, e, "tail cycle");
             return;
         }
@@ -469,9 +476,15 @@ int run_grid(darcy_ctx* ctx, const Job& job) {
             HIP_TRY(darcy_grid_cg(1, &g, nullptr, nullptr, cg_blocks, s), "rz");
             return DARCY_OK;
         }
-        const int key[7] = {n, mixed ? 1 : 0, o.pre_sweeps, o.post_sweeps, o.cycle_index, lT, cg_blocks};
-        const bool hit = ctx->cycle_exec && ctx->cycle_base == base && std::equal(key, key + 7, ctx->cycle_key);
+        const int key[8] = {n, mixed ? 1 : 0, o.pre_sweeps, o.post_sweeps, o.cycle_index, lT, cg_blocks, dense64 ? 1 : 0};
+        const bool hit = ctx->cycle_exec && ctx->cycle_base == base && std::equal(key, key + 8, ctx->cycle_key);
         if (!hit) {
+            if (!ctx->graph_checked) {  // every captured kernel within the device's launch limits
+                const char* what = "";
+                const hipError_t ce = darcy_grid_check_kernels(ctx->device, &what);
+                if (ce != hipSuccess) return hip_fail(ctx, ce, (std::string("graph kernel limits: ") + what).c_str());
+                ctx->graph_checked = true;
+            }
             if (ctx->cycle_exec) (void)hipGraphExecDestroy(ctx->cycle_exec);
             if (ctx->cycle_graph) (void)hipGraphDestroy(ctx->cycle_graph);
             ctx->cycle_exec = nullptr;
@@ -487,7 +500,7 @@ int run_grid(darcy_ctx* ctx, const Job& job) {
             ctx->cycle_graph = graph;
             HIP_TRY(hipGraphInstantiate(&ctx->cycle_exec, graph, nullptr, nullptr, 0), "graph instantiate");
             ctx->cycle_base = base;
-            std::copy(key, key + 7, ctx->cycle_key);
+            std::copy(key, key + 8, ctx->cycle_key);
         }
         HIP_TRY(hipGraphLaunch(ctx->cycle_exec, s), "graph launch");
         return DARCY_OK;
@@ -779,6 +792,8 @@ extern "C" int darcy_ctx_set_tuning(darcy_ctx* ctx, const char* key, int32_t val
         ctx->grid_cg_per_cu = value;
     } else if (k == "batch_step_cells" && value >= 128 && value <= 16384) {
         ctx->batch_step_cells = value;
+    } else if (k == "grid_fp64_dense_tail" && (value == 0 || value == 1)) {
+        ctx->grid_fp64_dense_tail = value;
     } else {
         return fail(ctx, DARCY_INVALID_INPUT, "unknown tuning key or value out of range: " + k);
     }
@@ -798,6 +813,8 @@ extern "C" int darcy_ctx_get_tuning(const darcy_ctx* ctx, const char* key, int32
         *value = ctx->grid_cg_per_cu;
     else if (k == "batch_step_cells")
         *value = ctx->batch_step_cells;
+    else if (k == "grid_fp64_dense_tail")
+        *value = ctx->grid_fp64_dense_tail;
     else
         return DARCY_INVALID_INPUT;
     return DARCY_OK;

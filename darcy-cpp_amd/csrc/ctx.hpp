@@ -58,6 +58,11 @@ struct darcy_ctx {
     // Cells per row-pipeline step of the persistent batch kernel's global
     // levels (tuned value).
     int batch_step_cells = 1024;
+    // Grid path, FP64 cycle: 1 (default) applies the levels of side <= 8 as
+    // one dense FP64 matrix per visit (the recursive cycle's linear map,
+    // formed in FP64 from unit vectors); 0 runs the recursion itself, which
+    // reproduces the reference's cycle bit for bit.
+    int grid_fp64_dense_tail = 1;
     // Device address and size of the library's static scratch array (the
     // allocation-free entry point), looked up when the context is created.
     void* scratch = nullptr;
@@ -69,5 +74,6 @@ struct darcy_ctx {
     hipGraphExec_t cycle_exec = nullptr;
     hipGraph_t cycle_graph = nullptr;
     const void* cycle_base = nullptr;
-    int cycle_key[7] = {0, 0, 0, 0, 0, 0, 0};
+    int cycle_key[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    bool graph_checked = false;  // launch limits of the captured kernels verified
 };

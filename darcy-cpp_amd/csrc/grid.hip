@@ -40,8 +40,10 @@ __global__ __launch_bounds__(kBlock) void kg_fp32(const Hier* hd, int l) {
     to_fp32(hd->n[l], hd->ex[l], hd->ny[l], hd->ex32[l], hd->ny32[l], hd->id32[l], e);
 }
 
-// One workgroup: the FP32 tail matrix into global memory.
-__global__ __launch_bounds__(kBlock) void kg_tail_matrix(const Hier* hd, CycleOpts o, float* tail_out) {
+// One workgroup: the tail matrix into global memory, in FP32 (the FP32
+// cycle's) or FP64 (the FP64 cycle's dense tail).
+template <class OT>
+__global__ __launch_bounds__(kBlock) void kg_tail_matrix(const Hier* hd, CycleOpts o, OT* tail_out) {
     __shared__ Shared s;
     __shared__ __align__(16) float pool[pool_elems<float>()];
     if (threadIdx.x == 0) {
@@ -137,11 +139,12 @@ constexpr int kTailThreads = 512;
 
 template <class T>
 __global__ __launch_bounds__(kTailThreads) void kg_tail(const Hier* hd, int l0, int fresh, CycleOpts o,
-                                                        const float* tail_in) {
+                                                        const float* tail_in, const double* tail64_in) {
     constexpr int NL = TailLdsSide<T>::value;
     __shared__ Shared s;
     __shared__ __align__(16) T pool[LdsPool<T, NL>::kElems];
     __shared__ __align__(16) float tail[sizeof(T) == 4 ? kTailLds : 1];
+    __shared__ __align__(16) double tail64[sizeof(T) == 8 ? kTailLds : 1];
     const Exec e{static_cast<int>(threadIdx.x), kTailThreads};
     // The hierarchy (about 2.2 KB) is copied by the whole workgroup, 8 bytes
     // per thread, rather than by one thread: this prologue runs on every one
@@ -171,9 +174,14 @@ __global__ __launch_bounds__(kTailThreads) void kg_tail(const Hier* hd, int l0, 
             const int dim = h.n[s.t.wave] * h.n[s.t.wave];
             stage_in(tail, tail_in, dim * dim, e);
         }
+    } else {
+        if (tail64_in && s.t.wave < h.L) {
+            const int dim = h.n[s.t.wave] * h.n[s.t.wave];
+            stage_in(tail64, tail64_in, dim * dim, e);
+        }
     }
     __syncthreads();
-    const Cycle<T> cyc{h, s.t, pool, tail, o, kStepCellsAlone};
+    const Cycle<T> cyc{h, s.t, pool, tail, o, kStepCellsAlone, sizeof(T) == 8 && tail64_in ? tail64 : nullptr};
     cyc.run(l0, fresh != 0, e);
     if (s.t.first <= l0) {  // the cycle ends with a barrier
         const LvP<T> g = global_level<T>(h, l0);
@@ -401,8 +409,8 @@ template __global__ void kg_restrict<double>(const Hier*, int);
 template __global__ void kg_restrict<float>(const Hier*, int);
 template __global__ void kg_postsmooth<double>(const Hier*, int, int);
 template __global__ void kg_postsmooth<float>(const Hier*, int, int);
-template __global__ void kg_tail<double>(const Hier*, int, int, CycleOpts, const float*);
-template __global__ void kg_tail<float>(const Hier*, int, int, CycleOpts, const float*);
+template __global__ void kg_tail<double>(const Hier*, int, int, CycleOpts, const float*, const double*);
+template __global__ void kg_tail<float>(const Hier*, int, int, CycleOpts, const float*, const double*);
 
 }  // namespace darcy_hip
 
@@ -425,7 +433,11 @@ extern "C" hipError_t darcy_grid_fp32(const Hier* hd, int l, int blocks, hipStre
     return hipGetLastError();
 }
 extern "C" hipError_t darcy_grid_tail_matrix(const Hier* hd, CycleOpts o, float* out, hipStream_t s) {
-    hipLaunchKernelGGL(kg_tail_matrix, dim3(1), dim3(kBlock), 0, s, hd, o, out);
+    hipLaunchKernelGGL(kg_tail_matrix<float>, dim3(1), dim3(kBlock), 0, s, hd, o, out);
+    return hipGetLastError();
+}
+extern "C" hipError_t darcy_grid_tail_matrix64(const Hier* hd, CycleOpts o, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(kg_tail_matrix<double>, dim3(1), dim3(kBlock), 0, s, hd, o, out);
     return hipGetLastError();
 }
 extern "C" hipError_t darcy_grid_presmooth(int mixed, const Hier* hd, int l, int kind, int blocks, hipStream_t s) {
@@ -451,11 +463,11 @@ extern "C" hipError_t darcy_grid_postsmooth(int mixed, const Hier* hd, int l, in
     return hipGetLastError();
 }
 extern "C" hipError_t darcy_grid_tail(int mixed, const Hier* hd, int l0, int fresh, CycleOpts o, const float* tail,
-                                      hipStream_t s) {
+                                      const double* tail64, hipStream_t s) {
     if (mixed)
-        hipLaunchKernelGGL(kg_tail<float>, dim3(1), dim3(kTailThreads), 0, s, hd, l0, fresh, o, tail);
+        hipLaunchKernelGGL(kg_tail<float>, dim3(1), dim3(kTailThreads), 0, s, hd, l0, fresh, o, tail, nullptr);
     else
-        hipLaunchKernelGGL(kg_tail<double>, dim3(1), dim3(kTailThreads), 0, s, hd, l0, fresh, o, tail);
+        hipLaunchKernelGGL(kg_tail<double>, dim3(1), dim3(kTailThreads), 0, s, hd, l0, fresh, o, nullptr, tail64);
     return hipGetLastError();
 }
 extern "C" hipError_t darcy_grid_cg(int which, const GridCg* g, const double* pold, double* pnew, int blocks,
@@ -484,3 +496,43 @@ extern "C" hipError_t darcy_grid_cg(int which, const GridCg* g, const double* po
     return hipGetLastError();
 }
 extern "C" size_t darcy_grid_state_bytes() { return sizeof(GridState); }
+
+// Launch limits of every kernel the grid path captures into its cycle graph,
+// checked on the host before the first capture: block size within the
+// kernel's and the device's thread limit, static LDS within the device's
+// per-workgroup limit (no kernel here takes dynamic LDS).  Returns the first
+// violation as an error code and names the kernel in *what.
+extern "C" hipError_t darcy_grid_check_kernels(int device, const char** what) {
+    int max_lds = 0, max_threads = 0;
+    hipError_t e = hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&max_threads, hipDeviceAttributeMaxThreadsPerBlock, device);
+    if (e != hipSuccess) return e;
+    struct K {
+        const void* fn;
+        int block;
+        const char* name;
+    };
+    const K ks[] = {
+        {reinterpret_cast<const void*>(&kg_presmooth<double>), kBlock, "kg_presmooth<double>"},
+        {reinterpret_cast<const void*>(&kg_presmooth<float>), kBlock, "kg_presmooth<float>"},
+        {reinterpret_cast<const void*>(&kg_restrict<double>), kBlock, "kg_restrict<double>"},
+        {reinterpret_cast<const void*>(&kg_restrict<float>), kBlock, "kg_restrict<float>"},
+        {reinterpret_cast<const void*>(&kg_postsmooth<double>), kBlock, "kg_postsmooth<double>"},
+        {reinterpret_cast<const void*>(&kg_postsmooth<float>), kBlock, "kg_postsmooth<float>"},
+        {reinterpret_cast<const void*>(&kg_tail<double>), kTailThreads, "kg_tail<double>"},
+        {reinterpret_cast<const void*>(&kg_tail<float>), kTailThreads, "kg_tail<float>"},
+        {reinterpret_cast<const void*>(&kg_rz), kBlock, "kg_rz"},
+        {reinterpret_cast<const void*>(&kg_final<1>), kFinalThreads, "kg_final<1>"},
+    };
+    for (const K& k : ks) {
+        hipFuncAttributes at{};
+        e = hipFuncGetAttributes(&at, k.fn);
+        if (e != hipSuccess) return e;
+        if (k.block > at.maxThreadsPerBlock || k.block > max_threads ||
+            static_cast<int>(at.sharedSizeBytes) > max_lds) {
+            if (what) *what = k.name;
+            return hipErrorInvalidConfiguration;
+        }
+    }
+    return hipSuccess;
+}

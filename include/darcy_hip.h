@@ -107,7 +107,14 @@ void darcy_ctx_destroy(darcy_ctx* ctx);
  *   "batch_step_cells" cells per row-pipeline step on the levels the
  *                     persistent batch kernel keeps in global memory
  *                     (128..16384, default 1024)
- * None changes any result: only the launch structure. */
+ * None of these changes any result: only the launch structure.  One more
+ * key changes the arithmetic of the grid path's FP64 cycle, not the method:
+ *   "grid_fp64_dense_tail" 1 (default): the levels of side <= 8 are applied
+ *                     as one dense FP64 matrix per visit, formed from the
+ *                     FP64 recursive cycle on unit vectors (the same linear
+ *                     map, rounded differently: solutions agree to about
+ *                     1e-14); 0: the recursion itself, bit for bit the
+ *                     reference's cycle. */
 int darcy_ctx_set_tuning(darcy_ctx* ctx, const char* key, int32_t value);
 int darcy_ctx_get_tuning(const darcy_ctx* ctx, const char* key, int32_t* value);
 

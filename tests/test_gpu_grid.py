@@ -42,8 +42,15 @@ def true_residual(a, u, f=1.0):
 @pytest.mark.parametrize("n", [65, 100, 256, 300])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_grid_path_equals_batch_path(n, precision):
+    """With the recursive FP64 cycle (grid_fp64_dense_tail = 0) the grid
+    path gives the persistent kernel's bits; the default dense FP64 tail is
+    checked against it in test_gpu_tuning.py."""
     a = two_phase(n, seed=n)
-    single = darcy.solve(a, precision=precision)  # n > 64: grid path
+    darcy._core.set_tuning("grid_fp64_dense_tail", 0)
+    try:
+        single = darcy.solve(a, precision=precision)  # n > 64: grid path
+    finally:
+        darcy._core.set_tuning("grid_fp64_dense_tail", 1)
     us, batch = darcy.solve_batch_uneven([a], precision=precision)  # persistent kernel
     assert single.iterations == batch.iterations[0]
     assert np.array_equal(single.u, us[0])
@@ -131,7 +138,11 @@ def test_grid_path_with_more_sweeps(precision, cycle_index, oracle):
     and in FP64 the oracle's iterations and u within 1e-9."""
     a = two_phase(150, seed=7)
     kw = dict(pre_sweeps=2, post_sweeps=3, cycle_index=cycle_index, precision=precision)
-    single = darcy.solve(a, **kw)
+    darcy._core.set_tuning("grid_fp64_dense_tail", 0)  # the recursive FP64 cycle: the persistent kernel's bits
+    try:
+        single = darcy.solve(a, **kw)
+    finally:
+        darcy._core.set_tuning("grid_fp64_dense_tail", 1)
     us, batch = darcy.solve_batch_uneven([a], **kw)
     assert single.converged and single.iterations == batch.iterations[0]
     assert np.array_equal(single.u, us[0])

@@ -92,6 +92,7 @@ def test_tail_side_equals_persistent_kernel(precision):
     a = two_phase(300, seed=11)
     with capi.Context(apply_tuning=False) as ctx:
         ctx.set_tuning("grid_min_n", 64)
+        ctx.set_tuning("grid_fp64_dense_tail", 0)  # the recursive cycle: the persistent kernel's bits
         u_ref, it_ref = _solve_persistent(ctx, a, precision)
         for side in (32, 64, 128):
             for graph in (0, 1):
@@ -136,3 +137,28 @@ def test_batch_step_cells_do_not_change_results(precision):
             for pa, pu in bufs:
                 ctx.free(pa)
                 ctx.free(pu)
+
+
+@pytest.mark.parametrize("n", [130, 300, 1024])
+def test_fp64_dense_tail_matches_the_recursive_cycle(n, oracle):
+    """The grid path's FP64 dense tail applies the same linear map as the
+    recursive cycle over the levels of side <= 8, rounded differently: the
+    same iteration count and solutions within 1e-12 of each other (and of
+    the oracle's to 1e-9), with and without graph replay."""
+    from conftest import rel_l2
+
+    a = two_phase(n, seed=21)
+    with capi.Context(apply_tuning=False) as ctx:
+        ctx.set_tuning("grid_min_n", 64)
+        ctx.set_tuning("grid_fp64_dense_tail", 0)
+        u_rec, it_rec = _solve(ctx, a, "fp64")
+        for graph in (0, 1):
+            print(f"dense tail n={n} graph {graph}", flush=True)
+            ctx.set_tuning("grid_fp64_dense_tail", 1)
+            ctx.set_tuning("grid_graph", graph)
+            u, it = _solve(ctx, a, "fp64")
+            assert it == it_rec
+            assert rel_l2(u, u_rec) <= 1e-12
+        assert ctx.get_tuning("grid_fp64_dense_tail") == 1
+    if n <= 300:
+        assert rel_l2(u, oracle.solve(a).u) <= 1e-9
