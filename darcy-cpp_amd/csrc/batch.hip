@@ -62,7 +62,7 @@ DARCY_INLINE void run_problem(Shared& s, T* pool, float* tail, const Problem& pb
         }
         // Coarse operators into LDS for the whole solve.
         for (int l = s.t.first; l < h.L; ++l) {
-            const LvP<T> g = global_level<T>(h, l);
+            const GlobalLv<T> g = global_level<T>(h, l);
             const int* f = s.t.off[l];
             const int nf = g.n * (g.n + 1), nc = g.n * g.n;
             for (int q = e.tid; q < nf; q += e.nthr) {
@@ -75,7 +75,7 @@ DARCY_INLINE void run_problem(Shared& s, T* pool, float* tail, const Problem& pb
     }
     const Cycle<T> cyc{h, s.t, pool, tail, CycleOpts{P.pre, P.post, P.gamma},
                        P.step_cells > 0 ? P.step_cells : kStepCellsShared};
-    const LvP<double> A0 = global_level<double>(h, 0);
+    const GlobalLv<double> A0 = global_level<double>(h, 0);
     float* const b0f = mixed ? static_cast<float*>(h.b[0]) : nullptr;
     const float* const x0f = mixed ? static_cast<const float*>(h.x[0]) : nullptr;
     auto zval = [&](int k) -> double { return mixed ? static_cast<double>(x0f[k]) : h.z[k]; };

@@ -363,7 +363,7 @@ DARCY_INLINE void chunk_pass(int N, int w, int nw, double* const* part, Load&& l
 #pragma unroll
                 for (int d = 0; d < ND; ++d) {
                     const double s = wave_sum(acc[d]);
-                    if (lane == 0) part[d][c0 + cc] = s;
+                    if (lane == 0) to_global(part[d])[c0 + cc] = s;
                 }
             }
         }
@@ -378,9 +378,15 @@ DARCY_INLINE void chunk_pass(int N, int w, int nw, double* const* part, Load&& l
 // too, where p_old is read anyway, instead of in the update pass: the same
 // operation on the same values, one read of p fewer per iteration.
 template <int CPR, bool DEFER, class Zt>
-DARCY_INLINE void cg_direction(int n, const LvP<double>& A0, const Zt* z, const double* pold, double beta, bool first,
-                               double* pnew, double* ap, int w, int nw, double* const* part, double* u = nullptr,
-                               double alpha_u = 0.0) {
+DARCY_INLINE void cg_direction(int n, const GlobalLv<double>& A0, const Zt* zf, const double* poldf, double beta,
+                               bool first, double* pnewf, double* apf, int w, int nw, double* const* part,
+                               double* uf = nullptr, double alpha_u = 0.0) {
+    // All of these are device allocations.
+    const auto z = to_global(zf);
+    const auto pold = to_global(poldf);
+    const auto pnew = to_global(pnewf);
+    const auto ap = to_global(apf);
+    const auto u = to_global(uf);
     struct PAp {
         double p, ap, u;
     };
@@ -431,7 +437,7 @@ DARCY_INLINE void cg_direction(int n, const LvP<double>& A0, const Zt* z, const 
     if (first)
         pass([&](int t) { return z[t]; }, [](Zt v) { return static_cast<double>(v); }, Flag<false>{});
     else if constexpr (DEFER) {
-        if (u)
+        if (uf)
             pass(zp, zpv, Flag<true>{});
         else
             pass(zp, zpv, Flag<false>{});
@@ -441,7 +447,8 @@ DARCY_INLINE void cg_direction(int n, const LvP<double>& A0, const Zt* z, const 
 
 // Lane l adds chunk sums l, l+64, l+128, ... in that order; the loads of
 // sixteen terms are issued before their adds.
-__device__ inline double final_sum(const double* part, int C) {
+__device__ inline double final_sum(const double* partf, int C) {
+    const auto part = to_global(partf);
     const int lane = threadIdx.x & (kWave - 1);
     double s = 0.0;
     int c = lane;
@@ -470,15 +477,20 @@ __device__ inline void reduce_parts(Shared& s, int nd, int C) {
 
 // ------------------------------------------------------------ the cycle
 
+// Level l of the hierarchy in global memory, as a global view.
 template <class T>
-DARCY_INLINE LvP<T> global_level(const Hier& h, int l);
+DARCY_INLINE GlobalLv<T> global_level(const Hier& h, int l);
 template <>
-DARCY_INLINE LvP<double> global_level<double>(const Hier& h, int l) {
-    return {h.n[l], h.ex[l], h.ny[l], h.id[l], static_cast<double*>(h.x[l]), static_cast<double*>(h.b[l])};
+DARCY_INLINE GlobalLv<double> global_level<double>(const Hier& h, int l) {
+    return {h.n[l], to_global<const double>(h.ex[l]), to_global<const double>(h.ny[l]),
+            to_global<const double>(h.id[l]), to_global(static_cast<double*>(h.x[l])),
+            to_global(static_cast<double*>(h.b[l]))};
 }
 template <>
-DARCY_INLINE LvP<float> global_level<float>(const Hier& h, int l) {
-    return {h.n[l], h.ex32[l], h.ny32[l], h.id32[l], static_cast<float*>(h.x[l]), static_cast<float*>(h.b[l])};
+DARCY_INLINE GlobalLv<float> global_level<float>(const Hier& h, int l) {
+    return {h.n[l], to_global<const float>(h.ex32[l]), to_global<const float>(h.ny32[l]),
+            to_global<const float>(h.id32[l]), to_global(static_cast<float*>(h.x[l])),
+            to_global(static_cast<float*>(h.b[l]))};
 }
 
 // The cycle of multigrid.hpp:97-127 run by one workgroup, with the coarse
@@ -499,7 +511,7 @@ struct Cycle {
     // operations, bit for bit).
     double* tail64 = nullptr;
 
-    DARCY_INLINE LvP<T> G(int l) const { return global_level<T>(h, l); }
+    DARCY_INLINE GlobalLv<T> G(int l) const { return global_level<T>(h, l); }
     DARCY_INLINE LvP<T> S(int l) const {
         const int* f = t.off[l];
         return {h.n[l], pool + f[0], pool + f[1], pool + f[2], pool + f[3], pool + f[4]};
@@ -581,8 +593,7 @@ struct Cycle {
             o.post == 1 ? up_stream(G(l), static_cast<const T*>(S(l + 1).x), step_cells, e)
                         : visit_up(G(l), static_cast<const T*>(S(l + 1).x), o, e, sync);
         else
-            o.post == 1 ? up_stream(G(l), static_cast<const T*>(G(l + 1).x), step_cells, e)
-                        : visit_up(G(l), static_cast<const T*>(G(l + 1).x), o, e, sync);
+            o.post == 1 ? up_stream(G(l), G(l + 1).x, step_cells, e) : visit_up(G(l), G(l + 1).x, o, e, sync);
     }
 
     // z = M r from level l0 (x_{l0} zero on entry when fresh0).

@@ -33,6 +33,8 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "layout.hpp"
 
 #define DARCY_INLINE __device__ __forceinline__
@@ -44,16 +46,67 @@ constexpr int kWaveTail = 8;  // levels with side <= 8 run on one wave
 
 // ------------------------------------------------------------ level views
 
-// Operator and vectors of one level in the cycle's arithmetic type T.
+// Where a level's arrays live, as the type of its pointers.  FlatPtr: any
+// memory (the LDS levels; host-side checks of the walks).  GlobalPtr: device
+// allocations, in the global address space, so that their accesses are
+// global instructions -- an SGPR base and a 32-bit VGPR offset, waited for
+// on the vector-memory counter alone -- rather than flat ones, which take a
+// 64-bit VGPR address and count against the LDS counter too.  A pointer the
+// kernel reads from LDS (the batch kernel's slot header) or from a buffer is
+// otherwise flat.
 template <class T>
-struct LvP {
-    int n;
-    const T* ex;  // n x (n+1) faces
-    const T* ny;  // (n+1) x n faces
-    const T* id;  // n x n reciprocal diagonal
-    T* x;
-    T* b;
+struct FlatPtr {
+    using type = T*;
 };
+template <class T>
+struct GlobalPtr {
+    using type = __attribute__((address_space(1))) T*;
+};
+
+// The element type of a pointer of either kind, without qualifiers.
+template <class P>
+struct Pointee;
+template <class T>
+struct Pointee<T*> {
+    using type = std::remove_cv_t<T>;
+};
+template <class T>
+struct Pointee<__attribute__((address_space(1))) T*> {
+    using type = std::remove_cv_t<T>;
+};
+template <class P>
+using pointee_t = typename Pointee<P>::type;
+
+// p viewed as a U pointer in the same memory.
+template <class U, class T>
+DARCY_HD __forceinline__ U* retype(T* p) {
+    return reinterpret_cast<U*>(p);
+}
+template <class U, class T>
+DARCY_HD __forceinline__ __attribute__((address_space(1))) U* retype(__attribute__((address_space(1))) T* p) {
+    return reinterpret_cast<__attribute__((address_space(1))) U*>(p);
+}
+
+// p, a pointer into a device allocation (never LDS), as a global pointer.
+template <class T>
+DARCY_INLINE typename GlobalPtr<T>::type to_global(T* p) {
+    return (typename GlobalPtr<T>::type)p;
+}
+
+// Operator and vectors of one level in the cycle's arithmetic type T.
+template <class T, template <class> class Space = FlatPtr>
+struct LvP {
+    using ptr = typename Space<T>::type;
+    using cptr = typename Space<const T>::type;
+    int n;
+    cptr ex;  // n x (n+1) faces
+    cptr ny;  // (n+1) x n faces
+    cptr id;  // n x n reciprocal diagonal
+    ptr x;
+    ptr b;
+};
+template <class T>
+using GlobalLv = LvP<T, GlobalPtr>;
 
 // ------------------------------------------------------------------ sync
 
@@ -102,8 +155,9 @@ DARCY_HD __forceinline__ T apply_combine(T e, T w, T nn, T s, T xk, T xE, T xW, 
     return v;
 }
 
-template <class T, class X>
-DARCY_INLINE T apply_at(int n, const T* ex, const T* ny, int i, int j, X&& x) {
+template <class P, class X>
+DARCY_INLINE pointee_t<P> apply_at(int n, P ex, P ny, int i, int j, X&& x) {
+    using T = pointee_t<P>;
     const int k = i * n + j, f = i * (n + 1) + j;
     const T e = ex[f + 1], w = ex[f], nn = ny[(i + 1) * n + j], s = ny[i * n + j];
     const bool hE = j + 1 < n, hW = j > 0, hN = i + 1 < n, hS = i > 0;
@@ -129,8 +183,8 @@ DARCY_HD __forceinline__ T inv_from_faces(T e, T w, T n, T s) {
     return T(1) / (((e + w) + n) + s);
 }
 
-template <class T>
-DARCY_INLINE T inv_diag_at(const LvP<T>& L, int i, int j) {
+template <class T, template <class> class Sp>
+DARCY_INLINE T inv_diag_at(const LvP<T, Sp>& L, int i, int j) {
     const int n = L.n, f = i * (n + 1) + j;
     return inv_from_faces(L.ex[f + 1], L.ex[f], L.ny[(i + 1) * n + j], L.ny[i * n + j]);
 }
@@ -167,8 +221,8 @@ DARCY_HD __forceinline__ T gs_combine(T e, T w, T nn, T s, T bk, T xE, T xW, T x
 // are issued before its arithmetic -- the scheduling barrier keeps them
 // together: left to itself, the scheduler of a register-bound kernel waits
 // for each neighbour's loads before issuing the next neighbour's.
-template <class T, class Ld, class Cv = Ident>
-DARCY_INLINE T gs_at(const LvP<T>& L, int i, int j, T bk, Ld&& ld, Cv&& cv = Cv{}) {
+template <class T, template <class> class Sp, class Ld, class Cv = Ident>
+DARCY_INLINE T gs_at(const LvP<T, Sp>& L, int i, int j, T bk, Ld&& ld, Cv&& cv = Cv{}) {
     const int n = L.n, f = i * (n + 1) + j;
     const bool hE = j + 1 < n, hW = j > 0, hN = i + 1 < n, hS = i > 0;
     const T e = L.ex[f + 1], w = L.ex[f], nn = L.ny[(i + 1) * n + j], s = L.ny[i * n + j];
@@ -199,10 +253,10 @@ DARCY_INLINE bool colour_cell(int q, int n, int h, int c, int& i, int& j) {
 }
 
 // One colour of red-black GS (level.hpp:145-161).
-template <class T>
-DARCY_INLINE void gs_colour(const LvP<T>& L, int colour, Exec e) {
+template <class T, template <class> class Sp>
+DARCY_INLINE void gs_colour(const LvP<T, Sp>& L, int colour, Exec e) {
     const int n = L.n, h = (n + 1) >> 1, total = n * h;
-    T* x = L.x;
+    const auto x = L.x;
     for (int q0 = e.tid; q0 < total; q0 += 4 * e.nthr) {
         T v[4];
         int kk[4];
@@ -226,12 +280,12 @@ DARCY_INLINE void gs_colour(const LvP<T>& L, int colour, Exec e) {
 // red sweep from zero gives b * inv_d at every red cell (each neighbour term
 // is e * 0); the black update reads those red values, recomputed here in the
 // same operations instead of being reloaded.
-template <class T>
-DARCY_INLINE void presmooth_from_zero(const LvP<T>& L, Exec e) {
+template <class T, template <class> class Sp>
+DARCY_INLINE void presmooth_from_zero(const LvP<T, Sp>& L, Exec e) {
     const int n = L.n, h = (n + 1) >> 1, total = n * h;
-    T* x = L.x;
-    const T* b = L.b;
-    const T* id = L.id;
+    const auto x = L.x;
+    const auto b = L.b;
+    const auto id = L.id;
     struct BI {
         T b, id;
     };
@@ -293,14 +347,28 @@ struct alignas(2 * sizeof(T)) Pair2 {
     T lo, hi;
 };
 
+// The pair at p (aligned), in either memory; member by member, which the
+// compiler merges into one access.
+template <class T, class P>
+DARCY_HD __forceinline__ Pair2<T> load_pair(P p) {
+    const auto q = retype<const Pair2<T>>(p);
+    return Pair2<T>{q->lo, q->hi};
+}
+template <class T, class P>
+DARCY_HD __forceinline__ void store_pair(P p, Pair2<T> v) {
+    const auto q = retype<Pair2<T>>(p);
+    q->lo = v.lo;
+    q->hi = v.hi;
+}
+
 // kPairs: n is even, so every pair starts at an even offset and both of its
 // cells are on the grid; b, ny and x move as pairs.
 // Values at (a, c0) and (a, c1) of an n-wide array (c1 = c0 + 1, or c0 when
 // the pair's second cell is off the grid).
-template <class T, bool kPairs>
-DARCY_HD __forceinline__ Pair2<T> walk_pair(const T* p, int n, int c0, int c1, int a) {
+template <class T, bool kPairs, class P>
+DARCY_HD __forceinline__ Pair2<T> walk_pair(P p, int n, int c0, int c1, int a) {
     if constexpr (kPairs) {
-        return *reinterpret_cast<const Pair2<T>*>(p + a * n + c0);
+        return load_pair<T>(p + a * n + c0);
     } else {
         return Pair2<T>{p[a * n + c0], p[a * n + c1]};
     }
@@ -309,17 +377,17 @@ DARCY_HD __forceinline__ Pair2<T> walk_pair(const T* p, int n, int c0, int c1, i
 // b, the ex faces and the south faces of the pair's cells on row a.  b and
 // ny pairs start at even offsets; the ex row (n + 1 long) holds an aligned
 // pair at f on even rows and at f + 1 on odd ones.
-template <class T, bool kPairs>
-DARCY_HD __forceinline__ WalkRow<T> walk_row(const LvP<T>& L, int c0, int c1, int a) {
+template <class T, bool kPairs, template <class> class Sp = FlatPtr>
+DARCY_HD __forceinline__ WalkRow<T> walk_row(const LvP<T, Sp>& L, int c0, int c1, int a) {
     const int n = L.n, f = a * (n + 1) + c0;
-    const T* ex = L.ex;
+    const auto ex = L.ex;
     const Pair2<T> bb = walk_pair<T, kPairs>(L.b, n, c0, c1, a), ss = walk_pair<T, kPairs>(L.ny, n, c0, c1, a);
     if constexpr (kPairs) {
         if (a & 1) {
-            const Pair2<T> q = *reinterpret_cast<const Pair2<T>*>(ex + f + 1);
+            const Pair2<T> q = load_pair<T>(ex + f + 1);
             return WalkRow<T>{bb.lo, bb.hi, ex[f], q.lo, q.hi, ss.lo, ss.hi};
         }
-        const Pair2<T> q = *reinterpret_cast<const Pair2<T>*>(ex + f);
+        const Pair2<T> q = load_pair<T>(ex + f);
         return WalkRow<T>{bb.lo, bb.hi, q.lo, q.hi, ex[f + 2], ss.lo, ss.hi};
     } else {
         return WalkRow<T>{bb.lo, bb.hi, ex[f], ex[f + 1], ex[f + 1 + (c1 - c0)], ss.lo, ss.hi};
@@ -342,15 +410,15 @@ struct NoShare {
     }
 };
 
-template <class T, bool kPairs, class Share = NoShare>
-DARCY_HD __forceinline__ void presmooth_walk_item(const LvP<T>& L, int k, int i0, int i1, Share sh = Share{}) {
+template <class T, bool kPairs, class Share = NoShare, template <class> class Sp = FlatPtr>
+DARCY_HD __forceinline__ void presmooth_walk_item(const LvP<T, Sp>& L, int k, int i0, int i1, Share sh = Share{}) {
     const int n = L.n, c0 = 2 * k;
     const bool two = c0 + 1 < n;     // the pair's second cell is on the grid
     const int c1 = two ? c0 + 1 : c0;  // its column, clamped for the loads
-    const T* ex = L.ex;
-    const T* ny = L.ny;
-    const T* b = L.b;
-    T* x = L.x;
+    const auto ex = L.ex;
+    const auto ny = L.ny;
+    const auto b = L.b;
+    const auto x = L.x;
     auto row = [&](int a) { return walk_row<T, kPairs>(L, c0, c1, a); };
     // North faces of row a - 1 (the south faces of row a), into (s0, s1).
     auto south = [&](int a, T& s0, T& s1) {
@@ -406,9 +474,9 @@ DARCY_HD __forceinline__ void presmooth_walk_item(const LvP<T>& L, int k, int i0
         const T cross = shared(a) ? (odd ? vw : ve) : cx.b * inv_from_faces(cx.e, cx.w, cx.nn, cx.s);
         const T blk = odd ? gs_combine(cur.m, cur.w0, cn0, cur.s0, cur.b0, rcur, cross, rnext, rprev, two, hX, hN, hS)
                           : gs_combine(cur.e1, cur.m, cn1, cur.s1, cur.b1, cross, rcur, rnext, rprev, hX, true, hN, hS);
-        T* xr = x + a * n + c0;
+        const auto xr = x + a * n + c0;
         if constexpr (kPairs) {
-            *reinterpret_cast<Pair2<T>*>(xr) = odd ? Pair2<T>{blk, rcur} : Pair2<T>{rcur, blk};
+            store_pair<T>(xr, odd ? Pair2<T>{blk, rcur} : Pair2<T>{rcur, blk});
         } else if (odd) {
             xr[0] = blk;
             if (two) xr[1] = rcur;
@@ -437,13 +505,13 @@ DARCY_HD __forceinline__ void presmooth_walk_item(const LvP<T>& L, int k, int i0
 // row's arithmetic.  Each residual is apply_at's expression and each coarse
 // value sums its aggregate from zero in the order (i0,j0),(i0,j1),(i1,j0),
 // (i1,j1), as restricted_residual does: the same bits.
-template <class T, bool kPairs, class Share = NoShare>
-DARCY_HD __forceinline__ void restrict_walk_item(const LvP<T>& L, T* bc, int J, int I0, int I1, Share sh = Share{}) {
+template <class T, bool kPairs, class Share = NoShare, template <class> class Sp = FlatPtr, class BP = T*>
+DARCY_HD __forceinline__ void restrict_walk_item(const LvP<T, Sp>& L, BP bc, int J, int I0, int I1, Share sh = Share{}) {
     const int n = L.n, m = (n + 1) >> 1, c0 = 2 * J;
     const bool two = c0 + 1 < n, west_on = c0 > 0, east_on = c0 + 2 < n;
     const int c1 = two ? c0 + 1 : c0;
-    const T* x = L.x;
-    const T* ny = L.ny;
+    const auto x = L.x;
+    const auto ny = L.ny;
     const int a0 = 2 * I0, a1 = 2 * I1 < n ? 2 * I1 : n;
     auto xpair = [&](int a) { return walk_pair<T, kPairs>(x, n, c0, c1, a); };
     Pair2<T> xp{T(0), T(0)}, xc = xpair(a0), xn{T(0), T(0)};
@@ -502,8 +570,8 @@ struct LaneShare {
 
 // The walk over a level: rows per item chosen so that the items about fill
 // the launch (at most kWalkRows; each item re-forms one red row above it).
-template <class T>
-DARCY_INLINE void presmooth_walk(const LvP<T>& L, Exec e) {
+template <class T, template <class> class Sp>
+DARCY_INLINE void presmooth_walk(const LvP<T, Sp>& L, Exec e) {
     const int n = L.n, h = (n + 1) >> 1;
     long long per = (static_cast<long long>(n) * h) / e.nthr;
     const int rows = per < 1 ? 1 : (per > kWalkRows ? kWalkRows : static_cast<int>(per));
@@ -523,8 +591,8 @@ DARCY_INLINE void presmooth_walk(const LvP<T>& L, Exec e) {
 
 // The restriction walk over a level, coarse rows per item chosen as in
 // presmooth_walk (at most kWalkRows / 2).
-template <class T>
-DARCY_INLINE void restrict_walk(const LvP<T>& L, T* bc, Exec e) {
+template <class T, template <class> class Sp, class BP>
+DARCY_INLINE void restrict_walk(const LvP<T, Sp>& L, BP bc, Exec e) {
     const int m = (L.n + 1) >> 1;
     long long per = (static_cast<long long>(m) * m) / e.nthr;
     const int rows = per < 1 ? 1 : (per > kWalkRows / 2 ? kWalkRows / 2 : static_cast<int>(per));
@@ -546,10 +614,10 @@ template <class T>
 struct XC {
     T x, c;
 };
-template <class T>
+template <class T, class PX, class PC>
 struct Prolonged {
-    const T* x;
-    const T* xc;
+    PX x;
+    PC xc;
     int n, m;
     DARCY_INLINE XC<T> operator()(int a, int c) const { return XC<T>{x[a * n + c], xc[(a >> 1) * m + (c >> 1)]}; }
 };
@@ -565,11 +633,11 @@ struct XPlusC {
 // aggregate, the value the reference stores before smoothing.  Red cells keep
 // their unprolongated values: the red sweep that follows overwrites them
 // without reading them.
-template <class T>
-DARCY_INLINE void black_with_prolong(const LvP<T>& L, const T* xc, Exec e) {
+template <class T, template <class> class Sp, class PC>
+DARCY_INLINE void black_with_prolong(const LvP<T, Sp>& L, PC xc, Exec e) {
     const int n = L.n, h = (n + 1) >> 1, total = n * h, m = (n + 1) >> 1;
-    T* x = L.x;
-    const Prolonged<T> pr{x, xc, n, m};
+    const auto x = L.x;
+    const Prolonged<T, decltype(x), PC> pr{x, xc, n, m};
     for (int q0 = e.tid; q0 < total; q0 += 4 * e.nthr) {
         T v[4];
         int kk[4];
@@ -591,19 +659,19 @@ DARCY_INLINE void black_with_prolong(const LvP<T>& L, const T* xc, Exec e) {
 
 // b_c = P^T (b - A x) over each <= 2x2 aggregate (level.hpp:122-137 then
 // :214-227, fused: the fine residual is never stored).
-template <class T>
-DARCY_INLINE T restricted_residual(const LvP<T>& L, int q, int m);
+template <class T, template <class> class Sp>
+DARCY_INLINE T restricted_residual(const LvP<T, Sp>& L, int q, int m);
 
-template <class T>
-DARCY_INLINE void residual_restrict(const LvP<T>& L, T* bc, Exec e) {
+template <class T, template <class> class Sp, class BP>
+DARCY_INLINE void residual_restrict(const LvP<T, Sp>& L, BP bc, Exec e) {
     const int m = (L.n + 1) >> 1, total = m * m;
     for (int q = e.tid; q < total; q += e.nthr) bc[q] = restricted_residual(L, q, m);
 }
 
 // Coarsest solve (multigrid.hpp:129-144): exact for 1x1, otherwise 8 + 8
 // symmetric sweeps (reached only when the 32-level cap stops coarsening).
-template <class T, class S>
-DARCY_INLINE void coarsest_solve(const LvP<T>& L, bool fresh, Exec e, S sync) {
+template <class T, template <class> class Sp, class S>
+DARCY_INLINE void coarsest_solve(const LvP<T, Sp>& L, bool fresh, Exec e, S sync) {
     if (L.n == 1) {
         if (e.tid == 0) L.x[0] = L.b[0] * L.id[0];
         sync();
@@ -630,8 +698,8 @@ DARCY_INLINE void coarsest_solve(const LvP<T>& L, bool fresh, Exec e, S sync) {
 
 // Going down through level l: pre-smoothing (red, black) then the restricted
 // residual into the next level's b (multigrid.hpp:107-115).
-template <class T, class S>
-DARCY_INLINE void visit_down(const LvP<T>& L, T* bc, bool fresh, const CycleOpts& o, Exec e, S sync) {
+template <class T, template <class> class Sp, class BP, class S>
+DARCY_INLINE void visit_down(const LvP<T, Sp>& L, BP bc, bool fresh, const CycleOpts& o, Exec e, S sync) {
     for (int s = 0; s < o.pre; ++s) {
         if (s == 0 && fresh) {
             presmooth_from_zero(L, e);
@@ -648,8 +716,8 @@ DARCY_INLINE void visit_down(const LvP<T>& L, T* bc, bool fresh, const CycleOpts
 
 // Coming up through level l: prolongation and post-smoothing (black, red)
 // (multigrid.hpp:120-126).
-template <class T, class S>
-DARCY_INLINE void visit_up(const LvP<T>& L, const T* xc, const CycleOpts& o, Exec e, S sync) {
+template <class T, template <class> class Sp, class PC, class S>
+DARCY_INLINE void visit_up(const LvP<T, Sp>& L, PC xc, const CycleOpts& o, Exec e, S sync) {
     black_with_prolong(L, xc, e);
     sync();
     gs_colour(L, 0, e);
@@ -720,8 +788,9 @@ DARCY_INLINE int rows_per_step(int n, int cells) {
 
 // x[k] = value(i, j) for the cells of colour `colour` in rows [ra, rb); four
 // cells per thread and step, loads before stores.
-template <class T, class F>
-DARCY_INLINE void update_rows(T* x, int n, int colour, int ra, int rb, Exec e, F&& value) {
+template <class P, class F>
+DARCY_INLINE void update_rows(P x, int n, int colour, int ra, int rb, Exec e, F&& value) {
+    using T = pointee_t<P>;
     const int h = (n + 1) >> 1, qa = ra * h, qb = rb * h;
     for (int q0 = qa + e.tid; q0 < qb; q0 += 4 * e.nthr) {
         T v[4];
@@ -749,10 +818,10 @@ DARCY_INLINE void update_rows(T* x, int n, int colour, int ra, int rb, Exec e, F
 // aggregate summed in the order (i0,j0),(i0,j1),(i1,j0),(i1,j1)
 // (level.hpp:122-137, :214-227).  The aggregate is unrolled with predicates
 // so that all loads issue before the sums.
-template <class T>
-DARCY_INLINE T restricted_residual(const LvP<T>& L, int q, int m) {
+template <class T, template <class> class Sp>
+DARCY_INLINE T restricted_residual(const LvP<T, Sp>& L, int q, int m) {
     const int n = L.n, I = q / m, J = q - I * m;
-    const T* x = L.x;
+    const auto x = L.x;
     T r[4];
     bool ok[4];
 #pragma unroll
@@ -770,8 +839,8 @@ DARCY_INLINE T restricted_residual(const LvP<T>& L, int q, int m) {
 
 // Restricted residual of the coarse rows [Ia, Ib), two coarse cells per
 // thread and step.
-template <class T>
-DARCY_INLINE void restrict_rows(const LvP<T>& L, T* bc, int Ia, int Ib, Exec e) {
+template <class T, template <class> class Sp, class BP>
+DARCY_INLINE void restrict_rows(const LvP<T, Sp>& L, BP bc, int Ia, int Ib, Exec e) {
     const int n = L.n, m = (n + 1) >> 1, qb = Ib * m;
     for (int q0 = Ia * m + e.tid; q0 < qb; q0 += 2 * e.nthr) {
         T v[2];
@@ -789,11 +858,11 @@ DARCY_INLINE void restrict_rows(const LvP<T>& L, T* bc, int Ia, int Ib, Exec e) 
 
 // One pre-smoothing sweep (red, black) and the restricted residual in a
 // single pass (multigrid.hpp:107-115 with pre_sweeps = 1).
-template <class T>
-DARCY_INLINE void down_stream(const LvP<T>& L, T* bc, bool fresh, int step_cells, Exec e) {
+template <class T, template <class> class Sp, class BP>
+DARCY_INLINE void down_stream(const LvP<T, Sp>& L, BP bc, bool fresh, int step_cells, Exec e) {
     const int n = L.n, m = (n + 1) >> 1, R = rows_per_step(n, step_cells);
-    T* x = L.x;
-    const T* b = L.b;
+    const auto x = L.x;
+    const auto b = L.b;
     auto xv = [&](int a, int c) { return x[a * n + c]; };
     int coarse_done = 0;
     for (int r0 = 0;; r0 += R) {
@@ -825,13 +894,13 @@ DARCY_INLINE void down_stream(const LvP<T>& L, T* bc, bool fresh, int step_cells
 // (multigrid.hpp:120-126 with post_sweeps = 1).  Black cells read their red
 // neighbours as x + x_c (the prolongated values); the red update trails the
 // black one by a row.
-template <class T>
-DARCY_INLINE void up_stream(const LvP<T>& L, const T* xc, int step_cells, Exec e) {
+template <class T, template <class> class Sp, class PC>
+DARCY_INLINE void up_stream(const LvP<T, Sp>& L, PC xc, int step_cells, Exec e) {
     const int n = L.n, m = (n + 1) >> 1, R = rows_per_step(n, step_cells);
-    T* x = L.x;
-    const T* b = L.b;
+    const auto x = L.x;
+    const auto b = L.b;
     auto xv = [&](int a, int c) { return x[a * n + c]; };
-    const Prolonged<T> pr{x, xc, n, m};
+    const Prolonged<T, decltype(x), PC> pr{x, xc, n, m};
     for (int r0 = 0;; r0 += R) {
         const int black_hi = r0 + R < n ? r0 + R : n;
         if (r0 < n)

@@ -58,7 +58,7 @@ __global__ __launch_bounds__(kBlock) void kg_tail_matrix(const Hier* hd, CycleOp
 
 template <class T>
 __global__ __launch_bounds__(kBlock) void kg_presmooth(const Hier* hd, int l, int kind) {
-    const LvP<T> L = global_level<T>(*hd, l);
+    const GlobalLv<T> L = global_level<T>(*hd, l);
     const Exec e = grid_exec();
     if (kind == 0)
         presmooth_walk(L, e);  // red + black from x = 0
@@ -68,16 +68,16 @@ __global__ __launch_bounds__(kBlock) void kg_presmooth(const Hier* hd, int l, in
 
 template <class T>
 __global__ __launch_bounds__(kBlock) void kg_restrict(const Hier* hd, int l) {
-    const LvP<T> L = global_level<T>(*hd, l);
-    restrict_walk(L, static_cast<T*>(hd->b[l + 1]), grid_exec());
+    const GlobalLv<T> L = global_level<T>(*hd, l);
+    restrict_walk(L, to_global(static_cast<T*>(hd->b[l + 1])), grid_exec());
 }
 
 template <class T>
 __global__ __launch_bounds__(kBlock) void kg_postsmooth(const Hier* hd, int l, int kind) {
-    const LvP<T> L = global_level<T>(*hd, l);
+    const GlobalLv<T> L = global_level<T>(*hd, l);
     const Exec e = grid_exec();
     if (kind == 0)
-        black_with_prolong(L, static_cast<const T*>(hd->x[l + 1]), e);
+        black_with_prolong(L, to_global(static_cast<const T*>(hd->x[l + 1])), e);
     else
         gs_colour(L, kind == 1 ? 0 : 1, e);  // 1: red, 2: black
 }
@@ -100,11 +100,11 @@ using gvec_out = __attribute__((address_space(1))) u32x4;
 // and step, four steps' loads issued before their stores.  Loads past the
 // end re-read the last vector (kept in registers: no per-thread arrays) and
 // are not stored.
-template <class T>
-DARCY_INLINE void stage_in(T* lds, const T* src, int count, Exec e) {
+template <class T, class P>
+DARCY_INLINE void stage_in(T* lds, P src, int count, Exec e) {
     constexpr int per = 16 / static_cast<int>(sizeof(T));
     const int nv = count / per, st = e.nthr;
-    gvec* g = (gvec*)src;  // generic -> global address space
+    gvec* g = (gvec*)src;  // global address space
     u32x4* d = reinterpret_cast<u32x4*>(lds);
     for (int v0 = e.tid; v0 < nv; v0 += 4 * st) {
         const int v1 = v0 + st, v2 = v0 + 2 * st, v3 = v0 + 3 * st;
@@ -119,8 +119,8 @@ DARCY_INLINE void stage_in(T* lds, const T* src, int count, Exec e) {
 }
 
 // count elements LDS -> global (both 16-byte aligned).
-template <class T>
-DARCY_INLINE void stage_out(T* dst, const T* lds, int count, Exec e) {
+template <class P, class T>
+DARCY_INLINE void stage_out(P dst, const T* lds, int count, Exec e) {
     constexpr int per = 16 / static_cast<int>(sizeof(T));
     const int nv = count / per;
     gvec_out* g = (gvec_out*)dst;
@@ -158,15 +158,15 @@ __global__ __launch_bounds__(kTailThreads) void kg_tail(const Hier* hd, int l0, 
     __syncthreads();
     const Hier& h = s.h;
     for (int l = s.t.first; l < h.L; ++l) {
-        const LvP<T> g = global_level<T>(h, l);
+        const GlobalLv<T> g = global_level<T>(h, l);
         const int* f = s.t.off[l];
         const int nf = g.n * (g.n + 1), nc = g.n * g.n;
         stage_in(pool + f[0], g.ex, nf, e);
         stage_in(pool + f[1], g.ny, nf, e);
         stage_in(pool + f[2], g.id, nc, e);
         if (l == l0) {
-            stage_in(pool + f[4], static_cast<const T*>(g.b), nc, e);
-            if (!fresh) stage_in(pool + f[3], static_cast<const T*>(g.x), nc, e);
+            stage_in(pool + f[4], g.b, nc, e);
+            if (!fresh) stage_in(pool + f[3], g.x, nc, e);
         }
     }
     if constexpr (sizeof(T) == 4) {
@@ -184,7 +184,7 @@ __global__ __launch_bounds__(kTailThreads) void kg_tail(const Hier* hd, int l0, 
     const Cycle<T> cyc{h, s.t, pool, tail, o, kStepCellsAlone, sizeof(T) == 8 && tail64_in ? tail64 : nullptr};
     cyc.run(l0, fresh != 0, e);
     if (s.t.first <= l0) {  // the cycle ends with a barrier
-        const LvP<T> g = global_level<T>(h, l0);
+        const GlobalLv<T> g = global_level<T>(h, l0);
         stage_out(g.x, pool + s.t.off[l0][3], g.n * g.n, e);
     }
 }
@@ -292,7 +292,7 @@ __global__ __launch_bounds__(kBlock) void kg_init(GridCg g) {
     const Hier& h = *g.hd;
     const int n = h.n[0], N = n * n;
     const double hh = 1.0 / static_cast<double>(n + 1), h2 = hh * hh;
-    const LvP<double> A0 = global_level<double>(h, 0);
+    const GlobalLv<double> A0 = global_level<double>(h, 0);
     float* b0f = g.mixed ? static_cast<float*>(h.b[0]) : nullptr;
     struct Init {
         double bk, rk, u0;
@@ -358,7 +358,7 @@ __global__ __launch_bounds__(kBlock) void kg_rz(GridCg g) {
 // p = z + beta p (first: p = z), ap = A p, chunk sums of <p, ap>.
 __global__ __launch_bounds__(kBlock) void kg_apply(GridCg g, const double* pold, double* pnew) {
     const Hier& h = *g.hd;
-    const LvP<double> A0 = global_level<double>(h, 0);
+    const GlobalLv<double> A0 = global_level<double>(h, 0);
     const bool first = g.st->first != 0;
     const double beta = g.st->beta;
     if (g.mixed)
