@@ -160,7 +160,7 @@ SolveParams solve_params(const darcy_ctx* ctx, const darcy_opts& o, int mode, bo
     p.mode = mode;
     p.record_history = history_cap > 0 ? 1 : 0;
     p.history_cap = history_cap;
-    p.step_cells = ctx->batch_step_cells;
+    p.step_cells = mixed ? ctx->batch_step_cells_fp32 : ctx->batch_step_cells;
     p.min_coarse = o.min_coarse_n;
     p.max_levels = o.max_levels;
     p.breakdown_at = o.breakdown_at;
@@ -792,6 +792,8 @@ extern "C" int darcy_ctx_set_tuning(darcy_ctx* ctx, const char* key, int32_t val
         ctx->grid_cg_per_cu = value;
     } else if (k == "batch_step_cells" && value >= 128 && value <= 16384) {
         ctx->batch_step_cells = value;
+    } else if (k == "batch_step_cells_fp32" && value >= 128 && value <= 16384) {
+        ctx->batch_step_cells_fp32 = value;
     } else if (k == "grid_fp64_dense_tail" && (value == 0 || value == 1)) {
         ctx->grid_fp64_dense_tail = value;
     } else {
@@ -813,6 +815,8 @@ extern "C" int darcy_ctx_get_tuning(const darcy_ctx* ctx, const char* key, int32
         *value = ctx->grid_cg_per_cu;
     else if (k == "batch_step_cells")
         *value = ctx->batch_step_cells;
+    else if (k == "batch_step_cells_fp32")
+        *value = ctx->batch_step_cells_fp32;
     else if (k == "grid_fp64_dense_tail")
         *value = ctx->grid_fp64_dense_tail;
     else

@@ -57,7 +57,8 @@ struct darcy_ctx {
     int grid_cg_per_cu = 4;
     // Cells per row-pipeline step of the persistent batch kernel's global
     // levels (tuned value).
-    int batch_step_cells = 1024;
+    int batch_step_cells = 1024;       // FP64 cycle
+    int batch_step_cells_fp32 = 4096;  // FP32 cycle
     // Grid path, FP64 cycle: 1 (default) applies the levels of side <= 8 as
     // one dense FP64 matrix per visit (the recursive cycle's linear map,
     // formed in FP64 from unit vectors); 0 runs the recursion itself, which

@@ -568,17 +568,23 @@ struct LaneShare {
     }
 };
 
-// The walk over a level: rows per item chosen so that the items about fill
-// the launch (at most kWalkRows; each item re-forms one red row above it).
+// Rows per walk item for `rows` x `cols` items' worth of work over `nthr`
+// threads: about one item per thread, at most `most` rows.
+DARCY_HD __forceinline__ int walk_strip(int rows, int cols, int nthr, int most) {
+    const long long per = (static_cast<long long>(rows) * cols) / nthr;
+    return per < 1 ? 1 : (per > most ? most : static_cast<int>(per));
+}
+
+// Pre-smoothing from zero of rows [ra, rb) as walks of `strip` rows.  Every
+// item re-forms the red row above it and the red row below it from b and the
+// faces, so the items of a range depend on nothing outside it.
 template <class T, template <class> class Sp>
-DARCY_INLINE void presmooth_walk(const LvP<T, Sp>& L, Exec e) {
+DARCY_INLINE void presmooth_walk_rows(const LvP<T, Sp>& L, int ra, int rb, int strip, Exec e) {
     const int n = L.n, h = (n + 1) >> 1;
-    long long per = (static_cast<long long>(n) * h) / e.nthr;
-    const int rows = per < 1 ? 1 : (per > kWalkRows ? kWalkRows : static_cast<int>(per));
-    const int strips = (n + rows - 1) / rows, total = strips * h;
+    const int strips = (rb - ra + strip - 1) / strip, total = strips * h;
     const int lane = threadIdx.x & (kWave - 1);
     for (int w = e.tid; w < total; w += e.nthr) {
-        const int st = w / h, k = w - st * h, i0 = st * rows, i1 = i0 + rows < n ? i0 + rows : n;
+        const int st = w / h, k = w - st * h, i0 = ra + st * strip, i1 = i0 + strip < rb ? i0 + strip : rb;
         // Consecutive lanes hold consecutive items, so lane l + 1 walks pair
         // k + 1 of the same strip when k + 1 < h (and l - 1 pair k - 1 when k > 0).
         const LaneShare sh{lane + 1 < kWave && k + 1 < h, lane > 0 && k > 0};
@@ -589,23 +595,37 @@ DARCY_INLINE void presmooth_walk(const LvP<T, Sp>& L, Exec e) {
     }
 }
 
-// The restriction walk over a level, coarse rows per item chosen as in
-// presmooth_walk (at most kWalkRows / 2).
+// The walk over a level: rows per item chosen so that the items about fill
+// the launch (at most kWalkRows).
+template <class T, template <class> class Sp>
+DARCY_INLINE void presmooth_walk(const LvP<T, Sp>& L, Exec e) {
+    const int n = L.n;
+    presmooth_walk_rows(L, 0, n, walk_strip(n, (n + 1) >> 1, e.nthr, kWalkRows), e);
+}
+
+// Residual and restriction of coarse rows [Ia, Ib) as walks of `strip`
+// coarse rows; x of fine rows 2 Ia - 1 .. 2 Ib must be final.
 template <class T, template <class> class Sp, class BP>
-DARCY_INLINE void restrict_walk(const LvP<T, Sp>& L, BP bc, Exec e) {
+DARCY_INLINE void restrict_walk_rows(const LvP<T, Sp>& L, BP bc, int Ia, int Ib, int strip, Exec e) {
     const int m = (L.n + 1) >> 1;
-    long long per = (static_cast<long long>(m) * m) / e.nthr;
-    const int rows = per < 1 ? 1 : (per > kWalkRows / 2 ? kWalkRows / 2 : static_cast<int>(per));
-    const int strips = (m + rows - 1) / rows, total = strips * m;
+    const int strips = (Ib - Ia + strip - 1) / strip, total = strips * m;
     const int lane = threadIdx.x & (kWave - 1);
     for (int w = e.tid; w < total; w += e.nthr) {
-        const int st = w / m, J = w - st * m, I0 = st * rows, I1 = I0 + rows < m ? I0 + rows : m;
+        const int st = w / m, J = w - st * m, I0 = Ia + st * strip, I1 = I0 + strip < Ib ? I0 + strip : Ib;
         const LaneShare sh{lane + 1 < kWave && J + 1 < m, lane > 0 && J > 0};
         if (L.n & 1)
             restrict_walk_item<T, false>(L, bc, J, I0, I1, sh);
         else
             restrict_walk_item<T, true>(L, bc, J, I0, I1, sh);
     }
+}
+
+// The restriction walk over a level, coarse rows per item chosen as in
+// presmooth_walk (at most kWalkRows / 2).
+template <class T, template <class> class Sp, class BP>
+DARCY_INLINE void restrict_walk(const LvP<T, Sp>& L, BP bc, Exec e) {
+    const int m = (L.n + 1) >> 1;
+    restrict_walk_rows(L, bc, 0, m, walk_strip(m, m, e.nthr, kWalkRows / 2), e);
 }
 
 // x and the coarse correction x_c of its aggregate, read together; XPlusC
@@ -780,7 +800,7 @@ DARCY_INLINE void cycle_levels(int l0, int L, bool fresh0, const CycleOpts& o, E
 // 352 -> 345 ms).  The step size does not change any result.
 constexpr int kStepCellsShared = 1024;
 constexpr int kStepCellsAlone = 4096;
-DARCY_INLINE int rows_per_step(int n, int cells) {
+DARCY_HD __forceinline__ int rows_per_step(int n, int cells) {
     int r = cells / n;
     r &= ~1;
     return r < 2 ? 2 : r;
@@ -854,6 +874,34 @@ DARCY_INLINE void restrict_rows(const LvP<T, Sp>& L, BP bc, int Ia, int Ib, Exec
             if (q0 + u * e.nthr < qb) bc[q0 + u * e.nthr] = v[u];
     }
     (void)n;
+}
+
+// A fresh down visit (pre-smoothing from x = 0, residual, restriction;
+// multigrid.hpp:107-115 with pre_sweeps = 1) by one workgroup as walks,
+// R rows per step: the pre-smoothing walks over the step's rows, a barrier,
+// then the restriction walks over the coarse rows whose three fine rows
+// around them are final.  The walks hold a column pair's rows in registers
+// and take the neighbouring pair's values by lane shuffles, so a cell costs
+// a few loads where the per-cell phases of down_stream re-read every
+// neighbour.  The restriction of step t reads rows below r1 only, and the
+// pre-smoothing of step t + 1 reads no x, so no barrier separates them.
+// Same values and operation order as down_stream (the walks restate
+// presmooth_from_zero and restricted_residual).
+template <class T, template <class> class Sp, class BP>
+DARCY_INLINE void down_walk(const LvP<T, Sp>& L, BP bc, int step_cells, Exec e) {
+    const int n = L.n, m = (n + 1) >> 1, R = rows_per_step(n, step_cells);
+    int coarse_done = 0;
+    for (int r0 = 0; r0 < n; r0 += R) {
+        const int r1 = r0 + R < n ? r0 + R : n;
+        presmooth_walk_rows(L, r0, r1, walk_strip(r1 - r0, m, e.nthr, kWalkRows), e);
+        __syncthreads();
+        // rows < r1 are final; coarse row I needs fine rows 2I - 1 .. 2I + 2
+        const int coarse_hi = r1 == n ? m : (r1 - 1) / 2;
+        if (coarse_hi > coarse_done)
+            restrict_walk_rows(L, bc, coarse_done, coarse_hi, walk_strip(coarse_hi - coarse_done, m, e.nthr, kWalkRows / 2), e);
+        coarse_done = coarse_hi;
+    }
+    __syncthreads();
 }
 
 // One pre-smoothing sweep (red, black) and the restricted residual in a

@@ -11,13 +11,14 @@ none of them changes a result, only how the work is launched:
   grid_graph      replay the grid path's cycle from a captured HIP graph
   batch_step_cells  cells per row-pipeline step of the persistent kernel's
                   levels in global memory (its tile of the smoothing and
-                  restriction sweeps)
+                  restriction sweeps), FP64 cycle
+  batch_step_cells_fp32  the same for the FP32 cycle
 
 The tuner times candidate settings on synthetic two-phase media and picks the
 fastest: first grid_tail_side x grid_graph over a set of grid-path sizes
 (total time), then grid_min_n as the largest size at which the persistent
-kernel still beats the grid path, then batch_step_cells on a batch of
-persistent-kernel solves.  Results go to a JSON table per device
+kernel still beats the grid path, then the two step sizes on a batch of
+persistent-kernel solves, each in its own precision.  Results go to a JSON table per device
 architecture (darcy/tuning/<arch>.json), which capi.Context and the
 process-wide context of darcy._core apply when they are created.
 
@@ -40,8 +41,8 @@ from . import capi
 from .fields import gaussian_random_field, two_phase_permeability
 
 TABLE_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning")
-KEYS = ("grid_min_n", "grid_tail_side", "grid_graph", "batch_step_cells")
-STEP_CELLS = (512, 1024, 2048, 4096)
+KEYS = ("grid_min_n", "grid_tail_side", "grid_graph", "batch_step_cells", "batch_step_cells_fp32")
+STEP_CELLS = (512, 1024, 2048, 4096, 8192)
 
 
 def table_path(arch: str) -> str:
@@ -131,17 +132,21 @@ def tune(ctx: capi.Context, sizes: Iterable[int], small: Iterable[int], repeats:
             log(json.dumps(results["switch"][-1]))
             if t_batch <= t_grid:
                 switch = n
-        # 3. The persistent kernel's pipeline step, on a batch of its solves.
-        step = None
-        for cells_per_step in STEP_CELLS:
-            ctx.set_tuning("batch_step_cells", cells_per_step)
-            ms = _time_batch(ctx, batch_n, batch_ptrs, batch_out, precision, repeats)
-            results["batch"].append({"batch_step_cells": cells_per_step, "n": batch_n, "count": batch_count,
-                                     "kernel_ms": ms})
-            log(json.dumps(results["batch"][-1]))
-            if step is None or ms < step[0]:
-                step = (ms, cells_per_step)
-        chosen = {"grid_min_n": switch, "grid_tail_side": tail, "grid_graph": graph, "batch_step_cells": step[1]}
+        # 3. The persistent kernel's pipeline step, on a batch of its solves,
+        #    for each cycle precision.
+        chosen = {"grid_min_n": switch, "grid_tail_side": tail, "grid_graph": graph}
+        for key, prec in (("batch_step_cells", "fp64"), ("batch_step_cells_fp32", "mixed")):
+            step = None
+            for cells_per_step in STEP_CELLS:
+                ctx.set_tuning(key, cells_per_step)
+                ms = _time_batch(ctx, batch_n, batch_ptrs, batch_out, prec, repeats)
+                results["batch"].append({key: cells_per_step, "precision": prec, "n": batch_n,
+                                         "count": batch_count, "kernel_ms": ms})
+                log(json.dumps(results["batch"][-1]))
+                if step is None or ms < step[0]:
+                    step = (ms, cells_per_step)
+            ctx.set_tuning(key, step[1])
+            chosen[key] = step[1]
     finally:
         for k, v in saved.items():
             ctx.set_tuning(k, v)

@@ -105,8 +105,9 @@ void darcy_ctx_destroy(darcy_ctx* ctx);
  *   "grid_cg_per_cu"  workgroups per CU of the grid path's streaming CG
  *                     kernels (1..8, default 4; not autotuned)
  *   "batch_step_cells" cells per row-pipeline step on the levels the
- *                     persistent batch kernel keeps in global memory
- *                     (128..16384, default 1024)
+ *                     persistent batch kernel keeps in global memory, in
+ *                     the FP64 cycle (128..16384, default 1024)
+ *   "batch_step_cells_fp32" the same in the FP32 cycle (default 4096)
  * None of these changes any result: only the launch structure.  One more
  * key changes the arithmetic of the grid path's FP64 cycle, not the method:
  *   "grid_fp64_dense_tail" 1 (default): the levels of side <= 8 are applied

@@ -62,8 +62,9 @@ def test_autotuner_picks_valid_shapes_and_restores_the_context():
         assert chosen["grid_tail_side"] in (32, 64, 128) and chosen["grid_graph"] in (0, 1)
         assert chosen["grid_min_n"] in (1, 64, 96)
         assert chosen["batch_step_cells"] in autotune.STEP_CELLS
+        assert chosen["batch_step_cells_fp32"] in autotune.STEP_CELLS
         assert len(table["results"]["grid"]) == 6 and len(table["results"]["switch"]) == 2
-        assert len(table["results"]["batch"]) == len(autotune.STEP_CELLS)
+        assert len(table["results"]["batch"]) == 2 * len(autotune.STEP_CELLS)
         autotune.apply(ctx, table)
         assert {k: ctx.get_tuning(k) for k in autotune.KEYS} == chosen
 
@@ -120,6 +121,7 @@ def test_batch_step_cells_do_not_change_results(precision):
             for cells in (128, 1024, 4096, 16384):
                 print(f"batch step {cells} {precision}", flush=True)
                 ctx.set_tuning("batch_step_cells", cells)
+                ctx.set_tuning("batch_step_cells_fp32", cells)
                 st = ctx.solve_batch_device(sizes, [b[0] for b in bufs], [b[1] for b in bufs], precision=precision)
                 us = []
                 for (_, pu), a in zip(bufs, media):

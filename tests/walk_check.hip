@@ -1,5 +1,5 @@
-// Host build of the grid path's vertical walks (presmooth_walk_item,
-// restrict_walk_item in device.hpp), checked bit for bit against plain loops:
+// Host build of the vertical walks (presmooth_walk_item, restrict_walk_item
+// in device.hpp), checked bit for bit against plain loops:
 // the red sweep from zero then the black sweep with the stored inv_diag
 // (level.hpp:145-161), and the residual summed over each 2x2 aggregate
 // (level.hpp:122-137, :214-227).  Every size from 1 cell up, odd and even
@@ -124,6 +124,93 @@ int check_restrict(int n, long long nthr, unsigned seed, bool pairs) {
     return bad;
 }
 
+// The fresh down visit as the persistent kernel runs it (down_walk): steps
+// of R rows, the pre-smoothing walks over a step's rows, then the
+// restriction walks over the coarse rows whose fine rows around them are
+// final.  Executed in that order here, a coarse row taken too early reads
+// the sentinel x of a row not yet smoothed and fails the comparison.
+template <class T>
+int check_down(int n, int step_cells, int nthr, unsigned seed, bool pairs) {
+    std::mt19937 g(seed);
+    std::uniform_real_distribution<double> U(0.1, 12.0), V(-1.0, 1.0);
+    const int m = (n + 1) / 2, h = m;
+    std::vector<T> ex((size_t)n * (n + 1)), ny((size_t)(n + 1) * n), b((size_t)n * n);
+    T* x = new T[(size_t)n * n];
+    T* bc = new T[(size_t)m * m];
+    std::vector<T> xr((size_t)n * n), bcr((size_t)m * m);
+    for (auto& v : ex) v = (T)U(g);
+    for (auto& v : ny) v = (T)U(g);
+    for (auto& v : b) v = (T)V(g);
+    for (size_t i = 0; i < (size_t)n * n; ++i) x[i] = (T)12345;
+    for (size_t i = 0; i < (size_t)m * m; ++i) bc[i] = (T)777;
+    auto inv = [&](int i, int j) {
+        const int f = i * (n + 1) + j;
+        return inv_from_faces(ex[f + 1], ex[f], ny[(i + 1) * n + j], ny[i * n + j]);
+    };
+    for (int i = 0; i < n; ++i)
+        for (int j = (i & 1); j < n; j += 2) xr[i * n + j] = b[i * n + j] * inv(i, j);
+    for (int i = 0; i < n; ++i)
+        for (int j = 1 - (i & 1); j < n; j += 2) {
+            const int f = i * (n + 1) + j, k = i * n + j;
+            T v = b[k];
+            if (j + 1 < n) v += ex[f + 1] * xr[k + 1];
+            if (j > 0) v += ex[f] * xr[k - 1];
+            if (i + 1 < n) v += ny[(i + 1) * n + j] * xr[k + n];
+            if (i > 0) v += ny[i * n + j] * xr[k - n];
+            xr[k] = v * inv(i, j);
+        }
+    for (int I = 0; I < m; ++I)
+        for (int J = 0; J < m; ++J) {
+            T s = T(0);
+            for (int d = 0; d < 4; ++d) {
+                const int i = 2 * I + (d >> 1), j = 2 * J + (d & 1);
+                if (i >= n || j >= n) continue;
+                const int f = i * (n + 1) + j, k = i * n + j;
+                const T e = ex[f + 1], w = ex[f], nn = ny[(i + 1) * n + j], so = ny[i * n + j];
+                const T dd = ((e + w) + nn) + so;
+                T v = dd * xr[k];
+                if (j + 1 < n) v -= e * xr[k + 1];
+                if (j > 0) v -= w * xr[k - 1];
+                if (i + 1 < n) v -= nn * xr[k + n];
+                if (i > 0) v -= so * xr[k - n];
+                s += b[k] - v;
+            }
+            bcr[I * m + J] = s;
+        }
+    LvP<T> L{n, ex.data(), ny.data(), nullptr, x, b.data()};
+    const int R = rows_per_step(n, step_cells);
+    int coarse_done = 0;
+    for (int r0 = 0; r0 < n; r0 += R) {
+        const int r1 = r0 + R < n ? r0 + R : n;
+        const int strip = walk_strip(r1 - r0, h, nthr, kWalkRows);
+        const int strips = (r1 - r0 + strip - 1) / strip;
+        for (int w = 0; w < strips * h; ++w) {
+            const int st = w / h, k = w - st * h, i0 = r0 + st * strip, i1 = i0 + strip < r1 ? i0 + strip : r1;
+            if ((n & 1) || !pairs) presmooth_walk_item<T, false>(L, k, i0, i1);
+            else presmooth_walk_item<T, true>(L, k, i0, i1);
+        }
+        const int coarse_hi = r1 == n ? m : (r1 - 1) / 2;
+        if (coarse_hi > coarse_done) {
+            const int cs = walk_strip(coarse_hi - coarse_done, m, nthr, kWalkRows / 2);
+            const int cstrips = (coarse_hi - coarse_done + cs - 1) / cs;
+            for (int w = 0; w < cstrips * m; ++w) {
+                const int st = w / m, J = w - st * m, I0 = coarse_done + st * cs, I1 = I0 + cs < coarse_hi ? I0 + cs : coarse_hi;
+                if ((n & 1) || !pairs) restrict_walk_item<T, false>(L, bc, J, I0, I1);
+                else restrict_walk_item<T, true>(L, bc, J, I0, I1);
+            }
+        }
+        coarse_done = coarse_hi;
+    }
+    int bad = 0;
+    for (size_t i = 0; i < (size_t)n * n; ++i) bad += std::memcmp(&x[i], &xr[i], sizeof(T)) != 0;
+    for (size_t i = 0; i < (size_t)m * m; ++i) bad += std::memcmp(&bc[i], &bcr[i], sizeof(T)) != 0;
+    printf("down %s n=%d step=%d nthr=%d pairs=%d: %s\n", sizeof(T) == 8 ? "f64" : "f32", n, step_cells, nthr, (int)pairs,
+           bad ? "MISMATCH" : "ok");
+    delete[] x;
+    delete[] bc;
+    return bad;
+}
+
 int main() {
     int bad = 0;
     const int sizes[] = {1, 2, 3, 5, 8, 65, 66, 67, 100, 125, 128, 129, 257, 512};
@@ -138,6 +225,16 @@ int main() {
                 bad += check_restrict<float>(n, t, seed++, p);
             }
         }
+    // the persistent kernel's pipelined visits: 256 threads, step sizes of
+    // the tuning range and beyond, odd and even sides
+    const int vsizes[] = {17, 18, 33, 64, 65, 100, 128, 129, 256, 257};
+    const int steps[] = {64, 512, 1024, 2048, 4096, 1 << 20};
+    for (int n : vsizes)
+        for (int st : steps)
+            for (int p = 0; p < 2; ++p) {
+                bad += check_down<double>(n, st, 256, seed++, p);
+                bad += check_down<float>(n, st, 256, seed++, p);
+            }
     printf(bad ? "FAILED\n" : "ALL OK\n");
     return bad != 0;
 }
