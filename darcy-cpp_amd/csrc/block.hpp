@@ -390,23 +390,28 @@ DARCY_INLINE void cg_direction(int n, const GlobalLv<double>& A0, const Zt* zf, 
     struct PAp {
         double p, ap, u;
     };
-    // `load(t)` reads what p at t is formed from and `pv` forms it.  All of a
-    // cell's loads are issued before any arithmetic (the scheduling barrier
-    // keeps them together): left to itself, the scheduler of this
-    // register-bound kernel waits for each stencil point's loads before
-    // issuing the next point's.
-    auto pass = [&](auto&& load, auto&& pv, auto upd) {
+    // `load(t)` reads what p at t is formed from, `load3(k)` the same at
+    // k - 1, k, k + 1 as one access per array (a value off the row is read
+    // and its product selected away), and `pv` forms p.  All of a cell's
+    // loads are issued before any arithmetic (the scheduling barrier keeps
+    // them together): left to itself, the scheduler of this register-bound
+    // kernel waits for each stencil point's loads before issuing the next
+    // point's.
+    auto pass = [&](auto&& load, auto&& load3, auto&& pv, auto upd) {
         constexpr bool U = decltype(upd)::value;
         chunk_pass<1, CPR>(
             n * n, w, nw, part,
             [&](int k) {
                 const int i = k / n, j = k - i * n, f = i * (n + 1) + j;
                 const bool hE = j + 1 < n, hW = j > 0, hN = i + 1 < n, hS = i > 0;
-                const int t[5] = {k, hE ? k + 1 : k, hW ? k - 1 : k, hN ? k + n : k, hS ? k - n : k};
                 const double e = A0.ex[f + 1], wf = A0.ex[f], nn = A0.ny[(i + 1) * n + j], s = A0.ny[i * n + j];
                 decltype(load(0)) l[5];
-#pragma unroll
-                for (int q = 0; q < 5; ++q) l[q] = load(t[q]);
+                const auto r3 = load3(k);
+                l[0] = r3.c;
+                l[1] = r3.e;
+                l[2] = r3.w;
+                l[3] = load(hN ? k + n : k);
+                l[4] = load(hS ? k - n : k);
                 double uk = 0.0;
                 if constexpr (U) uk = u[k];
                 __builtin_amdgcn_sched_barrier(0);
@@ -433,16 +438,22 @@ DARCY_INLINE void cg_direction(int n, const GlobalLv<double>& A0, const Zt* zf, 
         double p;
     };
     auto zp = [&](int t) { return ZP{z[t], pold[t]}; };
+    auto zp3 = [&](int t) {
+        const Row3<Zt> a = load_row3(z + t);
+        const Row3<double> b = load_row3(pold + t);
+        return Row3<ZP>{ZP{a.w, b.w}, ZP{a.c, b.c}, ZP{a.e, b.e}};
+    };
     auto zpv = [&](const ZP& v) { return static_cast<double>(v.z) + beta * v.p; };
     if (first)
-        pass([&](int t) { return z[t]; }, [](Zt v) { return static_cast<double>(v); }, Flag<false>{});
+        pass([&](int t) { return z[t]; }, [&](int t) { return load_row3(z + t); },
+             [](Zt v) { return static_cast<double>(v); }, Flag<false>{});
     else if constexpr (DEFER) {
         if (uf)
-            pass(zp, zpv, Flag<true>{});
+            pass(zp, zp3, zpv, Flag<true>{});
         else
-            pass(zp, zpv, Flag<false>{});
+            pass(zp, zp3, zpv, Flag<false>{});
     } else
-        pass(zp, zpv, Flag<false>{});
+        pass(zp, zp3, zpv, Flag<false>{});
 }
 
 // Lane l adds chunk sums l, l+64, l+128, ... in that order; the loads of

@@ -75,7 +75,12 @@ struct Pointee<__attribute__((address_space(1))) T*> {
     using type = std::remove_cv_t<T>;
 };
 template <class P>
-using pointee_t = typename Pointee<P>::type;
+using pointee_t = typename Pointee<std::remove_cv_t<P>>::type;
+
+template <class P>
+struct IsGlobalPtr : std::false_type {};
+template <class T>
+struct IsGlobalPtr<__attribute__((address_space(1))) T*> : std::true_type {};
 
 // p viewed as a U pointer in the same memory.
 template <class U, class T>
@@ -107,6 +112,75 @@ struct LvP {
 };
 template <class T>
 using GlobalLv = LvP<T, GlobalPtr>;
+
+// A cell's west, own and east values.
+template <class T>
+struct Row3 {
+    T w, c, e;
+};
+
+// Vectors of 2 and 3 elements aligned as one element: loaded as one access
+// (dwordx2 / dwordx3, or dwordx4 + dwordx2 for three doubles) wherever they
+// start.
+template <class T>
+struct Vec;
+template <>
+struct Vec<float> {
+    typedef float v2 __attribute__((ext_vector_type(2), aligned(4)));
+    typedef float v3 __attribute__((ext_vector_type(3), aligned(4)));
+};
+template <>
+struct Vec<double> {
+    typedef double v2 __attribute__((ext_vector_type(2), aligned(8)));
+    typedef double v3 __attribute__((ext_vector_type(3), aligned(8)));
+};
+
+// p[-1], p[0], p[1] as one access.  At a row's first or last cell the
+// element beyond it belongs to the neighbouring row, to the alignment
+// padding between arrays or to the guard bytes of the slot (or of the LDS
+// pool): it is read and not used.
+template <class P>
+DARCY_HD __forceinline__ Row3<pointee_t<P>> load_row3(P p) {
+    using T = pointee_t<P>;
+    const typename Vec<T>::v3 v = *retype<const typename Vec<T>::v3>(p - 1);
+    return Row3<T>{v.x, v.y, v.z};
+}
+
+// p[0], p[1] as one access.
+template <class P>
+DARCY_HD __forceinline__ Row3<pointee_t<P>> load_two(P p) {
+    using T = pointee_t<P>;
+    const typename Vec<T>::v2 v = *retype<const typename Vec<T>::v2>(p);
+    return Row3<T>{v.x, v.x, v.y};
+}
+
+// x as stored, read by grid coordinates; row3 reads a cell's west, own and
+// east values as one access (gs_at and apply_at take it where UsesRow3).
+template <class P>
+struct XRow {
+    static constexpr bool kRow3 = IsGlobalPtr<std::remove_cv_t<P>>::value;
+    P x;
+    int n;
+    DARCY_INLINE pointee_t<P> operator()(int a, int c) const { return x[a * n + c]; }
+    DARCY_INLINE Row3<pointee_t<P>> row3(int a, int c) const { return load_row3(x + a * n + c); }
+};
+
+// x as stored, read by flat index.
+template <class P>
+struct XFlat {
+    static constexpr bool kRow3 = IsGlobalPtr<std::remove_cv_t<P>>::value;
+    P x;
+    DARCY_INLINE pointee_t<P> operator()(int t) const { return x[t]; }
+    DARCY_INLINE Row3<pointee_t<P>> row3(int t) const { return load_row3(x + t); }
+};
+
+// Whether a loader's row3 is used: only for arrays in global memory.  In LDS
+// the three values are three reads either way (the wide read would need
+// 16-byte alignment), one more than the two neighbours alone.
+template <class Ld, class = void>
+struct UsesRow3 : std::false_type {};
+template <class Ld>
+struct UsesRow3<Ld, std::void_t<decltype(Ld::kRow3)>> : std::integral_constant<bool, Ld::kRow3> {};
 
 // ------------------------------------------------------------------ sync
 
@@ -161,8 +235,18 @@ DARCY_INLINE pointee_t<P> apply_at(int n, P ex, P ny, int i, int j, X&& x) {
     const int k = i * n + j, f = i * (n + 1) + j;
     const T e = ex[f + 1], w = ex[f], nn = ny[(i + 1) * n + j], s = ny[i * n + j];
     const bool hE = j + 1 < n, hW = j > 0, hN = i + 1 < n, hS = i > 0;
-    const T xk = x(k), xE = x(hE ? k + 1 : k), xW = x(hW ? k - 1 : k), xN = x(hN ? k + n : k),
-            xS = x(hS ? k - n : k);
+    T xk, xE, xW;
+    if constexpr (UsesRow3<std::decay_t<X>>::value) {
+        const Row3<T> r = x.row3(k);  // off-row values are read and not used
+        xk = r.c;
+        xE = r.e;
+        xW = r.w;
+    } else {
+        xk = x(k);
+        xE = x(hE ? k + 1 : k);
+        xW = x(hW ? k - 1 : k);
+    }
+    const T xN = x(hN ? k + n : k), xS = x(hS ? k - n : k);
     __builtin_amdgcn_sched_barrier(0);  // all loads in flight together, as in gs_at
     const T d = ((e + w) + nn) + s;
     T v = d * xk;
@@ -226,8 +310,16 @@ DARCY_INLINE T gs_at(const LvP<T, Sp>& L, int i, int j, T bk, Ld&& ld, Cv&& cv =
     const int n = L.n, f = i * (n + 1) + j;
     const bool hE = j + 1 < n, hW = j > 0, hN = i + 1 < n, hS = i > 0;
     const T e = L.ex[f + 1], w = L.ex[f], nn = L.ny[(i + 1) * n + j], s = L.ny[i * n + j];
-    const auto rE = ld(i, hE ? j + 1 : j), rW = ld(i, hW ? j - 1 : j), rN = ld(hN ? i + 1 : i, j),
-               rS = ld(hS ? i - 1 : i, j);
+    decltype(ld(i, j)) rE, rW;
+    if constexpr (UsesRow3<std::decay_t<Ld>>::value) {
+        const auto r = ld.row3(i, j);  // off-row values are read and not used
+        rE = r.e;
+        rW = r.w;
+    } else {
+        rE = ld(i, hE ? j + 1 : j);
+        rW = ld(i, hW ? j - 1 : j);
+    }
+    const auto rN = ld(hN ? i + 1 : i, j), rS = ld(hS ? i - 1 : i, j);
     __builtin_amdgcn_sched_barrier(0);
     const T inv = inv_from_faces(e, w, nn, s);
     const T xE = cv(rE), xW = cv(rW), xN = cv(rN), xS = cv(rS);
@@ -267,7 +359,7 @@ DARCY_INLINE void gs_colour(const LvP<T, Sp>& L, int colour, Exec e) {
             kk[u] = -1;
             if (q < total && colour_cell(q, n, h, colour, i, j)) {
                 kk[u] = i * n + j;
-                v[u] = gs_at(L, i, j, L.b[kk[u]], [&](int a, int c) { return x[a * n + c]; });
+                v[u] = gs_at(L, i, j, L.b[kk[u]], XRow<decltype(x)>{x, n});
             }
         }
 #pragma unroll
@@ -636,10 +728,19 @@ struct XC {
 };
 template <class T, class PX, class PC>
 struct Prolonged {
+    static constexpr bool kRow3 = IsGlobalPtr<std::remove_cv_t<PX>>::value && IsGlobalPtr<std::remove_cv_t<PC>>::value;
     PX x;
     PC xc;
     int n, m;
     DARCY_INLINE XC<T> operator()(int a, int c) const { return XC<T>{x[a * n + c], xc[(a >> 1) * m + (c >> 1)]}; }
+    // West and east neighbours of (a, c): x as one access, and their
+    // aggregates' x_c, always adjacent ((c - 1) >> 1 and (c + 1) >> 1), as
+    // another.  Off-grid values are read and not used.
+    DARCY_INLINE Row3<XC<T>> row3(int a, int c) const {
+        const Row3<T> v = load_row3(x + a * n + c);
+        const Row3<T> q = load_two(xc + (a >> 1) * m + ((c - 1) >> 1));
+        return Row3<XC<T>>{XC<T>{v.w, q.w}, XC<T>{v.c, q.c}, XC<T>{v.e, q.e}};
+    }
 };
 struct XPlusC {
     template <class T>
@@ -848,7 +949,7 @@ DARCY_INLINE T restricted_residual(const LvP<T, Sp>& L, int q, int m) {
     for (int d = 0; d < 4; ++d) {
         const int i = 2 * I + (d >> 1), j = 2 * J + (d & 1);
         ok[d] = i < n && j < n;
-        if (ok[d]) r[d] = L.b[i * n + j] - apply_at(n, L.ex, L.ny, i, j, [&](int t) { return x[t]; });
+        if (ok[d]) r[d] = L.b[i * n + j] - apply_at(n, L.ex, L.ny, i, j, XFlat<decltype(x)>{x});
     }
     T s = T(0);
 #pragma unroll
@@ -911,7 +1012,7 @@ DARCY_INLINE void down_stream(const LvP<T, Sp>& L, BP bc, bool fresh, int step_c
     const int n = L.n, m = (n + 1) >> 1, R = rows_per_step(n, step_cells);
     const auto x = L.x;
     const auto b = L.b;
-    auto xv = [&](int a, int c) { return x[a * n + c]; };
+    const XRow<decltype(x)> xv{x, n};
     int coarse_done = 0;
     for (int r0 = 0;; r0 += R) {
         const int red_hi = r0 + R < n ? r0 + R : n;
@@ -947,7 +1048,7 @@ DARCY_INLINE void up_stream(const LvP<T, Sp>& L, PC xc, int step_cells, Exec e) 
     const int n = L.n, m = (n + 1) >> 1, R = rows_per_step(n, step_cells);
     const auto x = L.x;
     const auto b = L.b;
-    auto xv = [&](int a, int c) { return x[a * n + c]; };
+    const XRow<decltype(x)> xv{x, n};
     const Prolonged<T, decltype(x), PC> pr{x, xc, n, m};
     for (int r0 = 0;; r0 += R) {
         const int black_hi = r0 + R < n ? r0 + R : n;

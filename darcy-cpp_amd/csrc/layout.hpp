@@ -99,6 +99,12 @@ DARCY_HD inline size_t plan_slot(char* base, int n0, bool mixed, Hier* h, int mi
     const size_t N0 = static_cast<size_t>(n0) * n0;
     Hier& t = *h;
     t.L = level_count(n0, min_coarse, max_levels);
+    // Guard bytes before the first array (and after the last): the kernels
+    // read a row's west, own and east values as one access, one element
+    // beyond an array's ends at its first and last cell (the value is then
+    // unused).  Between arrays the alignment padding or the next array is
+    // read.
+    take(kAlign);
     t.u = reinterpret_cast<double*>(take(N0 * 8));
     t.r = reinterpret_cast<double*>(take(N0 * 8));
     t.p = reinterpret_cast<double*>(take(N0 * 8));
@@ -137,6 +143,7 @@ DARCY_HD inline size_t plan_slot(char* base, int n0, bool mixed, Hier* h, int mi
         }
         n = (n + 1) / 2;
     }
+    take(kAlign);
     return off;
 }
 
