@@ -38,9 +38,6 @@ from darcy.fields import gaussian_random_field, two_phase_permeability  # noqa: 
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 matrix peak
-# Network FLOPs per 256 x 256 sample: sum of 2 * pixels * 9 * cin * cout over
-# its convolutions (the first conv's 8 staged input channels counted).
-NET_FLOPS_PER_SAMPLE = 4.7563e9
 # Bytes between the L2 caches and the memory fabric (an upper bound on HBM
 # bytes: Infinity Cache hits are included) moved by k_batch<float> in one
 # launch of the default workload's learned line (1000 x 256^2, W-cycle,
@@ -277,6 +274,7 @@ def main() -> int:
     ap.add_argument("--cycle", type=int, default=2)
     ap.add_argument("--cpu-sample", type=int, default=320)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--weights", default=None, help="warm-start weight file (default: the committed weights)")
     ap.add_argument("--copy-only", type=int, default=0,
                     help="only run the streaming copy kernel on this many MiB (profiler calibration)")
     args = ap.parse_args()
@@ -307,9 +305,10 @@ def main() -> int:
     a_ptrs = [a_base + i * cells * 8 for i in range(batch)]
     u_ptrs = [u_base + i * cells * 8 for i in range(batch)]
 
-    from darcy.learned import WarmStart, param_count
+    from darcy.learned import DEFAULT_WEIGHTS, WarmStart, network_flops, param_count
 
-    warm = WarmStart(ctx, n)  # fails loudly without the trained weights
+    warm = WarmStart(ctx, n, weights=args.weights or DEFAULT_WEIGHTS)  # fails loudly without trained weights
+    net_sub = warm.sub
     u0_base = ctx.alloc(a.nbytes)
     u0_ptrs = [u0_base + i * cells * 8 for i in range(batch)]
 
@@ -339,7 +338,8 @@ def main() -> int:
     ctx.close()
 
     learned, mixed = out["learned"], out["mixed"]
-    net_flops = NET_FLOPS_PER_SAMPLE * batch
+    flops_per_sample = network_flops(n, net_sub)
+    net_flops = flops_per_sample * batch
     line = {
         "metric": "solves/s to 1e-10 resid, 256² batch: learned+FP32-MG vs plain FP64 MG-CG",
         "value": round(learned["solves_per_s"], 2),
@@ -356,8 +356,9 @@ def main() -> int:
         "config": {
             "workload": f"{batch} x {n}x{n} two-phase (12/3) GRF media, seeds {SEED0}-{SEED0 + batch - 1}, f=1, "
                         f"tol 1e-10, {'W' if args.cycle == 2 else 'V'}-cycle MG-CG; learned warm start (BF16 "
-                        f"encoder-decoder, {param_count()} parameters) then FP32 cycle in FP64 flexible CG; "
-                        f"network inference inside the timed region",
+                        f"encoder-decoder, {param_count(net_sub)} parameters, "
+                        f"{'full' if net_sub == 1 else 'half'}-resolution body) then FP32 cycle in FP64 "
+                        f"flexible CG; network inference inside the timed region",
             "global_batch": batch,
             "grid": n,
             "parallelism": "single GPU",
@@ -382,7 +383,8 @@ def main() -> int:
             "speedup_vs_fp64": round(mixed["solves_per_s"] / out["fp64"]["solves_per_s"], 3),
         },
         "network": {
-            "flops_per_sample": NET_FLOPS_PER_SAMPLE,
+            "flops_per_sample": flops_per_sample,
+            "sub_pixel_factor": net_sub,
             "inference_ms": round(learned["inference_ms"], 3),
             "TFLOPs": round(net_flops / (learned["inference_ms"] * 1e-3) / 1e12, 1),
             "frac_bf16_peak": round(net_flops / (learned["inference_ms"] * 1e-3) / 1e12 / BF16_PEAK_TFLOPS, 4),

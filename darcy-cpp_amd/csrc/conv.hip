@@ -765,39 +765,42 @@ __global__ __launch_bounds__(kT) void k_reduce_splits(const float* part, int spl
     }
 }
 
-// ------------------------------------------------- the head (Cin -> 1, FP32)
+// ------------------------------------------ the head (Cin -> C, FP32)
+// The head is a 3x3 conv from the finest network level (side m = n / sub)
+// to C channels, yc: [b][m][m][C] FP32; w: [C][9][cin]; bias: [C].  With
+// sub = 1, C = 1 and yc is the output y itself.  With sub = 2, C = 5 and
+// the output map below turns yc into the n x n field y.
 
-__global__ __launch_bounds__(kT) void k_head_fwd(const bf16* x, Geo in, const float* w, const float* bias, float* y) {
+__global__ __launch_bounds__(kT) void k_head_fwd(const bf16* x, Geo in, int C, const float* w, const float* bias,
+                                                 float* yc) {
     const size_t P = static_cast<size_t>(in.b) * in.h * in.w;
     for (size_t p = blockIdx.x * static_cast<size_t>(kT) + threadIdx.x; p < P; p += static_cast<size_t>(gridDim.x) * kT) {
         const int ox = static_cast<int>(p % in.w), t = static_cast<int>(p / in.w), oy = t % in.h, b = t / in.h;
-        float s = bias[0];
-        for (int tap = 0; tap < 9; ++tap) {
-            const int iy = oy + tap / 3 - 1, ix = ox + tap % 3 - 1;
-            if (iy < 0 || iy >= in.h || ix < 0 || ix >= in.w) continue;
-            const bf16* xi = x + at(in, b, iy, ix);
-            const float* wt = w + tap * in.c;
-            for (int ci = 0; ci < in.c; ci += 8) {
-                const v8bf v = *reinterpret_cast<const v8bf*>(xi + ci);
+        for (int r = 0; r < C; ++r) {
+            float s = bias[r];
+            for (int tap = 0; tap < 9; ++tap) {
+                const int iy = oy + tap / 3 - 1, ix = ox + tap % 3 - 1;
+                if (iy < 0 || iy >= in.h || ix < 0 || ix >= in.w) continue;
+                const bf16* xi = x + at(in, b, iy, ix);
+                const float* wt = w + (r * 9 + tap) * in.c;
+                for (int ci = 0; ci < in.c; ci += 8) {
+                    const v8bf v = *reinterpret_cast<const v8bf*>(xi + ci);
 #pragma unroll
-                for (int t = 0; t < 8; ++t) s += wt[ci + t] * bf2f(v[t]);
+                    for (int t8 = 0; t8 < 8; ++t8) s += wt[ci + t8] * bf2f(v[t8]);
+                }
             }
+            yc[p * C + r] = s;
         }
-        y[p] = s;
     }
 }
 
-// The head for 32 input channels: a workgroup computes a 16 x 16 pixel tile
-// from an 18 x 18 halo tile of x staged once in LDS (pixel stride padded to
-// 40 channels), instead of every thread re-reading its 9 neighbours.
-// The head, 3x3 conv from 32 channels to 1 in FP32: a 16 x 16 output tile
-// per workgroup over its 18 x 18 bf16 input patch in LDS (a pixel's 32
-// channels padded to 40, so 16 lanes reading consecutive pixels hit disjoint
-// banks), 288 fused multiply-adds per pixel.  Sum order: bias, then taps,
-// then channels.  The bf16 patch (20 KB) leaves room for 7 workgroups per CU,
-// which the staging loads need more than the arithmetic needs FP32 operands.
-// With `out.u0` the kernel stores the warm start itself, u0 = f scale pos y
-// in FP64 as k_output forms it, instead of y.
+// The head for 32 input channels: a 16 x 16 pixel tile per workgroup over
+// its 18 x 18 bf16 input patch, staged once in LDS (a pixel's 32 channels
+// padded to 40, so 16 lanes reading consecutive pixels hit disjoint banks);
+// each thread forms the C outputs of its pixel from one pass over the patch,
+// 288 fused multiply-adds each.  Sum order per output: bias, then taps, then
+// channels.  With `out.u0` (C = 1 only) the kernel stores the warm start
+// itself, u0 = f scale pos y in FP64 as k_output forms it, instead of y.
 struct HeadOut {
     const float* pos;
     const double* f;
@@ -805,13 +808,14 @@ struct HeadOut {
     double* const* u0;
 };
 constexpr int kHeadTile = 16, kHeadC = 32, kHeadPad = 40;
+template <int C>
 __global__ __launch_bounds__(kT) void k_head_fwd_tiled(const bf16* x, Geo in, const float* w, const float* bias,
-                                                       float* y, HeadOut out) {
+                                                       float* yc, HeadOut out) {
     __shared__ __align__(16) bf16 tile[(kHeadTile + 2) * (kHeadTile + 2) * kHeadPad];
-    __shared__ float ws[9 * kHeadC + 1];
+    __shared__ __align__(16) float ws[C * (9 * kHeadC + 1)];  // [tap][ci][r], then the biases
     const int tx0 = blockIdx.x * kHeadTile, ty0 = blockIdx.y * kHeadTile, b = blockIdx.z;
-    for (int i = threadIdx.x; i < 9 * kHeadC; i += kT) ws[i] = w[i];
-    if (threadIdx.x == 0) ws[9 * kHeadC] = bias[0];
+    for (int i = threadIdx.x; i < C * 9 * kHeadC; i += kT) ws[(i % (9 * kHeadC)) * C + i / (9 * kHeadC)] = w[i];
+    if (threadIdx.x < C) ws[C * 9 * kHeadC + threadIdx.x] = bias[threadIdx.x];
     constexpr int side = kHeadTile + 2, chunks = side * side * (kHeadC / 8);
     stage_items<chunks>(
         [&](int q) {
@@ -824,28 +828,36 @@ __global__ __launch_bounds__(kT) void k_head_fwd_tiled(const bf16* x, Geo in, co
     __syncthreads();
     const int ly = threadIdx.x / kHeadTile, lx = threadIdx.x % kHeadTile;
     const int oy = ty0 + ly, ox = tx0 + lx;
-    float s = ws[9 * kHeadC];
+    float s[C];
+#pragma unroll
+    for (int r = 0; r < C; ++r) s[r] = ws[C * 9 * kHeadC + r];
     for (int tap = 0; tap < 9; ++tap) {
         const bf16* xp = tile + ((ly + tap / 3) * side + lx + tap % 3) * kHeadPad;
-        const float* wt = ws + tap * kHeadC;
 #pragma unroll
         for (int ci = 0; ci < kHeadC; ci += 8) {
             const v8bf v = *reinterpret_cast<const v8bf*>(xp + ci);
 #pragma unroll
-            for (int e = 0; e < 8; ++e) s = fmaf(wt[ci + e], bf2f(v[e]), s);
+            for (int e = 0; e < 8; ++e) {
+                const float xv = bf2f(v[e]);
+                const float* wp = ws + (tap * kHeadC + ci + e) * C;
+#pragma unroll
+                for (int r = 0; r < C; ++r) s[r] = fmaf(wp[r], xv, s[r]);
+            }
         }
     }
     if (oy < in.h && ox < in.w) {
         const size_t k = static_cast<size_t>(oy) * in.w + ox;
-        if (out.u0)
-            out.u0[b][k] = out.f[b] * out.scale * static_cast<double>(out.pos[k] * s);
-        else
-            y[static_cast<size_t>(b) * in.h * in.w + k] = s;
+        if (C == 1 && out.u0) {
+            out.u0[b][k] = out.f[b] * out.scale * static_cast<double>(out.pos[k] * s[0]);
+        } else {
+#pragma unroll
+            for (int r = 0; r < C; ++r) yc[((static_cast<size_t>(b) * in.h * in.w) + k) * C + r] = s[r];
+        }
     }
 }
 
-// dX[pixel][ci] = sum_tap w[tap][ci] dy[pixel - tap].
-__global__ __launch_bounds__(kT) void k_head_bwd_data(const float* dy, Geo in, const float* w, bf16* dx) {
+// dX[pixel][ci] = sum_tap sum_r w[r][tap][ci] dyc[pixel - tap][r].
+__global__ __launch_bounds__(kT) void k_head_bwd_data(const float* dyc, Geo in, int C, const float* w, bf16* dx) {
     const size_t P = static_cast<size_t>(in.b) * in.h * in.w;
     for (size_t q = blockIdx.x * static_cast<size_t>(kT) + threadIdx.x; q < P * in.c; q += static_cast<size_t>(gridDim.x) * kT) {
         const int ci = static_cast<int>(q % in.c);
@@ -855,32 +867,111 @@ __global__ __launch_bounds__(kT) void k_head_bwd_data(const float* dy, Geo in, c
         for (int tap = 0; tap < 9; ++tap) {
             const int oy = y + 1 - tap / 3, ox = x + 1 - tap % 3;
             if (oy < 0 || oy >= in.h || ox < 0 || ox >= in.w) continue;
-            s += w[tap * in.c + ci] * dy[(static_cast<size_t>(b) * in.h + oy) * in.w + ox];
+            const float* d = dyc + ((static_cast<size_t>(b) * in.h + oy) * in.w + ox) * C;
+            for (int r = 0; r < C; ++r) s += w[(r * 9 + tap) * in.c + ci] * d[r];
         }
         dx[q] = f2bf(s);
     }
 }
 
 // Head weight gradient, stage 1: workgroup g sums pixels [g per, (g+1) per)
-// for all 9 Cin + 1 weights (thread i owns weights i, i + 256, ...).
-__global__ __launch_bounds__(kT) void k_head_dw_part(const bf16* x, Geo in, const float* dy, int per, float* part) {
+// for all C (9 Cin + 1) weights and biases (thread i owns entries i,
+// i + 256, ...); part[g][r][9 Cin + 1], the bias last.
+__global__ __launch_bounds__(kT) void k_head_dw_part(const bf16* x, Geo in, int C, const float* dyc, int per,
+                                                     float* part) {
     const int nw = 9 * in.c;
     const size_t P = static_cast<size_t>(in.b) * in.h * in.w;
     const size_t p0 = static_cast<size_t>(blockIdx.x) * per;
-    for (int widx = threadIdx.x; widx <= nw; widx += kT) {
-        float s = 0.0f;
+    for (int e = threadIdx.x; e < C * (nw + 1); e += kT) {
+        const int r = e / (nw + 1), widx = e % (nw + 1);
         const int ci = widx % in.c, tap = widx / in.c;
+        float s = 0.0f;
         for (size_t p = p0; p < p0 + per && p < P; ++p) {
+            const float d = dyc[p * C + r];
             if (widx == nw) {
-                s += dy[p];
+                s += d;
                 continue;
             }
             const int ox = static_cast<int>(p % in.w), t = static_cast<int>(p / in.w), oy = t % in.h, b = t / in.h;
             const int iy = oy + tap / 3 - 1, ix = ox + tap % 3 - 1;
             if (iy < 0 || iy >= in.h || ix < 0 || ix >= in.w) continue;
-            s += dy[p] * bf2f(x[at(in, b, iy, ix) + ci]);
+            s += d * bf2f(x[at(in, b, iy, ix) + ci]);
         }
-        part[static_cast<size_t>(blockIdx.x) * (nw + 1) + widx] = s;
+        part[(static_cast<size_t>(blockIdx.x) * C + r) * (nw + 1) + widx] = s;
+    }
+}
+
+// ------------------------------------ the half-resolution output map (sub 2)
+// yc: [b][m][m][5] -> y: [b][n][n], n = 2m.  Cell (i, j) = (2I + di, 2J + dj)
+// takes the bilinear interpolation of channel 0 at its centre (cell-centred
+// grids: weights 3/4 on its own pixel's row / column and 1/4 on the
+// neighbour's on its side, the neighbour index clamped to the grid) plus
+// its own sub-pixel channel 1 + 2 di + dj.  The interpolation makes a
+// smooth output by construction; the sub-pixel channels carry the detail
+// inside a block.
+
+// Neighbouring pixel index on the side of fine index i, and the weight pair.
+__device__ __forceinline__ int sub2_side(int i, int m) {
+    const int I = i >> 1, J = (i & 1) ? I + 1 : I - 1;
+    return J < 0 ? 0 : (J >= m ? m - 1 : J);
+}
+
+__device__ __forceinline__ float sub2_value(const float* yc, int m, int i, int j) {
+    const int I = i >> 1, J = j >> 1, Ii = sub2_side(i, m), Jj = sub2_side(j, m);
+    auto c0 = [&](int a, int c) { return yc[(static_cast<size_t>(a) * m + c) * 5]; };
+    const float row_own = 0.75f * c0(I, J) + 0.25f * c0(I, Jj);
+    const float row_side = 0.75f * c0(Ii, J) + 0.25f * c0(Ii, Jj);
+    return (0.75f * row_own + 0.25f * row_side) + yc[(static_cast<size_t>(I) * m + J) * 5 + 1 + 2 * (i & 1) + (j & 1)];
+}
+
+// y (training) or u0 = f scale pos y (inference, with out.u0).
+__global__ __launch_bounds__(kT) void k_sub2_out(const float* yc, int n, int batch, float* y, HeadOut out) {
+    const int m = n / 2;
+    const size_t per = static_cast<size_t>(n) * n;
+    for (size_t q = blockIdx.x * static_cast<size_t>(kT) + threadIdx.x; q < per * batch; q += static_cast<size_t>(gridDim.x) * kT) {
+        const size_t b = q / per, k = q % per;
+        const int i = static_cast<int>(k / n), j = static_cast<int>(k % n);
+        const float v = sub2_value(yc + b * static_cast<size_t>(m) * m * 5, m, i, j);
+        if (out.u0)
+            out.u0[b][k] = out.f[b] * out.scale * static_cast<double>(out.pos[k] * v);
+        else
+            y[q] = v;
+    }
+}
+
+// The adjoint: dyc[I][J][1 + r] = dy of the pixel's cell r; dyc[I][J][0] =
+// sum over the cells whose interpolation reads pixel (I, J) of their
+// weight times dy (fine rows 2I - 1 .. 2I + 2 and the same columns).
+__device__ __forceinline__ float sub2_weight(int i, int I, int m) {
+    // Weight of coarse index I in fine index i's interpolation (0 if unused).
+    float w = 0.0f;
+    if ((i >> 1) == I) w += 0.75f;
+    if (sub2_side(i, m) == I) w += 0.25f;
+    return w;
+}
+
+__global__ __launch_bounds__(kT) void k_sub2_adjoint(const float* dy, int n, int batch, float* dyc) {
+    const int m = n / 2;
+    const size_t per = static_cast<size_t>(m) * m;
+    for (size_t q = blockIdx.x * static_cast<size_t>(kT) + threadIdx.x; q < per * batch; q += static_cast<size_t>(gridDim.x) * kT) {
+        const size_t b = q / per, k = q % per;
+        const int I = static_cast<int>(k / m), J = static_cast<int>(k % m);
+        const float* d = dy + b * static_cast<size_t>(n) * n;
+        float s = 0.0f;
+        for (int i = 2 * I - 1; i <= 2 * I + 2; ++i) {
+            if (i < 0 || i >= n) continue;
+            const float wi = sub2_weight(i, I, m);
+            if (wi == 0.0f) continue;
+            float r = 0.0f;
+            for (int j = 2 * J - 1; j <= 2 * J + 2; ++j) {
+                if (j < 0 || j >= n) continue;
+                r += sub2_weight(j, J, m) * d[static_cast<size_t>(i) * n + j];
+            }
+            s += wi * r;
+        }
+        float* o = dyc + q * 5;
+        o[0] = s;
+        for (int r = 0; r < 4; ++r) o[1 + r] = d[static_cast<size_t>(2 * I + (r >> 1)) * n + 2 * J + (r & 1)];
     }
 }
 
@@ -896,18 +987,29 @@ __global__ __launch_bounds__(kT) void k_position(int n, float* pos) {
     }
 }
 
-// Input features, NHWC bf16 with 8 channels: (ln a - mu) / sd, 1 (the
-// forcing, which enters the output linearly) and the position; channels
-// 3..7 are 0.  mu, sd map the two phases ln 3 and ln 12 to -1 and +1.
-__global__ __launch_bounds__(kT) void k_features(const double* const* a, const float* pos, int n, int batch, bf16* x) {
-    const size_t per = static_cast<size_t>(n) * n;
+// Input features, NHWC bf16 with 8 channels per pixel of the network's
+// finest level (side m = n / sub, sub 1 or 2): the sub x sub cells' (ln a -
+// mu) / sd in row-major order, then 1 (the forcing, which enters the output
+// linearly), then the position (the mean of the cells' pos, summed in
+// row-major order in FP32); the rest is 0.  mu, sd map the two phases ln 3
+// and ln 12 to -1 and +1.
+__global__ __launch_bounds__(kT) void k_features(const double* const* a, const float* pos, int n, int sub, int batch,
+                                                 bf16* x) {
+    const int m = n / sub;
+    const size_t per = static_cast<size_t>(m) * m;
     const double mu = 0.5 * (log(3.0) + log(12.0)), sd = 0.5 * (log(12.0) - log(3.0));
     for (size_t q = blockIdx.x * static_cast<size_t>(kT) + threadIdx.x; q < per * batch; q += static_cast<size_t>(gridDim.x) * kT) {
         const size_t b = q / per, k = q % per;
+        const int I = static_cast<int>(k / m), J = static_cast<int>(k % m);
         v8bf v = {};
-        v[0] = f2bf(static_cast<float>((log(a[b][k]) - mu) / sd));
-        v[1] = f2bf(1.0f);
-        v[2] = f2bf(pos[k]);
+        float ps = 0.0f;
+        for (int r = 0; r < sub * sub; ++r) {
+            const size_t kf = static_cast<size_t>(sub * I + r / sub) * n + sub * J + r % sub;
+            v[r] = f2bf(static_cast<float>((log(a[b][kf]) - mu) / sd));
+            ps += pos[kf];
+        }
+        v[sub * sub] = f2bf(1.0f);
+        v[sub * sub + 1] = f2bf(ps * (1.0f / static_cast<float>(sub * sub)));
         *reinterpret_cast<v8bf*>(x + 8 * q) = v;
     }
 }
@@ -1166,36 +1268,54 @@ hipError_t darcy_nn_conv_bwd_weight(const void* x, int b, int h, int w, int cin,
     return hipGetLastError();
 }
 
-hipError_t darcy_nn_head_fwd(const void* x, int b, int h, int w, int cin, const float* wgt, const float* bias, float* y,
-                             hipStream_t s) {
+// The head conv: x [b][h][w][cin] -> yc [b][h][w][C], C 1 or 5.
+hipError_t darcy_nn_head_fwd(const void* x, int b, int h, int w, int cin, int C, const float* wgt, const float* bias,
+                             float* yc, hipStream_t s) {
+    if (C != 1 && C != 5) return hipErrorInvalidValue;
     if (cin == kHeadC) {
         const dim3 grid((w + kHeadTile - 1) / kHeadTile, (h + kHeadTile - 1) / kHeadTile, b);
-        k_head_fwd_tiled<<<grid, kT, 0, s>>>(static_cast<const bf16*>(x), Geo{b, h, w, cin}, wgt, bias, y,
-                                             HeadOut{nullptr, nullptr, 0.0, nullptr});
+        const HeadOut none{nullptr, nullptr, 0.0, nullptr};
+        if (C == 1)
+            k_head_fwd_tiled<1><<<grid, kT, 0, s>>>(static_cast<const bf16*>(x), Geo{b, h, w, cin}, wgt, bias, yc, none);
+        else
+            k_head_fwd_tiled<5><<<grid, kT, 0, s>>>(static_cast<const bf16*>(x), Geo{b, h, w, cin}, wgt, bias, yc, none);
         return hipGetLastError();
     }
-    k_head_fwd<<<ew_grid(static_cast<size_t>(b) * h * w), kT, 0, s>>>(static_cast<const bf16*>(x), Geo{b, h, w, cin}, wgt,
-                                                                       bias, y);
+    k_head_fwd<<<ew_grid(static_cast<size_t>(b) * h * w), kT, 0, s>>>(static_cast<const bf16*>(x), Geo{b, h, w, cin}, C,
+                                                                       wgt, bias, yc);
     return hipGetLastError();
 }
 
-hipError_t darcy_nn_head_bwd_data(const float* dy, int b, int h, int w, int cin, const float* wgt, void* dx,
+hipError_t darcy_nn_head_bwd_data(const float* dyc, int b, int h, int w, int cin, int C, const float* wgt, void* dx,
                                   hipStream_t s) {
-    k_head_bwd_data<<<ew_grid(static_cast<size_t>(b) * h * w * cin), kT, 0, s>>>(dy, Geo{b, h, w, cin}, wgt,
+    k_head_bwd_data<<<ew_grid(static_cast<size_t>(b) * h * w * cin), kT, 0, s>>>(dyc, Geo{b, h, w, cin}, C, wgt,
                                                                                   static_cast<bf16*>(dx));
     return hipGetLastError();
 }
 
-// part: groups * (9 cin + 1) floats with groups = ceil(b h w / per); summed
-// in group order into dw [9 cin] and db [1].
-hipError_t darcy_nn_head_bwd_weight(const void* x, int b, int h, int w, int cin, const float* dy, int per, float* part,
-                                    float* dw, float* db, hipStream_t s) {
+// part: groups * C * (9 cin + 1) floats with groups = ceil(b h w / per);
+// summed in group order into dw [C][9 cin] and db [C].
+hipError_t darcy_nn_head_bwd_weight(const void* x, int b, int h, int w, int cin, int C, const float* dyc, int per,
+                                    float* part, float* dw, float* db, hipStream_t s) {
     const size_t P = static_cast<size_t>(b) * h * w;
     const int groups = static_cast<int>((P + per - 1) / per);
-    k_head_dw_part<<<groups, kT, 0, s>>>(static_cast<const bf16*>(x), Geo{b, h, w, cin}, dy, per, part);
+    k_head_dw_part<<<groups, kT, 0, s>>>(static_cast<const bf16*>(x), Geo{b, h, w, cin}, C, dyc, per, part);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    k_reduce_splits<<<ew_grid(9 * cin + 1), kT, 0, s>>>(part, groups, 1, 9 * cin, dw, db);
+    k_reduce_splits<<<ew_grid(static_cast<size_t>(C) * (9 * cin + 1)), kT, 0, s>>>(part, groups, C, 9 * cin, dw, db);
+    return hipGetLastError();
+}
+
+// Half-resolution output map: y [b][n][n] from yc [b][n/2][n/2][5].
+hipError_t darcy_nn_sub2_out(const float* yc, int n, int batch, float* y, hipStream_t s) {
+    k_sub2_out<<<ew_grid(static_cast<size_t>(n) * n * batch), kT, 0, s>>>(yc, n, batch, y,
+                                                                         HeadOut{nullptr, nullptr, 0.0, nullptr});
+    return hipGetLastError();
+}
+
+// Its adjoint: dyc [b][n/2][n/2][5] from dy [b][n][n].
+hipError_t darcy_nn_sub2_adjoint(const float* dy, int n, int batch, float* dyc, hipStream_t s) {
+    k_sub2_adjoint<<<ew_grid(static_cast<size_t>(n / 2) * (n / 2) * batch), kT, 0, s>>>(dy, n, batch, dyc);
     return hipGetLastError();
 }
 
@@ -1204,8 +1324,11 @@ hipError_t darcy_nn_position(int n, float* pos, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t darcy_nn_features(const double* const* a, const float* pos, int n, int batch, void* x, hipStream_t s) {
-    k_features<<<ew_grid(static_cast<size_t>(n) * n * batch), kT, 0, s>>>(a, pos, n, batch, static_cast<bf16*>(x));
+// x: [b][n/sub][n/sub][8].
+hipError_t darcy_nn_features(const double* const* a, const float* pos, int n, int sub, int batch, void* x,
+                             hipStream_t s) {
+    const size_t px = static_cast<size_t>(n / sub) * (n / sub) * batch;
+    k_features<<<ew_grid(px), kT, 0, s>>>(a, pos, n, sub, batch, static_cast<bf16*>(x));
     return hipGetLastError();
 }
 
@@ -1247,19 +1370,29 @@ hipError_t darcy_nn_output(const float* y, const float* pos, int n, int batch, c
     return hipGetLastError();
 }
 
-// The head and the output in one: u0 = f scale pos head(x) for n x n
-// problems (the 32-channel head stores u0 itself; other widths go through y).
-hipError_t darcy_nn_head_output(const void* x, int b, int n, int cin, const float* wgt, const float* bias, float* y,
-                                const float* pos, const double* f, double scale, double* const* u0, hipStream_t s) {
-    if (cin == kHeadC) {
-        const dim3 grid((n + kHeadTile - 1) / kHeadTile, (n + kHeadTile - 1) / kHeadTile, b);
-        k_head_fwd_tiled<<<grid, kT, 0, s>>>(static_cast<const bf16*>(x), Geo{b, n, n, cin}, wgt, bias, y,
-                                             HeadOut{pos, f, scale, u0});
+// The head and the output in one: u0 = f scale pos y for n x n problems, x
+// the finest network level (side n / sub).  sub 1: the 32-channel head
+// stores u0 itself (other widths go through yc); sub 2: the 5-channel head
+// writes yc, then the output map forms u0.
+hipError_t darcy_nn_head_output(const void* x, int b, int n, int cin, int sub, const float* wgt, const float* bias,
+                                float* yc, const float* pos, const double* f, double scale, double* const* u0,
+                                hipStream_t s) {
+    if (sub != 1 && sub != 2) return hipErrorInvalidValue;
+    if (sub == 2) {
+        hipError_t e = darcy_nn_head_fwd(x, b, n / 2, n / 2, cin, 5, wgt, bias, yc, s);
+        if (e != hipSuccess) return e;
+        k_sub2_out<<<ew_grid(static_cast<size_t>(n) * n * b), kT, 0, s>>>(yc, n, b, nullptr, HeadOut{pos, f, scale, u0});
         return hipGetLastError();
     }
-    const hipError_t e = darcy_nn_head_fwd(x, b, n, n, cin, wgt, bias, y, s);
+    if (cin == kHeadC) {
+        const dim3 grid((n + kHeadTile - 1) / kHeadTile, (n + kHeadTile - 1) / kHeadTile, b);
+        k_head_fwd_tiled<1><<<grid, kT, 0, s>>>(static_cast<const bf16*>(x), Geo{b, n, n, cin}, wgt, bias, yc,
+                                                HeadOut{pos, f, scale, u0});
+        return hipGetLastError();
+    }
+    const hipError_t e = darcy_nn_head_fwd(x, b, n, n, cin, 1, wgt, bias, yc, s);
     if (e != hipSuccess) return e;
-    return darcy_nn_output(y, pos, n, b, f, scale, u0, s);
+    return darcy_nn_output(yc, pos, n, b, f, scale, u0, s);
 }
 
 // partial: `blocks` floats.

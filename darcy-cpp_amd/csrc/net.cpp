@@ -25,14 +25,17 @@ hipError_t darcy_nn_conv_bwd_weight(const void* x, int b, int h, int w, int cin,
                                     int splits, float* part, float* dw, float* db, hipStream_t s);
 hipError_t darcy_nn_conv_fwd_merge(const void* x, int b, int h, int w, int cin, const void* wgt, const float* bias,
                                    int cout, const void* skip, void* z, void* act, hipStream_t s);
-hipError_t darcy_nn_head_fwd(const void* x, int b, int h, int w, int cin, const float* wgt, const float* bias, float* y,
-                             hipStream_t s);
-hipError_t darcy_nn_head_bwd_data(const float* dy, int b, int h, int w, int cin, const float* wgt, void* dx,
+hipError_t darcy_nn_head_fwd(const void* x, int b, int h, int w, int cin, int C, const float* wgt, const float* bias,
+                             float* yc, hipStream_t s);
+hipError_t darcy_nn_head_bwd_data(const float* dyc, int b, int h, int w, int cin, int C, const float* wgt, void* dx,
                                   hipStream_t s);
-hipError_t darcy_nn_head_bwd_weight(const void* x, int b, int h, int w, int cin, const float* dy, int per, float* part,
-                                    float* dw, float* db, hipStream_t s);
+hipError_t darcy_nn_head_bwd_weight(const void* x, int b, int h, int w, int cin, int C, const float* dyc, int per,
+                                    float* part, float* dw, float* db, hipStream_t s);
+hipError_t darcy_nn_sub2_out(const float* yc, int n, int batch, float* y, hipStream_t s);
+hipError_t darcy_nn_sub2_adjoint(const float* dy, int n, int batch, float* dyc, hipStream_t s);
 hipError_t darcy_nn_position(int n, float* pos, hipStream_t s);
-hipError_t darcy_nn_features(const double* const* a, const float* pos, int n, int batch, void* x, hipStream_t s);
+hipError_t darcy_nn_features(const double* const* a, const float* pos, int n, int sub, int batch, void* x,
+                             hipStream_t s);
 hipError_t darcy_nn_up_add_gelu(const void* c, int b, int h, int w, int ch, const void* skip, void* z, void* act,
                                 hipStream_t s);
 hipError_t darcy_nn_gelu_bwd(void* d, const void* z, size_t count, hipStream_t s);
@@ -41,8 +44,9 @@ hipError_t darcy_nn_add(void* dst, const void* src, size_t count, hipStream_t s)
 hipError_t darcy_nn_transpose(const void* in, int P, int C, void* out, hipStream_t s);
 hipError_t darcy_nn_output(const float* y, const float* pos, int n, int batch, const double* f, double scale,
                            double* const* u0, hipStream_t s);
-hipError_t darcy_nn_head_output(const void* x, int b, int n, int cin, const float* wgt, const float* bias, float* y,
-                                const float* pos, const double* f, double scale, double* const* u0, hipStream_t s);
+hipError_t darcy_nn_head_output(const void* x, int b, int n, int cin, int sub, const float* wgt, const float* bias,
+                                float* yc, const float* pos, const double* f, double scale, double* const* u0,
+                                hipStream_t s);
 hipError_t darcy_nn_loss(const float* y, const double* const* u, const double* const* a, const float* pos, int n,
                          int batch, double scale, float lam, float* dy, float* partial, int blocks, hipStream_t s);
 hipError_t darcy_nn_adam(float* w, const float* g, float* m, float* v, void* wb, size_t count, float lr, float b1,
@@ -53,9 +57,14 @@ hipError_t darcy_nn_weight_t(const void* w, int cout, int cin, void* wt, hipStre
 
 namespace {
 
-// Levels of the U-Net: level l has side n / 2^l and kChan[l] channels.  The
-// encoder conv D_l produces level l (stride 2 for l > 0, from level l - 1;
-// D_0 reads the 8-channel features).  The decoder conv U_i maps level i + 1
+// Levels of the U-Net: level l has side m / 2^l and kChan[l] channels, m =
+// n / sub the side of the finest network level (sub 1: the grid itself; sub
+// 2: half resolution, each network pixel standing for a 2 x 2 block of
+// cells, whose four ln a values are its input features; the head then has
+// 5 channels: one interpolated bilinearly to the cells, four adding each
+// cell's own detail).  The encoder conv D_l produces
+// level l (stride 2 for l > 0, from level l - 1; D_0 reads the 8-channel
+// features).  The decoder conv U_i maps level i + 1
 // (kChan[i+1] channels) to kChan[i] channels at that size; its output is
 // upsampled x2 (nearest), added to the encoder activation of level i and
 // passed through GELU.  Levels 0..3 (32-64-128-256) are the encoder-decoder
@@ -80,12 +89,16 @@ constexpr ConvSpec spec(int c) {
                        : ConvSpec{kChan[2 * kLevels - 1 - c], kChan[2 * kLevels - 2 - c], 1, 2 * kLevels - 1 - c};
 }
 
+// Output channels of the head: 1, or 5 at half resolution.
+constexpr int head_channels(int sub) { return sub == 1 ? 1 : 5; }
+
+// Parameter offsets.
 struct Layout {
     size_t w[kConvs + 1], b[kConvs + 1], total;
-    Layout() {
+    explicit Layout(int sub) {
         size_t o = 0;
         for (int c = 0; c <= kConvs; ++c) {
-            const int cin = c < kConvs ? spec(c).cin : kHeadCin, cout = c < kConvs ? spec(c).cout : 1;
+            const int cin = c < kConvs ? spec(c).cin : kHeadCin, cout = c < kConvs ? spec(c).cout : head_channels(sub);
             w[c] = o;
             o += static_cast<size_t>(cout) * 9 * cin;
             b[c] = o;
@@ -94,7 +107,10 @@ struct Layout {
         total = o;
     }
 };
-const Layout kLayout;
+const Layout kLayouts[2] = {Layout(1), Layout(2)};
+
+// Sub-pixel factor of the creation flags (DARCY_NET_HALF_RES: 2, else 1).
+int sub_of(int32_t flags) { return (flags & DARCY_NET_HALF_RES) ? 2 : 1; }
 
 int fail(darcy_ctx* c, int code, const std::string& msg) {
     if (c) c->error = msg;
@@ -124,20 +140,26 @@ int splits_for(int cout, int cin, size_t P) {
 
 struct darcy_net {
     darcy_ctx* ctx = nullptr;
-    int n = 0, max_batch = 0;
+    int n = 0, max_batch = 0, sub = 1;
     bool train = false;
+    const Layout* lay = &kLayouts[0];
     long step = 0;
-    DevBuf params, wb, wt, grad, m, v, pos, feat, y;
+    DevBuf params, wb, wt, grad, m, v, pos, feat, y, yc;
     DevBuf act[kLevels], z[kLevels];               // encoder outputs (after / before GELU)
     DevBuf zm[kLevels - 1], am[kLevels - 1];       // merges at level i (before / after GELU)
     DevBuf dA[kLevels], dM[kLevels - 1], dC[kLevels - 1];
-    DevBuf dy, tmp, wpart, hpart, lossp, tab_a, tab_u, tab_f;
+    DevBuf dy, dyc, tmp, wpart, hpart, lossp, tab_a, tab_u, tab_f;
 
-    int side(int level) const { return n >> level; }
+    int side(int level) const { return (n / sub) >> level; }
     size_t px(int level, int b) const { return static_cast<size_t>(b) * side(level) * side(level); }
-    const uint16_t* wbf(int c) const { return wb.as<uint16_t>() + kLayout.w[c]; }
-    const uint16_t* wtr(int c) const { return wt.as<uint16_t>() + kLayout.w[c]; }
-    const float* bias(int c) const { return params.as<float>() + kLayout.b[c]; }
+    size_t cells(int b) const { return static_cast<size_t>(b) * n * n; }  // the output y
+    int hc() const { return head_channels(sub); }
+    // The head conv's output (and its gradient): y itself at full resolution.
+    float* head_out() const { return (sub == 1 ? y : yc).as<float>(); }
+    float* head_grad() const { return (sub == 1 ? dy : dyc).as<float>(); }
+    const uint16_t* wbf(int c) const { return wb.as<uint16_t>() + lay->w[c]; }
+    const uint16_t* wtr(int c) const { return wt.as<uint16_t>() + lay->w[c]; }
+    const float* bias(int c) const { return params.as<float>() + lay->b[c]; }
 
     int alloc();
     int refresh_weights();
@@ -150,7 +172,7 @@ struct darcy_net {
 
 int darcy_net::alloc() {
     const int B = max_batch;
-    const size_t P = kLayout.total;
+    const size_t P = lay->total;
     auto bf = [](DevBuf& buf, size_t elems) { return buf.reserve(elems * 2); };
     NN_TRY(params.reserve(P * 4), "net parameters");
     NN_TRY(wb.reserve(P * 2), "net parameters");
@@ -159,7 +181,8 @@ int darcy_net::alloc() {
     NN_TRY(pos.reserve(static_cast<size_t>(n) * n * 4), "net position");
     NN_TRY(darcy_nn_position(n, pos.as<float>(), ctx->stream), "net position");
     NN_TRY(bf(feat, px(0, B) * kInC), "net activations");
-    NN_TRY(y.reserve(px(0, B) * 4), "net activations");
+    NN_TRY(y.reserve(cells(B) * 4), "net activations");
+    if (sub != 1) NN_TRY(yc.reserve(px(0, B) * hc() * 4), "net activations");
     for (int l = 0; l < kLevels; ++l) {
         NN_TRY(bf(act[l], px(l, B) * kChan[l]), "net activations");
         if (train) NN_TRY(bf(z[l], px(l, B) * kChan[l]), "net activations");
@@ -174,7 +197,8 @@ int darcy_net::alloc() {
         NN_TRY(v.reserve(P * 4), "optimizer state");
         NN_TRY(hipMemsetAsync(m.ptr, 0, P * 4, ctx->stream), "optimizer state");
         NN_TRY(hipMemsetAsync(v.ptr, 0, P * 4, ctx->stream), "optimizer state");
-        NN_TRY(dy.reserve(px(0, B) * 4), "net gradients");
+        NN_TRY(dy.reserve(cells(B) * 4), "net gradients");
+        if (sub != 1) NN_TRY(dyc.reserve(px(0, B) * hc() * 4), "net gradients");
         for (int l = 0; l < kLevels; ++l) NN_TRY(bf(dA[l], px(l, B) * kChan[l]), "net gradients");
         for (int i = 0; i + 1 < kLevels; ++i) {
             NN_TRY(bf(dM[i], px(i, B) * kChan[i]), "net gradients");
@@ -189,15 +213,16 @@ int darcy_net::alloc() {
         }
         NN_TRY(bf(tmp, tmax), "net gradients");
         NN_TRY(wpart.reserve(part * 4), "net gradients");
-        NN_TRY(hpart.reserve(((px(0, B) + kHeadPer - 1) / kHeadPer) * (9 * kHeadCin + 1) * 4), "net gradients");
+        NN_TRY(hpart.reserve(((px(0, B) + kHeadPer - 1) / kHeadPer) * hc() * (9 * kHeadCin + 1) * 4),
+               "net gradients");
         NN_TRY(lossp.reserve(kLossBlocks * 4), "net gradients");
     }
     return refresh_weights();
 }
 
 void darcy_net::release() {
-    DevBuf* all[] = {&params, &wb, &wt, &grad, &m, &v, &pos, &feat, &y, &dy, &tmp, &wpart, &hpart, &lossp,
-                     &tab_a, &tab_u, &tab_f};
+    DevBuf* all[] = {&params, &wb, &wt, &grad, &m, &v, &pos, &feat, &y, &yc, &dy, &dyc, &tmp, &wpart, &hpart,
+                     &lossp, &tab_a, &tab_u, &tab_f};
     for (DevBuf* b : all) b->release();
     for (int l = 0; l < kLevels; ++l) {
         act[l].release();
@@ -214,10 +239,10 @@ void darcy_net::release() {
 
 // bf16 copies of the weights and the flipped transposes used by backward data.
 int darcy_net::refresh_weights() {
-    NN_TRY(darcy_nn_to_bf16(params.as<float>(), wb.ptr, kLayout.total, ctx->stream), "net weights");
+    NN_TRY(darcy_nn_to_bf16(params.as<float>(), wb.ptr, lay->total, ctx->stream), "net weights");
     for (int c = 1; c < kConvs; ++c)
-        NN_TRY(darcy_nn_weight_t(wb.as<uint16_t>() + kLayout.w[c], spec(c).cout, spec(c).cin,
-                                 wt.as<uint16_t>() + kLayout.w[c], ctx->stream),
+        NN_TRY(darcy_nn_weight_t(wb.as<uint16_t>() + lay->w[c], spec(c).cout, spec(c).cin,
+                                 wt.as<uint16_t>() + lay->w[c], ctx->stream),
                "net weights");
     return DARCY_OK;
 }
@@ -243,7 +268,8 @@ int darcy_net::upload_tables(const void* const* a, const void* const* u, const d
 // head's output y, or with `to_u0` the warm starts u0 themselves.
 int darcy_net::forward(int b, int64_t offset, bool to_u0) {
     hipStream_t s = ctx->stream;
-    NN_TRY(darcy_nn_features(tab_a.as<const double*>() + offset, pos.as<float>(), n, b, feat.ptr, s), "net features");
+    NN_TRY(darcy_nn_features(tab_a.as<const double*>() + offset, pos.as<float>(), n, sub, b, feat.ptr, s),
+           "net features");
     const void* x = feat.ptr;
     for (int l = 0; l < kLevels; ++l) {
         const ConvSpec c = spec(l);
@@ -261,14 +287,15 @@ int darcy_net::forward(int b, int64_t offset, bool to_u0) {
         x = am[i].ptr;
     }
     if (to_u0) {
-        NN_TRY(darcy_nn_head_output(x, b, n, kHeadCin, params.as<float>() + kLayout.w[kConvs],
-                                    params.as<float>() + kLayout.b[kConvs], y.as<float>(), pos.as<float>(),
+        NN_TRY(darcy_nn_head_output(x, b, n, kHeadCin, sub, params.as<float>() + lay->w[kConvs],
+                                    params.as<float>() + lay->b[kConvs], head_out(), pos.as<float>(),
                                     tab_f.as<double>() + offset, kScale, tab_u.as<double*>() + offset, s),
                "net head");
     } else {
-        NN_TRY(darcy_nn_head_fwd(x, b, n, n, kHeadCin, params.as<float>() + kLayout.w[kConvs],
-                                 params.as<float>() + kLayout.b[kConvs], y.as<float>(), s),
+        NN_TRY(darcy_nn_head_fwd(x, b, side(0), side(0), kHeadCin, hc(), params.as<float>() + lay->w[kConvs],
+                                 params.as<float>() + lay->b[kConvs], head_out(), s),
                "net head");
+        if (sub != 1) NN_TRY(darcy_nn_sub2_out(yc.as<float>(), n, b, y.as<float>(), s), "net head");
     }
     return DARCY_OK;
 }
@@ -281,7 +308,7 @@ int darcy_net::wgrad(int c, const void* x, const void* dz, int b) {
     NN_TRY(darcy_nn_transpose(dz, static_cast<int>(P), sp.cout, tmp.ptr, ctx->stream), "net backward");
     const int splits = splits_for(sp.cout, sp.cin, P);
     NN_TRY(darcy_nn_conv_bwd_weight(x, b, h, h, sp.cin, tmp.ptr, sp.cout, sp.stride, splits, wpart.as<float>(),
-                                    grad.as<float>() + kLayout.w[c], grad.as<float>() + kLayout.b[c], ctx->stream),
+                                    grad.as<float>() + lay->w[c], grad.as<float>() + lay->b[c], ctx->stream),
            "net backward");
     return DARCY_OK;
 }
@@ -293,10 +320,12 @@ int darcy_net::backward(int b, int64_t offset, double* loss) {
                          pos.as<float>(), n, b, kScale, kEnergyWeight, dy.as<float>(), lossp.as<float>(), kLossBlocks,
                          s),
            "net loss");
-    const float* wh = params.as<float>() + kLayout.w[kConvs];
-    NN_TRY(darcy_nn_head_bwd_data(dy.as<float>(), b, n, n, kHeadCin, wh, dM[0].ptr, s), "net backward");
-    NN_TRY(darcy_nn_head_bwd_weight(am[0].ptr, b, n, n, kHeadCin, dy.as<float>(), kHeadPer, hpart.as<float>(),
-                                    grad.as<float>() + kLayout.w[kConvs], grad.as<float>() + kLayout.b[kConvs], s),
+    const float* wh = params.as<float>() + lay->w[kConvs];
+    const int m = side(0);
+    if (sub != 1) NN_TRY(darcy_nn_sub2_adjoint(dy.as<float>(), n, b, dyc.as<float>(), s), "net backward");
+    NN_TRY(darcy_nn_head_bwd_data(head_grad(), b, m, m, kHeadCin, hc(), wh, dM[0].ptr, s), "net backward");
+    NN_TRY(darcy_nn_head_bwd_weight(am[0].ptr, b, m, m, kHeadCin, hc(), head_grad(), kHeadPer, hpart.as<float>(),
+                                    grad.as<float>() + lay->w[kConvs], grad.as<float>() + lay->b[kConvs], s),
            "net backward");
     // Decoder, finest merge first: dM[i] holds the gradient of merge i's
     // activation; after the GELU step it is the gradient of the merge's sum,
@@ -345,7 +374,9 @@ hipError_t set_device(darcy_ctx* ctx) { return hipSetDevice(ctx->device); }
 
 }  // namespace
 
-extern "C" int64_t darcy_net_param_count(void) { return static_cast<int64_t>(kLayout.total); }
+extern "C" int64_t darcy_net_param_count(int32_t flags) {
+    return static_cast<int64_t>(kLayouts[sub_of(flags) - 1].total);
+}
 
 extern "C" void darcy_net_destroy(darcy_net* net) {
     if (!net) return;
@@ -358,13 +389,17 @@ extern "C" void darcy_net_destroy(darcy_net* net) {
     delete net;
 }
 
-extern "C" int darcy_net_create(darcy_ctx* ctx, int64_t n, int64_t max_batch, int32_t train, darcy_net** out) {
+extern "C" int darcy_net_create(darcy_ctx* ctx, int64_t n, int64_t max_batch, int32_t flags, darcy_net** out) {
     if (!ctx || !out) return DARCY_INVALID_INPUT;
     *out = nullptr;
     std::lock_guard<std::mutex> lock(ctx->mu);
-    const int64_t unit = int64_t(1) << (kLevels - 1);
+    if (flags & ~(DARCY_NET_TRAIN | DARCY_NET_HALF_RES)) return fail(ctx, DARCY_INVALID_INPUT, "unknown network flags");
+    const int sub = sub_of(flags);
+    const int64_t unit = int64_t(sub) << (kLevels - 1);
     if (n < unit || n % unit != 0 || n > 4096)
-        return fail(ctx, DARCY_INVALID_INPUT, "the warm-start network needs n a multiple of 64 in [64, 4096]");
+        return fail(ctx, DARCY_INVALID_INPUT,
+                    sub == 1 ? "the warm-start network needs n a multiple of 64 in [64, 4096]"
+                             : "the half-resolution warm-start network needs n a multiple of 128 in [128, 4096]");
     if (max_batch < 1 || static_cast<double>(max_batch) * n * n > 1.34e8)
         return fail(ctx, DARCY_INVALID_INPUT, "max_batch must be at least 1 and max_batch * n * n at most 1.34e8");
     NN_TRY(set_device(ctx), "hipSetDevice");
@@ -372,7 +407,9 @@ extern "C" int darcy_net_create(darcy_ctx* ctx, int64_t n, int64_t max_batch, in
     net->ctx = ctx;
     net->n = static_cast<int>(n);
     net->max_batch = static_cast<int>(max_batch);
-    net->train = train != 0;
+    net->train = (flags & DARCY_NET_TRAIN) != 0;
+    net->sub = sub;
+    net->lay = &kLayouts[sub - 1];
     const int rc = net->alloc();
     if (rc != DARCY_OK) {
         (void)hipStreamSynchronize(ctx->stream);
@@ -388,14 +425,14 @@ extern "C" int darcy_net_set_params(darcy_net* net, const float* params, int64_t
     if (check_net(net) || !params) return DARCY_INVALID_INPUT;
     darcy_ctx* ctx = net->ctx;
     std::lock_guard<std::mutex> lock(ctx->mu);
-    if (count != static_cast<int64_t>(kLayout.total))
+    if (count != static_cast<int64_t>(net->lay->total))
         return fail(ctx, DARCY_INVALID_INPUT, "parameter vector has the wrong length");
     NN_TRY(set_device(ctx), "hipSetDevice");
-    NN_TRY(hipMemcpyAsync(net->params.ptr, params, kLayout.total * 4, hipMemcpyHostToDevice, ctx->stream),
+    NN_TRY(hipMemcpyAsync(net->params.ptr, params, net->lay->total * 4, hipMemcpyHostToDevice, ctx->stream),
            "net parameters");
     if (net->train) {
-        NN_TRY(hipMemsetAsync(net->m.ptr, 0, kLayout.total * 4, ctx->stream), "optimizer state");
-        NN_TRY(hipMemsetAsync(net->v.ptr, 0, kLayout.total * 4, ctx->stream), "optimizer state");
+        NN_TRY(hipMemsetAsync(net->m.ptr, 0, net->lay->total * 4, ctx->stream), "optimizer state");
+        NN_TRY(hipMemsetAsync(net->v.ptr, 0, net->lay->total * 4, ctx->stream), "optimizer state");
         net->step = 0;
     }
     if (int r = net->refresh_weights()) return r;
@@ -407,10 +444,10 @@ extern "C" int darcy_net_get_params(darcy_net* net, float* params, int64_t count
     if (check_net(net) || !params) return DARCY_INVALID_INPUT;
     darcy_ctx* ctx = net->ctx;
     std::lock_guard<std::mutex> lock(ctx->mu);
-    if (count != static_cast<int64_t>(kLayout.total))
+    if (count != static_cast<int64_t>(net->lay->total))
         return fail(ctx, DARCY_INVALID_INPUT, "parameter vector has the wrong length");
     NN_TRY(set_device(ctx), "hipSetDevice");
-    NN_TRY(hipMemcpyAsync(params, net->params.ptr, kLayout.total * 4, hipMemcpyDeviceToHost, ctx->stream),
+    NN_TRY(hipMemcpyAsync(params, net->params.ptr, net->lay->total * 4, hipMemcpyDeviceToHost, ctx->stream),
            "net parameters");
     NN_TRY(hipStreamSynchronize(ctx->stream), "net parameters");
     return DARCY_OK;
@@ -459,7 +496,7 @@ int train_or_grad(darcy_net* net, int64_t batch, const double* const* a, const d
     if (int r = net->forward(b, 0)) return r;
     if (int r = net->backward(b, 0, loss)) return r;
     if (grad_out) {
-        NN_TRY(hipMemcpyAsync(grad_out, net->grad.ptr, kLayout.total * 4, hipMemcpyDeviceToHost, ctx->stream),
+        NN_TRY(hipMemcpyAsync(grad_out, net->grad.ptr, net->lay->total * 4, hipMemcpyDeviceToHost, ctx->stream),
                "net gradient");
         NN_TRY(hipStreamSynchronize(ctx->stream), "net gradient");
     }
@@ -469,11 +506,11 @@ int train_or_grad(darcy_net* net, int64_t batch, const double* const* a, const d
         const float c1 = 1.0f - std::pow(b1, static_cast<float>(net->step));
         const float c2 = 1.0f - std::pow(b2, static_cast<float>(net->step));
         NN_TRY(darcy_nn_adam(net->params.as<float>(), net->grad.as<float>(), net->m.as<float>(), net->v.as<float>(),
-                             net->wb.ptr, kLayout.total, *lr, b1, b2, 1e-8f, c1, c2, ctx->stream),
+                             net->wb.ptr, net->lay->total, *lr, b1, b2, 1e-8f, c1, c2, ctx->stream),
                "net update");
         for (int c = 1; c < kConvs; ++c)
-            NN_TRY(darcy_nn_weight_t(net->wb.as<uint16_t>() + kLayout.w[c], spec(c).cout, spec(c).cin,
-                                     net->wt.as<uint16_t>() + kLayout.w[c], ctx->stream),
+            NN_TRY(darcy_nn_weight_t(net->wb.as<uint16_t>() + net->lay->w[c], spec(c).cout, spec(c).cin,
+                                     net->wt.as<uint16_t>() + net->lay->w[c], ctx->stream),
                    "net update");
     }
     return DARCY_OK;

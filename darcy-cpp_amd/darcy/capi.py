@@ -79,7 +79,7 @@ def _load() -> C.CDLL:
     lib.darcy_ctx_get_tuning.argtypes = [vp, C.c_char_p, C.POINTER(i32)]
     # Learned warm start.
     fp, u16p = C.POINTER(C.c_float), C.POINTER(C.c_uint16)
-    lib.darcy_net_param_count.argtypes = []
+    lib.darcy_net_param_count.argtypes = [i32]
     lib.darcy_net_param_count.restype = i64
     lib.darcy_net_create.argtypes = [vp, i64, i64, i32, C.POINTER(vp)]
     lib.darcy_net_destroy.argtypes = [vp]
@@ -354,19 +354,30 @@ class Context:
         return c
 
 
-def net_param_count() -> int:
-    return int(_load().darcy_net_param_count())
+NET_TRAIN, NET_HALF_RES = 1, 2  # darcy_net_create flags
+
+
+def _net_flags(train: bool, sub: int) -> int:
+    if sub not in (1, 2):
+        raise ValueError("the network's sub-pixel factor must be 1 or 2")
+    return (NET_TRAIN if train else 0) | (NET_HALF_RES if sub == 2 else 0)
+
+
+def net_param_count(sub: int = 1) -> int:
+    return int(_load().darcy_net_param_count(_net_flags(False, sub)))
 
 
 class Net:
-    """A darcy_net: the warm-start network on one context."""
+    """A darcy_net: the warm-start network on one context; `sub` = 2 runs it
+    at half the grid's resolution (2 x 2 blocks of cells per network pixel)."""
 
-    def __init__(self, ctx: Context, n: int, max_batch: int, train: bool = False):
+    def __init__(self, ctx: Context, n: int, max_batch: int, train: bool = False, sub: int = 1):
         self.ctx, self.lib = ctx, ctx.lib
-        self.n, self.max_batch, self.train = n, max_batch, train
-        self.count = int(self.lib.darcy_net_param_count())
+        self.n, self.max_batch, self.train, self.sub = n, max_batch, train, sub
+        flags = _net_flags(train, sub)
+        self.count = int(self.lib.darcy_net_param_count(flags))
         self.ptr = C.c_void_p()
-        ctx._check(self.lib.darcy_net_create(ctx.ptr, n, max_batch, int(train), C.byref(self.ptr)))
+        ctx._check(self.lib.darcy_net_create(ctx.ptr, n, max_batch, flags, C.byref(self.ptr)))
 
     def close(self) -> None:
         if self.ptr:

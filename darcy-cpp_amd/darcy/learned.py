@@ -1,8 +1,11 @@
 """Learned warm start: a BF16 conv encoder-decoder that predicts u0.
 
 The network (C ABI ``darcy_net_*`` in include/darcy_hip.h) maps the
-permeability of an n x n problem (n a multiple of 64) to an initial guess
-u0; CG then runs to the requested tolerance from u0, so the solution is the
+permeability of an n x n problem to an initial guess u0.  With sub-pixel
+factor 1 it runs on the grid itself (n a multiple of 64); with factor 2 at
+half the grid's resolution (n a multiple of 128), each network pixel taking
+a 2 x 2 block's four ln a values in; its output is then a bilinearly
+interpolated channel plus a detail channel per cell of the block; CG then runs to the requested tolerance from u0, so the solution is the
 solver's, and a poor guess only costs iterations.  It has no counterpart in
 the reference, whose solve starts from u = 0 (solver.hpp:99).
 
@@ -15,8 +18,9 @@ held-out seeds 20000-20999.
     python -m darcy.learned eval  [--weights weights.npz]
 
 Weights are stored as a plain ``.npz``: the conv weights as the bf16 values
-the GPU computes with, biases and the head in FP32, plus training metadata;
-loaded with ``allow_pickle=False``.
+the GPU computes with, biases and the head in FP32, the sub-pixel factor
+(``sub``, 1 when absent) and training metadata; loaded with
+``allow_pickle=False``.
 """
 
 from __future__ import annotations
@@ -40,30 +44,60 @@ TRAIN_SEEDS = (0, 20000)
 EVAL_SEEDS = (20000, 21000)
 
 # Channels of the U-Net levels and (cin, cout) of its convolutions in
-# parameter order: D_0..D_6, U_5..U_0, the head.
+# parameter order: D_0..D_6, U_5..U_0, the head (1 output, or 5 at half
+# resolution: the interpolated channel and the four cells' detail).
 _CHAN = [32, 64, 128, 256, 256, 256, 256]
-_CONVS = ([(8 if l == 0 else _CHAN[l - 1], _CHAN[l]) for l in range(len(_CHAN))]
-          + [(_CHAN[i + 1], _CHAN[i]) for i in range(len(_CHAN) - 2, -1, -1)] + [(32, 1)])
 
 
-def param_count() -> int:
-    return sum(cout * 9 * cin + cout for cin, cout in _CONVS)
+def _convs(sub: int) -> list[tuple[int, int]]:
+    return ([(8 if l == 0 else _CHAN[l - 1], _CHAN[l]) for l in range(len(_CHAN))]
+            + [(_CHAN[i + 1], _CHAN[i]) for i in range(len(_CHAN) - 2, -1, -1)] + [(32, 1 if sub == 1 else 5)])
 
 
-def init_params(seed: int) -> np.ndarray:
+def param_count(sub: int = 1) -> int:
+    return sum(cout * 9 * cin + cout for cin, cout in _convs(sub))
+
+
+def network_flops(n: int, sub: int = 1) -> float:
+    """FLOPs of one n x n prediction: 2 * pixels * 9 * cin * cout summed over
+    the convolutions (the first conv's 8 staged input channels counted) and
+    the head."""
+    m = n // sub
+    total = 0.0
+    for l in range(len(_CHAN)):  # D_l produces level l
+        total += 2.0 * (m >> l) ** 2 * 9 * (8 if l == 0 else _CHAN[l - 1]) * _CHAN[l]
+    for i in range(len(_CHAN) - 2, -1, -1):  # U_i runs at level i + 1
+        total += 2.0 * (m >> (i + 1)) ** 2 * 9 * _CHAN[i + 1] * _CHAN[i]
+    return total + 2.0 * m * m * 9 * 32 * (1 if sub == 1 else 5)
+
+
+def sub_of_count(count: int) -> int:
+    """The sub-pixel factor whose network has `count` parameters."""
+    for sub in (1, 2):
+        if count == param_count(sub):
+            return sub
+    raise ValueError(f"{count} parameters fit no warm-start network")
+
+
+def init_params(seed: int, sub: int = 1) -> np.ndarray:
     """He-normal weights for GELU, zero biases; the first conv's unused input
-    channels (3..7) are zero; the head starts small."""
+    channels (sub^2 + 2 .. 7) are zero; the head starts small, its detail
+    channels at zero (a smooth first output)."""
     rng = np.random.default_rng(seed)
     out = []
-    for i, (cin, cout) in enumerate(_CONVS):
-        fan_in = 9 * (3 if i == 0 else cin)
-        std = math.sqrt(2.0 / fan_in) if i < len(_CONVS) - 1 else 0.1 * math.sqrt(1.0 / fan_in)
+    convs = _convs(sub)
+    used = sub * sub + 2
+    for i, (cin, cout) in enumerate(convs):
+        fan_in = 9 * (used if i == 0 else cin)
+        std = math.sqrt(2.0 / fan_in) if i < len(convs) - 1 else 0.1 * math.sqrt(1.0 / fan_in)
         w = rng.normal(0.0, std, size=(cout, 3, 3, cin)).astype(np.float32)
         if i == 0:
-            w[..., 3:] = 0.0
+            w[..., used:] = 0.0
+        if i == len(convs) - 1:
+            w[1:] = 0.0
         out += [w.ravel(), np.zeros(cout, dtype=np.float32)]
     p = np.concatenate(out)
-    assert p.size == param_count()
+    assert p.size == param_count(sub)
     return p
 
 
@@ -81,12 +115,13 @@ def media(n: int, seeds: Sequence[int], workers: int = 16) -> np.ndarray:
         return np.stack(pool.map(_medium, [(n, s) for s in seeds], chunksize=64))
 
 
-def _fp32_index() -> np.ndarray:
+def _fp32_index(sub: int = 1) -> np.ndarray:
     """Parameters the GPU uses in FP32: every bias and the head's weights."""
     idx, o = [], 0
-    for i, (cin, cout) in enumerate(_CONVS):
+    convs = _convs(sub)
+    for i, (cin, cout) in enumerate(convs):
         nw = cout * 9 * cin
-        if i == len(_CONVS) - 1:
+        if i == len(convs) - 1:
             idx.append(np.arange(o, o + nw))
         idx.append(np.arange(o + nw, o + nw + cout))
         o += nw + cout
@@ -99,13 +134,16 @@ def _bf16_bits(x: np.ndarray) -> np.ndarray:
 
 
 def save_params(path: str, params: np.ndarray, **meta) -> None:
-    """The conv weights as the bf16 the GPU computes with; biases and the head in FP32."""
-    idx = _fp32_index()
+    """The conv weights as the bf16 the GPU computes with; biases and the head
+    in FP32; the sub-pixel factor follows from the parameter count."""
+    sub = sub_of_count(np.asarray(params).size)
+    idx = _fp32_index(sub)
     np.savez(path, weights_bf16=_bf16_bits(params), fp32_index=idx,
-             fp32_values=np.asarray(params, dtype=np.float32)[idx], **meta)
+             fp32_values=np.asarray(params, dtype=np.float32)[idx], sub=np.int64(sub), **meta)
 
 
-def load_params(path: str = DEFAULT_WEIGHTS) -> np.ndarray:
+def load_weights(path: str = DEFAULT_WEIGHTS) -> tuple[np.ndarray, int]:
+    """(parameters, sub-pixel factor) of a weight file."""
     if not os.path.exists(path):
         raise FileNotFoundError(f"{path}: no trained warm-start weights (python -m darcy.learned train)")
     with np.load(path, allow_pickle=False) as z:
@@ -114,9 +152,14 @@ def load_params(path: str = DEFAULT_WEIGHTS) -> np.ndarray:
         else:
             p = (z["weights_bf16"].astype(np.uint32) << 16).view(np.float32).copy()
             p[z["fp32_index"]] = z["fp32_values"]
-    if p.size != param_count():
-        raise ValueError(f"{path}: {p.size} parameters, expected {param_count()}")
-    return p
+        sub = int(z["sub"]) if "sub" in z else 1
+    if p.size != param_count(sub):
+        raise ValueError(f"{path}: {p.size} parameters, expected {param_count(sub)}")
+    return p, sub
+
+
+def load_params(path: str = DEFAULT_WEIGHTS) -> np.ndarray:
+    return load_weights(path)[0]
 
 
 class WarmStart:
@@ -125,8 +168,12 @@ class WarmStart:
     def __init__(self, ctx: capi.Context, n: int = 256, max_batch: int = 1000, params: np.ndarray | None = None,
                  weights: str = DEFAULT_WEIGHTS):
         self.ctx, self.n = ctx, n
-        self.net = capi.Net(ctx, n, max_batch)
-        self.net.set_params(load_params(weights) if params is None else params)
+        if params is None:
+            params, self.sub = load_weights(weights)
+        else:
+            self.sub = sub_of_count(np.asarray(params).size)
+        self.net = capi.Net(ctx, n, max_batch, sub=self.sub)
+        self.net.set_params(params)
 
     def close(self) -> None:
         self.net.close()
@@ -180,10 +227,10 @@ class DeviceSet:
 
 
 def train(ctx: capi.Context, data: DeviceSet, epochs: int, batch: int = 32, lr: float = 1e-3, seed: int = 0,
-          log=print) -> tuple[np.ndarray, list]:
+          log=print, sub: int = 1) -> tuple[np.ndarray, list]:
     """Adam with linear warm-up and cosine decay; returns (params, history)."""
-    net = capi.Net(ctx, data.n, batch, train=True)
-    net.set_params(init_params(seed))
+    net = capi.Net(ctx, data.n, batch, train=True, sub=sub)
+    net.set_params(init_params(seed, sub))
     rng = np.random.default_rng(seed)
     steps_per_epoch = data.count // batch
     total, warm = epochs * steps_per_epoch, min(500, epochs * steps_per_epoch // 10)
@@ -249,10 +296,12 @@ def main(argv: Sequence[str] | None = None) -> int:
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--train-count", type=int, default=TRAIN_SEEDS[1] - TRAIN_SEEDS[0])
     ap.add_argument("--eval-count", type=int, default=EVAL_SEEDS[1] - EVAL_SEEDS[0])
-    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--epochs", type=int, default=60)
     ap.add_argument("--batch", type=int, default=32)
-    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--lr", type=float, default=2e-4)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--sub", type=int, choices=(1, 2), default=2,
+                    help="sub-pixel factor: 2 runs the network at half the grid's resolution")
     ap.add_argument("--workers", type=int, default=16)
     ap.add_argument("--out", default=DEFAULT_WEIGHTS)
     ap.add_argument("--weights", default=DEFAULT_WEIGHTS)
@@ -273,7 +322,7 @@ def main(argv: Sequence[str] | None = None) -> int:
             print(json.dumps({"pairs": data.count, "solve_seconds": round(time.perf_counter() - t1, 1),
                               "mean_iterations": float(data.iterations.mean())}), flush=True)
             params, history = train(ctx, data, args.epochs, args.batch, args.lr, args.seed,
-                                    log=lambda s: print(s, flush=True))
+                                    log=lambda s: print(s, flush=True), sub=args.sub)
             data.free()
         else:
             params, history = load_params(args.weights), []

@@ -223,29 +223,44 @@ int darcy_grf_fields(darcy_ctx* ctx, int64_t count, int64_t n, double alpha, dou
  * darcy_solve / darcy_solve_batch, and CG still runs to tol from it.
  *
  * The network maps (ln a, the forcing, a position channel) on an n x n grid
- * (n a multiple of 64) to u0 through a U-Net encoder-decoder with 3x3
- * kernels, GELU, bf16 weights and activations and FP32 accumulation.  Level
- * l has side n / 2^l and channels 32, 64, 128, 256 (levels 0-3, the
+ * to u0 through a U-Net encoder-decoder with 3x3 kernels, GELU, bf16
+ * weights and activations and FP32 accumulation.  Its finest level has side
+ * m = n / sub: sub = 1 runs it on the grid itself (n a multiple of 64);
+ * with DARCY_NET_HALF_RES, sub = 2 (n a multiple of 128), each network pixel
+ * stands for a 2 x 2 block of cells: its input features carry the block's
+ * four ln a values and its head gives a smooth channel, interpolated to the
+ * cells, and a detail channel per cell of the block, a quarter of the work.
+ * Level
+ * l has side m / 2^l and channels 32, 64, 128, 256 (levels 0-3, the
  * encoder-decoder proper), then 256, 256, 256 (levels 4-6, a context stack
  * that gives every cell a receptive field spanning the grid):
- *   enc: D_0: conv 8->32 at n; D_l: stride-2 conv from level l-1 to level l
+ *   features: 8 channels per pixel: the sub^2 cells' (ln a - mu) / sd, 1,
+ *        the mean position of the cells, zeros (mu, sd map ln 3, ln 12 to
+ *        -1, +1)
+ *   enc: D_0: conv 8->32 at m; D_l: stride-2 conv from level l-1 to level l
  *   dec: U_i (i = 5..0): conv at level i+1 to the channels of level i, x2
  *        nearest upsampling, + the encoder activation of level i, GELU
- *   head: conv 32->1 in FP32;  u0 = f * pos * y / 64,
- *        pos = sin(pi (i + 1/2) / n) sin(pi (j + 1/2) / n).
+ *   head: conv 32->1 in FP32 (sub 1: y itself), or 32->5 (sub 2): y at
+ *        cell (2I + di, 2J + dj) is channel 0 interpolated bilinearly to the
+ *        cell's centre (weights 3/4, 1/4 per direction, the neighbour index
+ *        clamped to the grid) plus channel 1 + 2 di + dj of pixel (I, J);
+ *        u0 = f * pos * y / 64, pos = sin(pi (i + 1/2) / n) sin(pi (j + 1/2) / n).
  * Parameters are one FP32 vector: for each conv in the order D_0..D_6,
  * U_5..U_0, head: W[cout][3][3][cin] then bias[cout] (D_0 has 8 input
- * channels, 3 of them used).
+ * channels, sub^2 + 2 of them used).
  * ------------------------------------------------------------------------ */
 typedef struct darcy_net darcy_net;
 
-/* Length of the parameter vector. */
-int64_t darcy_net_param_count(void);
+#define DARCY_NET_TRAIN 1    /* also allocate what training needs */
+#define DARCY_NET_HALF_RES 2 /* sub = 2: the network at half the grid's resolution */
+
+/* Length of the parameter vector of a network created with `flags`. */
+int64_t darcy_net_param_count(int32_t flags);
 
 /* A network for n x n problems, processing at most max_batch at a time
- * (max_batch * n * n at most 1.34e8); train != 0 also allocates what
- * training needs.  Parameters start at 0. */
-int darcy_net_create(darcy_ctx* ctx, int64_t n, int64_t max_batch, int32_t train, darcy_net** out);
+ * (max_batch * n * n at most 1.34e8); flags: DARCY_NET_TRAIN,
+ * DARCY_NET_HALF_RES.  Parameters start at 0. */
+int darcy_net_create(darcy_ctx* ctx, int64_t n, int64_t max_batch, int32_t flags, darcy_net** out);
 void darcy_net_destroy(darcy_net* net);
 
 /* Host parameter vector in / out; set also resets the optimizer state. */

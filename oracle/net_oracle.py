@@ -4,12 +4,18 @@ Test infrastructure: only tests, the smoke check and the benchmark's CPU leg
 use it, as the checker of the GPU network (libdarcy_hip: darcy_net_*,
 darcy_nn_*).  The reference has no learned component, so nothing here
 restates reference code; the network is the one the C header describes: a
-U-Net whose level l has side n / 2^l and CHAN[l] channels,
+U-Net whose level l has side m / 2^l and CHAN[l] channels, m = n / sub the
+side of its finest level (sub = 1: the grid; sub = 2: one network pixel per
+2 x 2 block of cells),
 
-    enc: D_0 conv 8->32 at n, D_l stride-2 conv from level l-1 to level l
+    features: the sub^2 cells' normalised ln a, 1, the cells' mean position
+    enc: D_0 conv 8->32 at m, D_l stride-2 conv from level l-1 to level l
     dec: U_i (i = L-2..0) conv at level i+1 to CHAN[i] channels, nearest x2
          upsampling, + the encoder activation of level i, GELU
-    head: conv 32->1;  u0 = f * pos * y / 64
+    head: conv 32->1 (sub = 1), or 32->5 (sub = 2): channel 0 interpolated
+          bilinearly to the cells (cell-centred, indices clamped to the
+          grid), plus channel 1+2di+dj at cell (2I+di, 2J+dj);
+          u0 = f * pos * y / 64
 
 and the training loss (sum e^2 + 1000 e^T A e) / cells, e = pos y - 64 u, A the
 finite-volume operator of the permeability.
@@ -45,10 +51,10 @@ def conv_specs() -> list[tuple[int, int, int, int]]:
     return out
 
 
-def layout() -> tuple[list[tuple[int, int, tuple[int, ...]]], int]:
+def layout(sub: int = 1) -> tuple[list[tuple[int, int, tuple[int, ...]]], int]:
     """[(w_offset, b_offset, w_shape)] for the convs and the head, and the total."""
     out, o = [], 0
-    for cin, cout, _, _ in conv_specs() + [(HEAD_CIN, 1, 1, 0)]:
+    for cin, cout, _, _ in conv_specs() + [(HEAD_CIN, 1 if sub == 1 else 5, 1, 0)]:
         shape = (cout, 3, 3, cin)
         w = o
         o += int(np.prod(shape))
@@ -57,8 +63,16 @@ def layout() -> tuple[list[tuple[int, int, tuple[int, ...]]], int]:
     return out, o
 
 
+def sub_of(params: np.ndarray) -> int:
+    """The sub-pixel factor whose network has params.size parameters."""
+    for sub in (1, 2):
+        if params.size == layout(sub)[1]:
+            return sub
+    raise ValueError(f"{params.size} parameters fit no network")
+
+
 def unpack(params: np.ndarray) -> list[tuple[np.ndarray, np.ndarray]]:
-    lay, total = layout()
+    lay, total = layout(sub_of(params))
     assert params.shape == (total,)
     res = []
     for w, b, shape in lay:
@@ -197,16 +211,62 @@ def down_sum(d: np.ndarray) -> np.ndarray:
     return d.reshape(b, h // 2, 2, w // 2, 2, c).sum(axis=(2, 4))
 
 
+def _bilinear_matrix(m: int) -> np.ndarray:
+    """(2m, m) cell-centred linear interpolation: fine index i = 2I + d takes
+    3/4 of I and 1/4 of its neighbour on side d (I - 1 or I + 1, clamped)."""
+    P = np.zeros((2 * m, m))
+    for i in range(2 * m):
+        I = i // 2
+        side = min(max(I + (1 if i % 2 else -1), 0), m - 1)
+        P[i, I] += 0.75
+        P[i, side] += 0.25
+    return P
+
+
+def sub2_out(yc: np.ndarray) -> np.ndarray:
+    """(B, m, m, 5) -> (B, 2m, 2m): bilinear channel 0 plus the detail channels."""
+    P = _bilinear_matrix(yc.shape[1])
+    return np.einsum("ia,bac,jc->bij", P, yc[..., 0], P) + pixel_shuffle(yc[..., 1:], 2)
+
+
+def sub2_adjoint(d: np.ndarray) -> np.ndarray:
+    """The adjoint of sub2_out: (B, 2m, 2m) -> (B, m, m, 5)."""
+    P = _bilinear_matrix(d.shape[1] // 2)
+    return np.concatenate([np.einsum("ia,bij,jc->bac", P, d, P)[..., None], pixel_unshuffle(d, 2)], axis=-1)
+
+
+def pixel_shuffle(y: np.ndarray, sub: int) -> np.ndarray:
+    """(B, m, m, sub^2) -> (B, sub m, sub m): channel di*sub+dj to cell (sub I+di, sub J+dj)."""
+    b, m, _, _ = y.shape
+    return y.reshape(b, m, m, sub, sub).transpose(0, 1, 3, 2, 4).reshape(b, sub * m, sub * m)
+
+
+def pixel_unshuffle(d: np.ndarray, sub: int) -> np.ndarray:
+    """The inverse (and adjoint) of pixel_shuffle."""
+    b, n, _ = d.shape
+    m = n // sub
+    return d.reshape(b, m, sub, m, sub).transpose(0, 1, 3, 2, 4).reshape(b, m, m, sub * sub)
+
+
 # ------------------------------------------------------------ the network
 
-def features(a: np.ndarray) -> np.ndarray:
-    """(B, n, n) permeability -> (B, n, n, 8) features before rounding."""
+def features(a: np.ndarray, sub: int = 1) -> np.ndarray:
+    """(B, n, n) permeability -> (B, n/sub, n/sub, 8) features before rounding:
+    the sub^2 cells' (ln a - mu) / sd (row-major, as float32), 1, and the
+    cells' mean position (summed in row-major order in float32)."""
     bsz, n, _ = a.shape
-    x = np.zeros((bsz, n, n, 8))
+    m = n // sub
+    x = np.zeros((bsz, m, m, 8))
     mu, sd = 0.5 * (LN3 + LN12), 0.5 * (LN12 - LN3)
-    x[..., 0] = ((np.log(a) - mu) / sd).astype(np.float32)
-    x[..., 1] = 1.0
-    x[..., 2] = position(n)
+    la = ((np.log(a) - mu) / sd).astype(np.float32)
+    pos = position(n).astype(np.float32)
+    ps = np.zeros((m, m), dtype=np.float32)
+    for r in range(sub * sub):
+        di, dj = divmod(r, sub)
+        x[..., r] = la[:, di::sub, dj::sub]
+        ps = (ps + pos[di::sub, dj::sub]).astype(np.float32)
+    x[..., sub * sub] = 1.0
+    x[..., sub * sub + 1] = ps * np.float32(1.0 / (sub * sub))
     return x
 
 
@@ -215,6 +275,7 @@ class Net:
 
     def __init__(self, params: np.ndarray, emulate: bool = False):
         self.emulate = emulate
+        self.sub = sub_of(np.asarray(params))
         self.layers = unpack(np.asarray(params, dtype=np.float64))
         self.R = bf16 if emulate else (lambda v: v)
         nconv = 2 * LEVELS - 1
@@ -227,7 +288,7 @@ class Net:
 
     def forward(self, a: np.ndarray) -> dict:
         R, s = self.R, {}
-        x = R(features(a))
+        x = R(features(a, self.sub))
         s["x"] = x
         for l in range(LEVELS):
             w, b = self.w[l]
@@ -241,7 +302,8 @@ class Net:
             s[f"zm{i}"], s[f"am{i}"] = z, R(gelu(z))
             x = s[f"am{i}"]
         w, b = self.w[-1]
-        s["y"] = conv_forward(x, w, b, 1)[..., 0]
+        yc = conv_forward(x, w, b, 1)
+        s["y"] = yc[..., 0] if self.sub == 1 else sub2_out(yc)
         return s
 
     def loss(self, a: np.ndarray, u: np.ndarray) -> tuple[float, dict]:
@@ -259,7 +321,7 @@ class Net:
         R = self.R
         loss, s = self.loss(a, u)
         grads: list = [None] * (2 * LEVELS)
-        dy = s["dy"][..., None]
+        dy = s["dy"][..., None] if self.sub == 1 else sub2_adjoint(s["dy"])
         w, _ = self.w[-1]
         grads[-1] = conv_backward_weight(s["am0"], dy, 1)
         dm = R(conv_backward_data(dy, w, s["am0"].shape, 1))
@@ -291,16 +353,20 @@ class Net:
         return f[:, None, None] * SCALE * (position(a.shape[1]) * s["y"])
 
 
-def init_params(seed: int) -> np.ndarray:
-    """He-normal conv weights for GELU, zero biases; unused input channels 3..7 zero."""
+def init_params(seed: int, sub: int = 1) -> np.ndarray:
+    """He-normal conv weights for GELU, zero biases; unused input channels
+    (sub^2 + 2 .. 7) zero."""
     rng = np.random.default_rng(seed)
-    lay, total = layout()
+    lay, total = layout(sub)
+    used = sub * sub + 2
     p = np.zeros(total, dtype=np.float32)
     for i, (w, b, shape) in enumerate(lay):
-        fan_in = 9 * (3 if i == 0 else shape[3])
+        fan_in = 9 * (used if i == 0 else shape[3])
         std = np.sqrt(2.0 / fan_in) if i < len(lay) - 1 else 0.1 * np.sqrt(1.0 / fan_in)
         wt = rng.normal(0.0, std, size=shape).astype(np.float32)
         if i == 0:
-            wt[..., 3:] = 0.0
+            wt[..., used:] = 0.0
+        if i == len(lay) - 1:
+            wt[1:] = 0.0  # the detail channels start at zero
         p[w:b] = wt.ravel()
     return p

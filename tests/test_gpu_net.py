@@ -117,10 +117,10 @@ def _device_batch(ctx, arrays):
     return ptrs
 
 
-def _params(seed=3):
+def _params(seed=3, sub=1):
     rng = np.random.default_rng(seed)
-    p = no.init_params(seed)
-    lay, _ = no.layout()
+    p = no.init_params(seed, sub)
+    lay, _ = no.layout(sub)
     for w, b, shape in lay:  # nonzero biases so every path carries signal
         p[b:b + shape[0]] = rng.normal(size=shape[0]) * 0.05
     return p
@@ -130,23 +130,27 @@ def _target(a):
     return 0.012 * no.position(a.shape[1])[None] * (6.0 / a) ** 0.3
 
 
-@pytest.fixture(scope="module")
-def net_case(ctx):
-    n, seeds = 64, [5, 6]
+# The full-resolution network at n = 64 and the half-resolution one (sub = 2:
+# 2 x 2 blocks of cells per network pixel, a four-channel sub-pixel head) at
+# n = 128, whose levels then have the same sides.
+@pytest.fixture(scope="module", params=[1, 2], ids=["full-res", "half-res"])
+def net_case(ctx, request):
+    sub = request.param
+    n, seeds = 64 * sub, [5, 6]
     a = np.stack([two_phase(n, seed=s) for s in seeds])
     u = _target(a)
     a_ptrs, u_ptrs = _device_batch(ctx, a), _device_batch(ctx, u)
-    yield n, a, u, a_ptrs, u_ptrs
+    yield n, sub, a, u, a_ptrs, u_ptrs
     for p in a_ptrs + u_ptrs:
         ctx.free(p)
 
 
 def test_net_gradient_matches_oracle(ctx, net_case):
-    n, a, u, a_ptrs, u_ptrs = net_case
-    params = _params()
-    net = capi.Net(ctx, n, max_batch=4, train=True)
+    n, sub, a, u, a_ptrs, u_ptrs = net_case
+    params = _params(3, sub)
+    net = capi.Net(ctx, n, max_batch=4, train=True, sub=sub)
     net.set_params(params)
-    print("net gradient n=64 batch 2", flush=True)
+    print(f"net gradient n={n} sub={sub} batch 2", flush=True)
     loss, g = net.gradient(a_ptrs, u_ptrs)
     net.close()
     loss_e, g_e = no.Net(params, emulate=True).gradient(a, u)
@@ -156,7 +160,7 @@ def test_net_gradient_matches_oracle(ctx, net_case):
     # up to 7e-3 (bf16 rounding flips); emulated and exact FP64 gradients
     # differ by up to 1.3e-2.  Bounds: about 3x those.
     assert abs(loss - loss_e) <= 5e-4 * loss_e
-    lay, _ = no.layout()
+    lay, _ = no.layout(sub)
     for i, (w, b, shape) in enumerate(lay):
         sl = slice(w, b + shape[0])
         assert rel_l2(g[sl], g_e[sl]) < 2e-2, f"layer {i} vs emulated oracle"
@@ -164,13 +168,13 @@ def test_net_gradient_matches_oracle(ctx, net_case):
 
 
 def test_net_predict_matches_oracle(ctx, net_case):
-    n, a, u, a_ptrs, _ = net_case
-    params = _params(4)
-    net = capi.Net(ctx, n, max_batch=1)  # two chunks of one
+    n, sub, a, u, a_ptrs, _ = net_case
+    params = _params(4, sub)
+    net = capi.Net(ctx, n, max_batch=1, sub=sub)  # two chunks of one
     net.set_params(params)
     out = [ctx.alloc(n * n * 8) for _ in a_ptrs]
     f = [1.0, 2.5]
-    print("net predict n=64", flush=True)
+    print(f"net predict n={n} sub={sub}", flush=True)
     net.predict_device(a_ptrs, f, out)
     got = np.stack([np.empty((n, n)) for _ in out])
     for i, p in enumerate(out):
@@ -185,11 +189,11 @@ def test_net_predict_matches_oracle(ctx, net_case):
     # differs from exact FP64 by 9.5e-3.  Indexing mistakes give O(1).
     assert rel_l2(got, want) < 2e-2
     assert rel_l2(got, exact) < 4e-2
-    np.testing.assert_array_equal(net_param_roundtrip(ctx, n, params), params)
+    np.testing.assert_array_equal(net_param_roundtrip(ctx, n, params, sub), params)
 
 
-def net_param_roundtrip(ctx, n, params):
-    net = capi.Net(ctx, n, max_batch=1)
+def net_param_roundtrip(ctx, n, params, sub):
+    net = capi.Net(ctx, n, max_batch=1, sub=sub)
     net.set_params(params)
     out = net.get_params()
     net.close()
@@ -197,10 +201,10 @@ def net_param_roundtrip(ctx, n, params):
 
 
 def test_adam_steps_reduce_loss(ctx, net_case):
-    n, a, u, a_ptrs, u_ptrs = net_case
-    net = capi.Net(ctx, n, max_batch=2, train=True)
-    net.set_params(_params(5))
-    print("net train steps n=64", flush=True)
+    n, sub, a, u, a_ptrs, u_ptrs = net_case
+    net = capi.Net(ctx, n, max_batch=2, train=True, sub=sub)
+    net.set_params(_params(5, sub))
+    print(f"net train steps n={n} sub={sub}", flush=True)
     losses = [net.train_step(a_ptrs, u_ptrs, 1e-3) for _ in range(30)]
     net.close()
     assert np.all(np.isfinite(losses))
@@ -210,6 +214,8 @@ def test_adam_steps_reduce_loss(ctx, net_case):
 def test_net_rejects_bad_shapes(ctx):
     with pytest.raises(ValueError, match="multiple of 64"):
         capi.Net(ctx, 96, 1)
+    with pytest.raises(ValueError, match="multiple of 128"):
+        capi.Net(ctx, 64, 1, sub=2)
     net = capi.Net(ctx, 64, 2)
     with pytest.raises(ValueError, match="without training buffers"):
         net.train_step([0], [0], 1e-3)
