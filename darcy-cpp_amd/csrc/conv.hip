@@ -794,67 +794,15 @@ __global__ __launch_bounds__(kT) void k_head_fwd(const bf16* x, Geo in, int C, c
     }
 }
 
-// The head for 32 input channels: a 16 x 16 pixel tile per workgroup over
-// its 18 x 18 bf16 input patch, staged once in LDS (a pixel's 32 channels
-// padded to 40, so 16 lanes reading consecutive pixels hit disjoint banks);
-// each thread forms the C outputs of its pixel from one pass over the patch,
-// 288 fused multiply-adds each.  Sum order per output: bias, then taps, then
-// channels.  With `out.u0` (C = 1 only) the kernel stores the warm start
-// itself, u0 = f scale pos y in FP64 as k_output forms it, instead of y.
+// Where the head writes the warm start itself (u0 = f scale pos y in FP64,
+// as k_output forms it) instead of y.
 struct HeadOut {
     const float* pos;
     const double* f;
     double scale;
     double* const* u0;
 };
-constexpr int kHeadTile = 16, kHeadC = 32, kHeadPad = 40;
-template <int C>
-__global__ __launch_bounds__(kT) void k_head_fwd_tiled(const bf16* x, Geo in, const float* w, const float* bias,
-                                                       float* yc, HeadOut out) {
-    __shared__ __align__(16) bf16 tile[(kHeadTile + 2) * (kHeadTile + 2) * kHeadPad];
-    __shared__ __align__(16) float ws[C * (9 * kHeadC + 1)];  // [tap][ci][r], then the biases
-    const int tx0 = blockIdx.x * kHeadTile, ty0 = blockIdx.y * kHeadTile, b = blockIdx.z;
-    for (int i = threadIdx.x; i < C * 9 * kHeadC; i += kT) ws[(i % (9 * kHeadC)) * C + i / (9 * kHeadC)] = w[i];
-    if (threadIdx.x < C) ws[C * 9 * kHeadC + threadIdx.x] = bias[threadIdx.x];
-    constexpr int side = kHeadTile + 2, chunks = side * side * (kHeadC / 8);
-    stage_items<chunks>(
-        [&](int q) {
-            const int ch = (q & 3) * 8, p = q >> 2, iy = ty0 + p / side - 1, ix = tx0 + p % side - 1;
-            v8bf v = {};
-            if (iy >= 0 && iy < in.h && ix >= 0 && ix < in.w) v = *reinterpret_cast<const v8bf*>(x + at(in, b, iy, ix) + ch);
-            return v;
-        },
-        [&](int q, const v8bf& v) { *reinterpret_cast<v8bf*>(tile + (q >> 2) * kHeadPad + (q & 3) * 8) = v; });
-    __syncthreads();
-    const int ly = threadIdx.x / kHeadTile, lx = threadIdx.x % kHeadTile;
-    const int oy = ty0 + ly, ox = tx0 + lx;
-    float s[C];
-#pragma unroll
-    for (int r = 0; r < C; ++r) s[r] = ws[C * 9 * kHeadC + r];
-    for (int tap = 0; tap < 9; ++tap) {
-        const bf16* xp = tile + ((ly + tap / 3) * side + lx + tap % 3) * kHeadPad;
-#pragma unroll
-        for (int ci = 0; ci < kHeadC; ci += 8) {
-            const v8bf v = *reinterpret_cast<const v8bf*>(xp + ci);
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const float xv = bf2f(v[e]);
-                const float* wp = ws + (tap * kHeadC + ci + e) * C;
-#pragma unroll
-                for (int r = 0; r < C; ++r) s[r] = fmaf(wp[r], xv, s[r]);
-            }
-        }
-    }
-    if (oy < in.h && ox < in.w) {
-        const size_t k = static_cast<size_t>(oy) * in.w + ox;
-        if (C == 1 && out.u0) {
-            out.u0[b][k] = out.f[b] * out.scale * static_cast<double>(out.pos[k] * s[0]);
-        } else {
-#pragma unroll
-            for (int r = 0; r < C; ++r) yc[((static_cast<size_t>(b) * in.h * in.w) + k) * C + r] = s[r];
-        }
-    }
-}
+constexpr int kHeadC = 32;
 
 // dX[pixel][ci] = sum_tap sum_r w[r][tap][ci] dyc[pixel - tap][r].
 __global__ __launch_bounds__(kT) void k_head_bwd_data(const float* dyc, Geo in, int C, const float* w, bf16* dx) {
@@ -901,6 +849,109 @@ __global__ __launch_bounds__(kT) void k_head_dw_part(const bf16* x, Geo in, int 
     }
 }
 
+// The head on the matrix cores: the FP32 weights are split as w = hi + lo
+// with hi = bf16(w), lo = bf16(w - hi) (|w - hi - lo| <= 2^-18 |w|), and
+// each K step multiplies the bf16 activations by hi and by lo into the same
+// FP32 accumulator (bf16 x bf16 products are exact in FP32).  A workgroup
+// owns a TH x TW pixel tile (TH TW = 256) and stages its (TH+2) x (TW+2)
+// patch of 32 channels once, as k_conv_halo does; the B operand is one
+// 16-column tile of which the first C columns are the outputs.  Output:
+// yc [b][h][w][C] FP32 = acc + bias, or with out.u0 (C = 1) the warm start.
+// whl: [2][C][9][32] bf16 (hi, then lo).
+template <int TW, int C>
+__global__ __launch_bounds__(kT) void k_head_mfma(const bf16* x, Geo in, const bf16* whl, const float* bias,
+                                                  float* yc, HeadOut out) {
+    constexpr int TH = 256 / TW, PW = TW + 2, PP = PW * (TH + 2), RB = 80;
+    constexpr int kPatch = PP * RB, kW = 2 * 9 * C * RB, kOut = 256 * C * 4;
+    constexpr int kBytes = kPatch + kW > kOut ? kPatch + kW : kOut;
+    __shared__ __align__(16) char smem[kBytes];
+    auto lp = [&](int pix, int gg) -> v8bf& { return *reinterpret_cast<v8bf*>(smem + pix * RB + 16 * gg); };
+    auto lw = [&](int part, int tap, int n, int gg) -> v8bf& {
+        return *reinterpret_cast<v8bf*>(smem + kPatch + ((part * 9 + tap) * C + n) * RB + 16 * gg);
+    };
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, r = lane & 15, g = lane >> 4;
+    const int tiles_x = in.w / TW, tiles_y = in.h / TH, nb = gridDim.x;
+    int t = blockIdx.x;
+    if ((nb & 7) == 0) t = (t & 7) * (nb >> 3) + (t >> 3);
+    const int tx = t % tiles_x, q = t / tiles_x, ty = q % tiles_y, b = q / tiles_y;
+    const int x0 = tx * TW, y0 = ty * TH;
+    // items [0, 4 PP): the patch, (pixel, chunk); then the weights,
+    // (part, tap, n, chunk) for the C live columns.
+    stage_items<4 * PP + 4 * 2 * 9 * C>(
+        [&](int qq) {
+            v8bf v = {};
+            const int gg = qq & 3, row = qq >> 2;
+            if (row < PP) {
+                const int iy = y0 - 1 + row / PW, ix = x0 - 1 + row % PW;
+                if (iy >= 0 && iy < in.h && ix >= 0 && ix < in.w)
+                    v = *reinterpret_cast<const v8bf*>(x + at(in, b, iy, ix) + 8 * gg);
+            } else {
+                const int rest = row - PP, n = rest % C, pt = rest / C;  // pt = part * 9 + tap
+                v = *reinterpret_cast<const v8bf*>(whl + ((pt / 9) * C + n) * 288 + (pt % 9) * 32 + 8 * gg);
+            }
+            return v;
+        },
+        [&](int qq, const v8bf& v) {
+            const int gg = qq & 3, row = qq >> 2;
+            if (row < PP) {
+                lp(row, gg) = v;
+            } else {
+                const int rest = row - PP, pt = rest / C;
+                lw(pt / 9, pt % 9, rest % C, gg) = v;
+            }
+        });
+    __syncthreads();
+    v4f acc[4] = {};
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap) {
+        const int ky = tap / 3, kx = tap % 3;
+        // columns r >= C of the B tile are zero
+        const v8bf bh = r < C ? lw(0, tap, r, g) : v8bf{}, bl = r < C ? lw(1, tap, r, g) : v8bf{};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int p = 64 * wave + 16 * i + r;
+            const v8bf a = lp((p / TW + ky) * PW + p % TW + kx, g);
+            acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bh, acc[i], 0, 0, 0);
+            acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bl, acc[i], 0, 0, 0);
+        }
+    }
+    // Lane (r, g) holds column r of tile rows 4 g .. 4 g + 3: staged as
+    // [pixel][C] in LDS, then stored row by row of the tile, consecutive
+    // threads on consecutive floats.
+    __syncthreads();
+    float* st = reinterpret_cast<float*>(smem);
+    if (r < C) {
+        const float bn = bias[r];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) st[(64 * wave + 16 * i + 4 * g + e) * C + r] = acc[i][e] + bn;
+    }
+    __syncthreads();
+    if (C == 1 && out.u0) {
+        const int p = tid;
+        const size_t k = static_cast<size_t>(y0 + p / TW) * in.w + x0 + p % TW;
+        out.u0[b][k] = out.f[b] * out.scale * static_cast<double>(out.pos[k] * st[p]);
+        return;
+    }
+    for (int e = tid; e < 256 * C; e += kT) {
+        const int p = e / C, rr = e % C;
+        const size_t k = static_cast<size_t>(y0 + p / TW) * in.w + x0 + p % TW;
+        yc[(static_cast<size_t>(b) * in.h * in.w + k) * C + rr] = st[e];
+    }
+}
+
+// hi / lo bf16 parts of the head's FP32 weights: whl[0][i] = bf16(w[i]),
+// whl[1][i] = bf16(w[i] - hi).
+__global__ __launch_bounds__(kT) void k_head_split(const float* w, int count, bf16* whl) {
+    for (int i = blockIdx.x * kT + threadIdx.x; i < count; i += gridDim.x * kT) {
+        const float v = w[i];
+        const bf16 hi = f2bf(v);
+        whl[i] = hi;
+        whl[count + i] = f2bf(v - bf2f(hi));
+    }
+}
+
 // ------------------------------------ the half-resolution output map (sub 2)
 // yc: [b][m][m][5] -> y: [b][n][n], n = 2m.  Cell (i, j) = (2I + di, 2J + dj)
 // takes the bilinear interpolation of channel 0 at its centre (cell-centred
@@ -916,26 +967,56 @@ __device__ __forceinline__ int sub2_side(int i, int m) {
     return J < 0 ? 0 : (J >= m ? m - 1 : J);
 }
 
-__device__ __forceinline__ float sub2_value(const float* yc, int m, int i, int j) {
-    const int I = i >> 1, J = j >> 1, Ii = sub2_side(i, m), Jj = sub2_side(j, m);
-    auto c0 = [&](int a, int c) { return yc[(static_cast<size_t>(a) * m + c) * 5]; };
-    const float row_own = 0.75f * c0(I, J) + 0.25f * c0(I, Jj);
-    const float row_side = 0.75f * c0(Ii, J) + 0.25f * c0(Ii, Jj);
-    return (0.75f * row_own + 0.25f * row_side) + yc[(static_cast<size_t>(I) * m + J) * 5 + 1 + 2 * (i & 1) + (j & 1)];
-}
-
-// y (training) or u0 = f scale pos y (inference, with out.u0).
+// y (training) or u0 = f scale pos y (inference, with out.u0): one thread
+// per pixel (I, J) forms its block's four cells from channel 0 of the 3 x 3
+// pixels around it (indices clamped) and its four detail channels, and
+// stores each row's two cells as one 16-byte (u0) or 8-byte (y) store.
 __global__ __launch_bounds__(kT) void k_sub2_out(const float* yc, int n, int batch, float* y, HeadOut out) {
     const int m = n / 2;
-    const size_t per = static_cast<size_t>(n) * n;
+    const size_t per = static_cast<size_t>(m) * m;
     for (size_t q = blockIdx.x * static_cast<size_t>(kT) + threadIdx.x; q < per * batch; q += static_cast<size_t>(gridDim.x) * kT) {
         const size_t b = q / per, k = q % per;
-        const int i = static_cast<int>(k / n), j = static_cast<int>(k % n);
-        const float v = sub2_value(yc + b * static_cast<size_t>(m) * m * 5, m, i, j);
-        if (out.u0)
-            out.u0[b][k] = out.f[b] * out.scale * static_cast<double>(out.pos[k] * v);
-        else
-            y[q] = v;
+        const int I = static_cast<int>(k / m), J = static_cast<int>(k % m);
+        const float* ycb = yc + b * per * 5;
+        const int rows[3] = {I > 0 ? I - 1 : 0, I, I + 1 < m ? I + 1 : m - 1};
+        const int cols[3] = {J > 0 ? J - 1 : 0, J, J + 1 < m ? J + 1 : m - 1};
+        float c[3][3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int e = 0; e < 3; ++e) c[a][e] = ycb[(static_cast<size_t>(rows[a]) * m + cols[e]) * 5];
+        const float* det = ycb + k * 5 + 1;
+        const float d0 = det[0], d1 = det[1], d2 = det[2], d3 = det[3];
+#pragma unroll
+        for (int di = 0; di < 2; ++di) {
+            // cell row 2I + di: own row I, side row I - 1 (di 0) or I + 1 (di 1)
+            const int side = di ? 2 : 0;
+            float v[2];
+#pragma unroll
+            for (int dj = 0; dj < 2; ++dj) {
+                const int cs = dj ? 2 : 0;
+                const float row_own = 0.75f * c[1][1] + 0.25f * c[1][cs];
+                const float row_side = 0.75f * c[side][1] + 0.25f * c[side][cs];
+                v[dj] = (0.75f * row_own + 0.25f * row_side) + (di ? (dj ? d3 : d2) : (dj ? d1 : d0));
+            }
+            const size_t kf = static_cast<size_t>(2 * I + di) * n + 2 * J;
+            if (out.u0) {
+                const float2 p2 = *reinterpret_cast<const float2*>(out.pos + kf);
+                const double fs = out.f[b] * out.scale;
+                double2 u;
+                u.x = fs * static_cast<double>(p2.x * v[0]);
+                u.y = fs * static_cast<double>(p2.y * v[1]);
+                double* dst = out.u0[b] + kf;
+                if ((reinterpret_cast<uintptr_t>(dst) & 15) == 0) {
+                    *reinterpret_cast<double2*>(dst) = u;
+                } else {  // a caller's array that is only 8-byte aligned
+                    dst[0] = u.x;
+                    dst[1] = u.y;
+                }
+            } else {
+                *reinterpret_cast<float2*>(y + b * static_cast<size_t>(n) * n + kf) = make_float2(v[0], v[1]);
+            }
+        }
     }
 }
 
@@ -1003,10 +1084,25 @@ __global__ __launch_bounds__(kT) void k_features(const double* const* a, const f
         const int I = static_cast<int>(k / m), J = static_cast<int>(k % m);
         v8bf v = {};
         float ps = 0.0f;
-        for (int r = 0; r < sub * sub; ++r) {
-            const size_t kf = static_cast<size_t>(sub * I + r / sub) * n + sub * J + r % sub;
-            v[r] = f2bf(static_cast<float>((log(a[b][kf]) - mu) / sd));
-            ps += pos[kf];
+        const double* ab = a[b];
+        if (sub == 2 && (reinterpret_cast<uintptr_t>(ab) & 15) == 0) {
+            // the block's two rows as one 16-byte load each (pos rows as 8)
+            const size_t k0 = static_cast<size_t>(2 * I) * n + 2 * J;
+            const double2 r0 = *reinterpret_cast<const double2*>(ab + k0);
+            const double2 r1 = *reinterpret_cast<const double2*>(ab + k0 + n);
+            const float2 p0 = *reinterpret_cast<const float2*>(pos + k0);
+            const float2 p1 = *reinterpret_cast<const float2*>(pos + k0 + n);
+            v[0] = f2bf(static_cast<float>((log(r0.x) - mu) / sd));
+            v[1] = f2bf(static_cast<float>((log(r0.y) - mu) / sd));
+            v[2] = f2bf(static_cast<float>((log(r1.x) - mu) / sd));
+            v[3] = f2bf(static_cast<float>((log(r1.y) - mu) / sd));
+            ps = ((p0.x + p0.y) + p1.x) + p1.y;
+        } else {
+            for (int r = 0; r < sub * sub; ++r) {
+                const size_t kf = static_cast<size_t>(sub * I + r / sub) * n + sub * J + r % sub;
+                v[r] = f2bf(static_cast<float>((log(ab[kf]) - mu) / sd));
+                ps += pos[kf];
+            }
         }
         v[sub * sub] = f2bf(1.0f);
         v[sub * sub + 1] = f2bf(ps * (1.0f / static_cast<float>(sub * sub)));
@@ -1199,6 +1295,40 @@ using namespace darcy_nn;
 
 // ------------------------------------------------------------ launchers
 // All pointers are device pointers; shapes were checked by the caller.
+// The head conv: x [b][h][w][cin] -> yc [b][h][w][C], C 1 or 5.  With the
+// hi / lo weights whl (darcy_nn_head_split), 32 channels and a tileable
+// image it runs on the matrix cores (k_head_mfma), else in FP32 one pixel
+// per thread.  With u0 (C = 1 only) the head stores the warm start itself.
+template <int C>
+static bool launch_head_mfma(const bf16* x, Geo in, const bf16* whl, const float* bias, float* yc, HeadOut out,
+                      hipStream_t s) {
+    if (!whl || in.c != kHeadC) return false;
+    const int tw = in.w % 64 == 0 && in.h % 4 == 0 ? 64 : (in.w % 32 == 0 && in.h % 8 == 0 ? 32 : 0);
+    if (tw == 0) return false;
+    const unsigned grid = static_cast<unsigned>(in.w / tw) * (in.h / (256 / tw)) * in.b;
+    if (tw == 64)
+        k_head_mfma<64, C><<<grid, kT, 0, s>>>(x, in, whl, bias, yc, out);
+    else
+        k_head_mfma<32, C><<<grid, kT, 0, s>>>(x, in, whl, bias, yc, out);
+    return true;
+}
+
+static hipError_t head_conv(const void* x, Geo in, int C, const float* wgt, const void* whl, const float* bias, float* yc,
+                     HeadOut out, hipStream_t s) {
+    if (C != 1 && C != 5) return hipErrorInvalidValue;
+    const bf16* xb = static_cast<const bf16*>(x);
+    const bf16* hl = static_cast<const bf16*>(whl);
+    const bool done = C == 1 ? launch_head_mfma<1>(xb, in, hl, bias, yc, out, s)
+                             : launch_head_mfma<5>(xb, in, hl, bias, yc, out, s);
+    if (done) return hipGetLastError();
+    k_head_fwd<<<ew_grid(static_cast<size_t>(in.b) * in.h * in.w), kT, 0, s>>>(xb, in, C, wgt, bias, yc);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !out.u0) return e;
+    k_output<<<ew_grid(static_cast<size_t>(in.h) * in.w * in.b), kT, 0, s>>>(yc, out.pos, in.h, in.b, out.f, out.scale,
+                                                                             out.u0);
+    return hipGetLastError();
+}
+
 extern "C" {
 
 hipError_t darcy_nn_mfma_probe(const void* A, const void* B, float* C, hipStream_t s) {
@@ -1268,21 +1398,15 @@ hipError_t darcy_nn_conv_bwd_weight(const void* x, int b, int h, int w, int cin,
     return hipGetLastError();
 }
 
-// The head conv: x [b][h][w][cin] -> yc [b][h][w][C], C 1 or 5.
-hipError_t darcy_nn_head_fwd(const void* x, int b, int h, int w, int cin, int C, const float* wgt, const float* bias,
-                             float* yc, hipStream_t s) {
-    if (C != 1 && C != 5) return hipErrorInvalidValue;
-    if (cin == kHeadC) {
-        const dim3 grid((w + kHeadTile - 1) / kHeadTile, (h + kHeadTile - 1) / kHeadTile, b);
-        const HeadOut none{nullptr, nullptr, 0.0, nullptr};
-        if (C == 1)
-            k_head_fwd_tiled<1><<<grid, kT, 0, s>>>(static_cast<const bf16*>(x), Geo{b, h, w, cin}, wgt, bias, yc, none);
-        else
-            k_head_fwd_tiled<5><<<grid, kT, 0, s>>>(static_cast<const bf16*>(x), Geo{b, h, w, cin}, wgt, bias, yc, none);
-        return hipGetLastError();
-    }
-    k_head_fwd<<<ew_grid(static_cast<size_t>(b) * h * w), kT, 0, s>>>(static_cast<const bf16*>(x), Geo{b, h, w, cin}, C,
-                                                                       wgt, bias, yc);
+// x: [b][h][w][cin] -> yc [b][h][w][C] (head_conv).
+hipError_t darcy_nn_head_fwd(const void* x, int b, int h, int w, int cin, int C, const float* wgt, const void* whl,
+                             const float* bias, float* yc, hipStream_t s) {
+    return head_conv(x, Geo{b, h, w, cin}, C, wgt, whl, bias, yc, HeadOut{nullptr, nullptr, 0.0, nullptr}, s);
+}
+
+// hi / lo parts of the head's count FP32 weights: whl [2][count] bf16.
+hipError_t darcy_nn_head_split(const float* w, int count, void* whl, hipStream_t s) {
+    k_head_split<<<ew_grid(static_cast<size_t>(count)), kT, 0, s>>>(w, count, static_cast<bf16*>(whl));
     return hipGetLastError();
 }
 
@@ -1308,7 +1432,7 @@ hipError_t darcy_nn_head_bwd_weight(const void* x, int b, int h, int w, int cin,
 
 // Half-resolution output map: y [b][n][n] from yc [b][n/2][n/2][5].
 hipError_t darcy_nn_sub2_out(const float* yc, int n, int batch, float* y, hipStream_t s) {
-    k_sub2_out<<<ew_grid(static_cast<size_t>(n) * n * batch), kT, 0, s>>>(yc, n, batch, y,
+    k_sub2_out<<<ew_grid(static_cast<size_t>(n / 2) * (n / 2) * batch), kT, 0, s>>>(yc, n, batch, y,
                                                                          HeadOut{nullptr, nullptr, 0.0, nullptr});
     return hipGetLastError();
 }
@@ -1371,28 +1495,20 @@ hipError_t darcy_nn_output(const float* y, const float* pos, int n, int batch, c
 }
 
 // The head and the output in one: u0 = f scale pos y for n x n problems, x
-// the finest network level (side n / sub).  sub 1: the 32-channel head
-// stores u0 itself (other widths go through yc); sub 2: the 5-channel head
-// writes yc, then the output map forms u0.
-hipError_t darcy_nn_head_output(const void* x, int b, int n, int cin, int sub, const float* wgt, const float* bias,
-                                float* yc, const float* pos, const double* f, double scale, double* const* u0,
-                                hipStream_t s) {
+// the finest network level (side n / sub).  sub 1: the head stores u0
+// itself; sub 2: the 5-channel head writes yc, then the output map forms u0.
+hipError_t darcy_nn_head_output(const void* x, int b, int n, int cin, int sub, const float* wgt, const void* whl,
+                                const float* bias, float* yc, const float* pos, const double* f, double scale,
+                                double* const* u0, hipStream_t s) {
     if (sub != 1 && sub != 2) return hipErrorInvalidValue;
     if (sub == 2) {
-        hipError_t e = darcy_nn_head_fwd(x, b, n / 2, n / 2, cin, 5, wgt, bias, yc, s);
+        hipError_t e = darcy_nn_head_fwd(x, b, n / 2, n / 2, cin, 5, wgt, whl, bias, yc, s);
         if (e != hipSuccess) return e;
-        k_sub2_out<<<ew_grid(static_cast<size_t>(n) * n * b), kT, 0, s>>>(yc, n, b, nullptr, HeadOut{pos, f, scale, u0});
+        k_sub2_out<<<ew_grid(static_cast<size_t>(n / 2) * (n / 2) * b), kT, 0, s>>>(yc, n, b, nullptr,
+                                                                                    HeadOut{pos, f, scale, u0});
         return hipGetLastError();
     }
-    if (cin == kHeadC) {
-        const dim3 grid((n + kHeadTile - 1) / kHeadTile, (n + kHeadTile - 1) / kHeadTile, b);
-        k_head_fwd_tiled<1><<<grid, kT, 0, s>>>(static_cast<const bf16*>(x), Geo{b, n, n, cin}, wgt, bias, yc,
-                                                HeadOut{pos, f, scale, u0});
-        return hipGetLastError();
-    }
-    const hipError_t e = darcy_nn_head_fwd(x, b, n, n, cin, 1, wgt, bias, yc, s);
-    if (e != hipSuccess) return e;
-    return darcy_nn_output(yc, pos, n, b, f, scale, u0, s);
+    return head_conv(x, Geo{b, n, n, cin}, 1, wgt, whl, bias, yc, HeadOut{pos, f, scale, u0}, s);
 }
 
 // partial: `blocks` floats.

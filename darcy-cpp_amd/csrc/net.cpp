@@ -25,8 +25,9 @@ hipError_t darcy_nn_conv_bwd_weight(const void* x, int b, int h, int w, int cin,
                                     int splits, float* part, float* dw, float* db, hipStream_t s);
 hipError_t darcy_nn_conv_fwd_merge(const void* x, int b, int h, int w, int cin, const void* wgt, const float* bias,
                                    int cout, const void* skip, void* z, void* act, hipStream_t s);
-hipError_t darcy_nn_head_fwd(const void* x, int b, int h, int w, int cin, int C, const float* wgt, const float* bias,
-                             float* yc, hipStream_t s);
+hipError_t darcy_nn_head_fwd(const void* x, int b, int h, int w, int cin, int C, const float* wgt, const void* whl,
+                             const float* bias, float* yc, hipStream_t s);
+hipError_t darcy_nn_head_split(const float* w, int count, void* whl, hipStream_t s);
 hipError_t darcy_nn_head_bwd_data(const float* dyc, int b, int h, int w, int cin, int C, const float* wgt, void* dx,
                                   hipStream_t s);
 hipError_t darcy_nn_head_bwd_weight(const void* x, int b, int h, int w, int cin, int C, const float* dyc, int per,
@@ -44,9 +45,9 @@ hipError_t darcy_nn_add(void* dst, const void* src, size_t count, hipStream_t s)
 hipError_t darcy_nn_transpose(const void* in, int P, int C, void* out, hipStream_t s);
 hipError_t darcy_nn_output(const float* y, const float* pos, int n, int batch, const double* f, double scale,
                            double* const* u0, hipStream_t s);
-hipError_t darcy_nn_head_output(const void* x, int b, int n, int cin, int sub, const float* wgt, const float* bias,
-                                float* yc, const float* pos, const double* f, double scale, double* const* u0,
-                                hipStream_t s);
+hipError_t darcy_nn_head_output(const void* x, int b, int n, int cin, int sub, const float* wgt, const void* whl,
+                                const float* bias, float* yc, const float* pos, const double* f, double scale,
+                                double* const* u0, hipStream_t s);
 hipError_t darcy_nn_loss(const float* y, const double* const* u, const double* const* a, const float* pos, int n,
                          int batch, double scale, float lam, float* dy, float* partial, int blocks, hipStream_t s);
 hipError_t darcy_nn_adam(float* w, const float* g, float* m, float* v, void* wb, size_t count, float lr, float b1,
@@ -144,7 +145,7 @@ struct darcy_net {
     bool train = false;
     const Layout* lay = &kLayouts[0];
     long step = 0;
-    DevBuf params, wb, wt, grad, m, v, pos, feat, y, yc;
+    DevBuf params, wb, wt, whl, grad, m, v, pos, feat, y, yc;
     DevBuf act[kLevels], z[kLevels];               // encoder outputs (after / before GELU)
     DevBuf zm[kLevels - 1], am[kLevels - 1];       // merges at level i (before / after GELU)
     DevBuf dA[kLevels], dM[kLevels - 1], dC[kLevels - 1];
@@ -177,6 +178,7 @@ int darcy_net::alloc() {
     NN_TRY(params.reserve(P * 4), "net parameters");
     NN_TRY(wb.reserve(P * 2), "net parameters");
     NN_TRY(wt.reserve(P * 2), "net parameters");
+    NN_TRY(whl.reserve(static_cast<size_t>(2) * hc() * 9 * kHeadCin * 2), "net parameters");
     NN_TRY(hipMemsetAsync(params.ptr, 0, P * 4, ctx->stream), "net parameters");
     NN_TRY(pos.reserve(static_cast<size_t>(n) * n * 4), "net position");
     NN_TRY(darcy_nn_position(n, pos.as<float>(), ctx->stream), "net position");
@@ -221,7 +223,7 @@ int darcy_net::alloc() {
 }
 
 void darcy_net::release() {
-    DevBuf* all[] = {&params, &wb, &wt, &grad, &m, &v, &pos, &feat, &y, &yc, &dy, &dyc, &tmp, &wpart, &hpart,
+    DevBuf* all[] = {&params, &wb, &wt, &whl, &grad, &m, &v, &pos, &feat, &y, &yc, &dy, &dyc, &tmp, &wpart, &hpart,
                      &lossp, &tab_a, &tab_u, &tab_f};
     for (DevBuf* b : all) b->release();
     for (int l = 0; l < kLevels; ++l) {
@@ -237,9 +239,12 @@ void darcy_net::release() {
     }
 }
 
-// bf16 copies of the weights and the flipped transposes used by backward data.
+// bf16 copies of the weights, the flipped transposes used by backward data
+// and the head's hi / lo split.
 int darcy_net::refresh_weights() {
     NN_TRY(darcy_nn_to_bf16(params.as<float>(), wb.ptr, lay->total, ctx->stream), "net weights");
+    NN_TRY(darcy_nn_head_split(params.as<float>() + lay->w[kConvs], hc() * 9 * kHeadCin, whl.ptr, ctx->stream),
+           "net weights");
     for (int c = 1; c < kConvs; ++c)
         NN_TRY(darcy_nn_weight_t(wb.as<uint16_t>() + lay->w[c], spec(c).cout, spec(c).cin,
                                  wt.as<uint16_t>() + lay->w[c], ctx->stream),
@@ -287,13 +292,13 @@ int darcy_net::forward(int b, int64_t offset, bool to_u0) {
         x = am[i].ptr;
     }
     if (to_u0) {
-        NN_TRY(darcy_nn_head_output(x, b, n, kHeadCin, sub, params.as<float>() + lay->w[kConvs],
+        NN_TRY(darcy_nn_head_output(x, b, n, kHeadCin, sub, params.as<float>() + lay->w[kConvs], whl.ptr,
                                     params.as<float>() + lay->b[kConvs], head_out(), pos.as<float>(),
                                     tab_f.as<double>() + offset, kScale, tab_u.as<double*>() + offset, s),
                "net head");
     } else {
         NN_TRY(darcy_nn_head_fwd(x, b, side(0), side(0), kHeadCin, hc(), params.as<float>() + lay->w[kConvs],
-                                 params.as<float>() + lay->b[kConvs], head_out(), s),
+                                 whl.ptr, params.as<float>() + lay->b[kConvs], head_out(), s),
                "net head");
         if (sub != 1) NN_TRY(darcy_nn_sub2_out(yc.as<float>(), n, b, y.as<float>(), s), "net head");
     }
@@ -512,6 +517,9 @@ int train_or_grad(darcy_net* net, int64_t batch, const double* const* a, const d
             NN_TRY(darcy_nn_weight_t(net->wb.as<uint16_t>() + net->lay->w[c], spec(c).cout, spec(c).cin,
                                      net->wt.as<uint16_t>() + net->lay->w[c], ctx->stream),
                    "net update");
+        NN_TRY(darcy_nn_head_split(net->params.as<float>() + net->lay->w[kConvs], net->hc() * 9 * kHeadCin,
+                                   net->whl.ptr, ctx->stream),
+               "net update");
     }
     return DARCY_OK;
 }
