@@ -1249,7 +1249,12 @@ __global__ __launch_bounds__(kT) void k_adam(float* w, const float* g, float* m,
         const float mi = b1 * m[i] + (1.0f - b1) * gi, vi = b2 * v[i] + (1.0f - b2) * gi * gi;
         m[i] = mi;
         v[i] = vi;
-        const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+        // The step ratio m^ / (sqrt(v^) + eps) is at most about 1 while the
+        // gradient's scale holds; it is bounded to [-1, 1] so that a
+        // gradient arriving where v^ is still tiny cannot move a weight far
+        // in one step (the blow-ups that stalled half-resolution training).
+        const float ratio = fminf(fmaxf((mi / c1) / (sqrtf(vi / c2) + eps), -1.0f), 1.0f);
+        const float wi = w[i] - lr * ratio;
         w[i] = wi;
         if (wb) wb[i] = f2bf(wi);
     }

@@ -296,9 +296,9 @@ def main(argv: Sequence[str] | None = None) -> int:
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--train-count", type=int, default=TRAIN_SEEDS[1] - TRAIN_SEEDS[0])
     ap.add_argument("--eval-count", type=int, default=EVAL_SEEDS[1] - EVAL_SEEDS[0])
-    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--epochs", type=int, default=100)
     ap.add_argument("--batch", type=int, default=32)
-    ap.add_argument("--lr", type=float, default=2e-4)
+    ap.add_argument("--lr", type=float, default=4e-4)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--sub", type=int, choices=(1, 2), default=2,
                     help="sub-pixel factor: 2 runs the network at half the grid's resolution")

@@ -108,14 +108,20 @@ void darcy_ctx_destroy(darcy_ctx* ctx);
  *                     persistent batch kernel keeps in global memory, in
  *                     the FP64 cycle (128..16384, default 1024)
  *   "batch_step_cells_fp32" the same in the FP32 cycle (default 4096)
- * None of these changes any result: only the launch structure.  One more
- * key changes the arithmetic of the grid path's FP64 cycle, not the method:
- *   "grid_fp64_dense_tail" 1 (default): the levels of side <= 8 are applied
- *                     as one dense FP64 matrix per visit, formed from the
- *                     FP64 recursive cycle on unit vectors (the same linear
- *                     map, rounded differently: solutions agree to about
- *                     1e-14); 0: the recursion itself, bit for bit the
- *                     reference's cycle. */
+ * None of these changes any result: only the launch structure.  Two more
+ * keys change the rounding of the grid path's cycle, not the method:
+ *   "grid_fp64_dense_tail" 1 (default): the FP64 cycle applies its tail
+ *                     levels (see grid_dense_side) as one dense FP64 matrix
+ *                     per visit, formed from the FP64 recursive cycle on
+ *                     unit vectors (the same linear map, rounded
+ *                     differently: solutions agree to about 1e-14); 0: the
+ *                     recursion itself, bit for bit the reference's cycle
+ *   "grid_dense_side" 16 (default): the dense tail matrix covers the levels
+ *                     from the first of side <= 16 (a 256 x 256 matrix at
+ *                     most, read from L2 by the tail workgroup); 8: the
+ *                     levels of side <= 8 (64 x 64, in LDS), which gives
+ *                     the FP32 cycle of the persistent batch kernel bit for
+ *                     bit. */
 int darcy_ctx_set_tuning(darcy_ctx* ctx, const char* key, int32_t value);
 int darcy_ctx_get_tuning(const darcy_ctx* ctx, const char* key, int32_t* value);
 
@@ -271,7 +277,8 @@ int darcy_net_get_params(darcy_net* net, float* params, int64_t count);
  * `count` problems (any count; processed max_batch at a time). */
 int darcy_net_predict(darcy_net* net, int64_t count, const double* const* a, const double* f, double* const* u0);
 
-/* One Adam step (beta 0.9 / 0.999, eps 1e-8) over a batch of
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8, each weight's step ratio bounded to
+ * [-1, 1] times the rate) over a batch of
  * device-resident pairs (a, u) with f = 1, on the loss
  *   (sum_cells e^2 + 1000 e^T A e) / cells,  e = pos y - 64 u,
  * A the finite-volume operator of a (the energy norm CG converges in).
