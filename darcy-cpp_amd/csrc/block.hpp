@@ -521,6 +521,14 @@ struct Cycle {
     // or null for the recursive register-resident cycle (the reference's
     // operations, bit for bit).
     double* tail64 = nullptr;
+    // Grid-path tail only: the levels from big_l (side <= 16) down as one
+    // dense matrix in global memory, column k at big + k * big_ld (big_ld a
+    // multiple of 16 bytes), applied by the whole workgroup; big_scratch is
+    // LDS for the residual (256 entries) and the partial sums (16 bytes per
+    // thread).  big_l = -1: not used.
+    const T* big = nullptr;
+    int big_l = -1, big_ld = 0;
+    T* big_scratch = nullptr;
 
     DARCY_INLINE GlobalLv<T> G(int l) const { return global_level<T>(h, l); }
     DARCY_INLINE LvP<T> S(int l) const {
@@ -582,6 +590,49 @@ struct Cycle {
         sync();
     }
 
+    // The levels from big_l down as the dense matrix M: x = M b fresh, x +=
+    // M (b - A x) on a repeated visit.  Threads own V consecutive rows (one
+    // 16-byte load of a column) and groups of threads ranges of k, summed in
+    // order; the groups' partial sums are then added in group order.  M
+    // (256 KB in FP32, 512 KB in FP64 at side 16) is read from L2.
+    DARCY_INLINE void dense_big(int l, bool fresh, Exec e) const {
+        constexpr int V = 16 / static_cast<int>(sizeof(T));
+        typedef T vec __attribute__((ext_vector_type(V)));
+        using gvecT = __attribute__((address_space(1))) const vec;
+        const LvP<T> A = S(l);
+        const int n = A.n, dim = n * n, ld = big_ld;
+        T* res = big_scratch;
+        T* part = big_scratch + 256;
+        const T* rhs = A.b;
+        if (!fresh) {
+            for (int i = e.tid; i < dim; i += e.nthr) {
+                const int r = i / n, c = i - r * n;
+                res[i] = A.b[i] - apply_at(n, A.ex, A.ny, r, c, [&](int q) { return A.x[q]; });
+            }
+            __syncthreads();
+            rhs = res;
+        }
+        const int rows = ld / V, groups = e.nthr / rows, kper = (dim + groups - 1) / groups;
+        const int rg = e.tid % rows, g = e.tid / rows;
+        if (g < groups) {
+            const int k0 = g * kper, k1 = k0 + kper < dim ? k0 + kper : dim;
+            gvecT* col = (gvecT*)(big + rg * V);
+            const int stride = ld / V;
+            vec acc = {};
+#pragma unroll 8
+            for (int k = k0; k < k1; ++k) acc += col[k * stride] * rhs[k];
+#pragma unroll
+            for (int v = 0; v < V; ++v) part[g * ld + rg * V + v] = acc[v];
+        }
+        __syncthreads();
+        for (int i = e.tid; i < dim; i += e.nthr) {
+            T sum = part[i];
+            for (int q = 1; q < groups; ++q) sum += part[q * ld + i];
+            A.x[i] = fresh ? sum : A.x[i] + sum;
+        }
+        __syncthreads();
+    }
+
     // Global-memory levels stream one pass per visit when one sweep is
     // asked for (the default); with more sweeps they run phase by phase.
     // In the FP32 cycle a fresh visit runs as walks: 1000 x 256^2 launch
@@ -635,13 +686,15 @@ struct Cycle {
         int l = l0;
         bool fresh = fresh0;
         for (;;) {
-            while (l + 1 < L && l < t.wave) {
+            while (l + 1 < L && l < t.wave && l != big_l) {
                 down(l, fresh, e);
                 done &= ~(1u << l);
                 ++l;
                 fresh = true;
             }
-            if (l >= t.wave) {
+            if (l == big_l) {
+                dense_big(l, fresh, e);
+            } else if (l >= t.wave) {
                 if (threadIdx.x < kWave) {
                     const Exec we{static_cast<int>(threadIdx.x), kWave};
                     if constexpr (sizeof(T) == 4)
@@ -675,12 +728,15 @@ struct Cycle {
 // Tail matrix of the cycle from level tl.  Column k is the FP64 recursive
 // cycle from level tl -- the reference's arithmetic, bit-exact with the
 // oracle -- applied to the unit vector e_k, rounded to OT (FP32 for the FP32
-// cycle, FP64 for the FP64 cycle's dense tail) and stored by columns.  The
-// four waves take columns in turn; `scratch` (the LDS pool, before it
-// receives the operators) holds the FP64 operators of the tail levels and
-// each wave's x and b.
+// cycle, FP64 for the FP64 cycle's dense tail) and stored by columns, column
+// k at tail + k * ld (ld >= dim; rows dim..ld-1 are zero).  The waves of the
+// launch take columns in turn (this workgroup's first is col0, then every
+// col_step); `scratch` (the LDS pool, before it receives the operators, or
+// an array of its own) holds the FP64 operators of the tail levels and each
+// wave's x and b.
 template <class OT>
-DARCY_INLINE void build_tail_matrix(const Hier& h, int tl, const CycleOpts& o, void* scratch, OT* tail) {
+DARCY_INLINE void build_tail_matrix(const Hier& h, int tl, const CycleOpts& o, void* scratch, OT* tail, int ld = 0,
+                                    int col0 = 0, int col_step = kWaves) {
     double* dp = static_cast<double*>(scratch);
     const int L = h.L, lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
     auto op_off = [&](int l) {
@@ -698,7 +754,9 @@ DARCY_INLINE void build_tail_matrix(const Hier& h, int tl, const CycleOpts& o, v
         for (int q = threadIdx.x; q < n * n; q += kBlock) dp[base + 2 * nf + q] = h.id[l][q];
     }
     __syncthreads();
-    double* vec = dp + ops_end + wave * 170;
+    int vsz = 0;  // x and b of every level from tl, per wave
+    for (int q = tl; q < L; ++q) vsz += 2 * h.n[q] * h.n[q];
+    double* vec = dp + ops_end + wave * vsz;
     auto view = [&](int l) {
         int off = 0;
         for (int q = tl; q < l; ++q) off += 2 * h.n[q] * h.n[q];
@@ -707,13 +765,14 @@ DARCY_INLINE void build_tail_matrix(const Hier& h, int tl, const CycleOpts& o, v
                            vec + off + n * n};
     };
     const int dim = h.n[tl] * h.n[tl];
+    if (ld == 0) ld = dim;
     const WaveSync sync;
-    for (int k = wave; k < dim; k += kWaves) {
+    for (int k = col0 + wave; k < dim; k += col_step) {
         const LvP<double> top = view(tl);
         for (int q = lane; q < dim; q += kWave) top.b[q] = q == k ? 1.0 : 0.0;
         sync();
         cycle_levels<double>(tl, L, true, o, Exec{lane, kWave}, sync, view);
-        for (int q = lane; q < dim; q += kWave) tail[k * dim + q] = static_cast<OT>(top.x[q]);
+        for (int q = lane; q < ld; q += kWave) tail[static_cast<size_t>(k) * ld + q] = q < dim ? static_cast<OT>(top.x[q]) : OT(0);
         sync();
     }
     __syncthreads();

@@ -31,7 +31,9 @@ extern "C" hipError_t darcy_grid_presmooth(int mixed, const Hier* hd, int l, int
 extern "C" hipError_t darcy_grid_restrict(int mixed, const Hier* hd, int l, int blocks, hipStream_t s);
 extern "C" hipError_t darcy_grid_postsmooth(int mixed, const Hier* hd, int l, int kind, int blocks, hipStream_t s);
 extern "C" hipError_t darcy_grid_tail(int mixed, const Hier* hd, int l0, int fresh, CycleOptsC o, const float* tail,
-                                      const double* tail64, hipStream_t s);
+                                      const double* tail64, const void* big, int big_l, int big_ld, hipStream_t s);
+extern "C" hipError_t darcy_grid_big_matrix(int mixed, const Hier* hd, CycleOptsC o, int lb, int dim, void* out, int ld,
+                                            hipStream_t s);
 extern "C" hipError_t darcy_grid_cg(int which, const GridCg* g, const double* pold, double* pnew, int blocks,
                                     hipStream_t s);
 extern "C" hipError_t darcy_hip_batch_occupancy(int mixed, int* blocks_per_cu);
@@ -348,7 +350,8 @@ int run_grid(darcy_ctx* ctx, const Job& job) {
         return at;
     };
     const size_t o_slot = take(slot_bytes), o_h = take(sizeof(Hier)), o_st = take(sizeof(GridState)),
-                 o_bad = take(8), o_tail = take(64 * 64 * 8), o_hist = take(hist ? hist_n * 8 : 8);
+                 o_bad = take(8), o_tail = take(64 * 64 * 8), o_big = take(256 * 256 * 8),
+                 o_hist = take(hist ? hist_n * 8 : 8);
     const bool host = !job.device_ptrs;
     const size_t o_a = host ? take(cells) : 0, o_rhs = host && job.rhs && job.rhs[0] ? take(cells) : 0,
                  o_u0 = host && job.in && job.in[0] ? take(cells) : 0, o_u = host ? take(cells) : 0;
@@ -362,6 +365,9 @@ int run_grid(darcy_ctx* ctx, const Job& job) {
     // The FP64 cycle's tail as a dense FP64 matrix (grid_fp64_dense_tail),
     // in the same workspace.
     double* tail64 = reinterpret_cast<double*>(base + o_tail);
+    // The levels from the first of side <= 16 down as one dense matrix
+    // (grid_dense_side 16), read by the tail from L2.
+    void* big = base + o_big;
     double* history = hist ? (host ? reinterpret_cast<double*>(base + o_hist) : job.history) : nullptr;
 
     const double* a = job.a[0];
@@ -400,9 +406,23 @@ int run_grid(darcy_ctx* ctx, const Job& job) {
     int lT = 0;
     while (lT < h.L - 1 && h.n[lT] > T) ++lT;
     const CycleOptsC co{o.pre_sweeps, o.post_sweeps, o.cycle_index};
-    if (mixed && h.L > 1) HIP_TRY(darcy_grid_tail_matrix(hd, co, tail, s), "tail matrix");
     const bool dense64 = mg && !mixed && h.L > 1 && ctx->grid_fp64_dense_tail;
-    if (dense64) HIP_TRY(darcy_grid_tail_matrix64(hd, co, tail64, s), "tail matrix");
+    int big_l = -1, big_ld = 0;
+    if ((mixed || dense64) && ctx->grid_dense_side == 16) {
+        for (int l = lT + 1; l < h.L; ++l)
+            if (h.n[l] <= 16) {
+                big_l = l;
+                break;
+            }
+    }
+    if (big_l > 0) {
+        const int dim = h.n[big_l] * h.n[big_l], vec = mixed ? 4 : 2;  // elements per 16 bytes
+        big_ld = (dim + vec - 1) / vec * vec;
+        HIP_TRY(darcy_grid_big_matrix(mixed ? 1 : 0, hd, co, big_l, dim, big, big_ld, s), "tail matrix");
+    } else {
+        if (mixed && h.L > 1) HIP_TRY(darcy_grid_tail_matrix(hd, co, tail, s), "tail matrix");
+        if (dense64) HIP_TRY(darcy_grid_tail_matrix64(hd, co, tail64, s), "tail matrix");
+    }
 
     GridCg g{hd,    st,         a,          rhs,          job.f ? job.f[0] : 1.0,           u0, u, history,
              o.tol, o.max_iter, mixed ? 1 : 0, static_cast<int>(hist_n), o.breakdown_at};
@@ -423,7 +443,8 @@ int run_grid(darcy_ctx* ctx, const Job& job) {
     std::function<void(int, bool)> cycle = [&](int l, bool fresh) {
         if (rc) return;
         if (l >= lT) {
-            const hipError_t e = darcy_grid_tail(mixed, hd, l, fresh ? 1 : 0, co, tail, dense64 ? tail64 : nullptr, s);
+            const hipError_t e = darcy_grid_tail(mixed, hd, l, fresh ? 1 : 0, co, tail, dense64 ? tail64 : nullptr,
+                                                 big_l > 0 ? big : nullptr, big_l, big_ld, s);
             if (e != hipSuccess) rc = hip_fail(ctx, e, "tail cycle");
             return;
         }
@@ -476,8 +497,9 @@ int run_grid(darcy_ctx* ctx, const Job& job) {
             HIP_TRY(darcy_grid_cg(1, &g, nullptr, nullptr, cg_blocks, s), "rz");
             return DARCY_OK;
         }
-        const int key[8] = {n, mixed ? 1 : 0, o.pre_sweeps, o.post_sweeps, o.cycle_index, lT, cg_blocks, dense64 ? 1 : 0};
-        const bool hit = ctx->cycle_exec && ctx->cycle_base == base && std::equal(key, key + 8, ctx->cycle_key);
+        const int key[9] = {n, mixed ? 1 : 0, o.pre_sweeps, o.post_sweeps, o.cycle_index, lT, cg_blocks, dense64 ? 1 : 0,
+                            big_l};
+        const bool hit = ctx->cycle_exec && ctx->cycle_base == base && std::equal(key, key + 9, ctx->cycle_key);
         if (!hit) {
             if (!ctx->graph_checked) {  // every captured kernel within the device's launch limits
                 const char* what = "";
@@ -500,7 +522,7 @@ int run_grid(darcy_ctx* ctx, const Job& job) {
             ctx->cycle_graph = graph;
             HIP_TRY(hipGraphInstantiate(&ctx->cycle_exec, graph, nullptr, nullptr, 0), "graph instantiate");
             ctx->cycle_base = base;
-            std::copy(key, key + 8, ctx->cycle_key);
+            std::copy(key, key + 9, ctx->cycle_key);
         }
         HIP_TRY(hipGraphLaunch(ctx->cycle_exec, s), "graph launch");
         return DARCY_OK;
@@ -796,6 +818,8 @@ extern "C" int darcy_ctx_set_tuning(darcy_ctx* ctx, const char* key, int32_t val
         ctx->batch_step_cells_fp32 = value;
     } else if (k == "grid_fp64_dense_tail" && (value == 0 || value == 1)) {
         ctx->grid_fp64_dense_tail = value;
+    } else if (k == "grid_dense_side" && (value == 8 || value == 16)) {
+        ctx->grid_dense_side = value;
     } else {
         return fail(ctx, DARCY_INVALID_INPUT, "unknown tuning key or value out of range: " + k);
     }
@@ -819,6 +843,8 @@ extern "C" int darcy_ctx_get_tuning(const darcy_ctx* ctx, const char* key, int32
         *value = ctx->batch_step_cells_fp32;
     else if (k == "grid_fp64_dense_tail")
         *value = ctx->grid_fp64_dense_tail;
+    else if (k == "grid_dense_side")
+        *value = ctx->grid_dense_side;
     else
         return DARCY_INVALID_INPUT;
     return DARCY_OK;

@@ -64,6 +64,11 @@ struct darcy_ctx {
     // formed in FP64 from unit vectors); 0 runs the recursion itself, which
     // reproduces the reference's cycle bit for bit.
     int grid_fp64_dense_tail = 1;
+    // Largest side the grid path's tail applies as one dense matrix: 8 (the
+    // levels of side <= 8, kept in LDS) or 16 (the levels from the first of
+    // side <= 16, read from L2).  The FP64 cycle uses either only with
+    // grid_fp64_dense_tail.
+    int grid_dense_side = 16;
     // Device address and size of the library's static scratch array (the
     // allocation-free entry point), looked up when the context is created.
     void* scratch = nullptr;
@@ -75,6 +80,6 @@ struct darcy_ctx {
     hipGraphExec_t cycle_exec = nullptr;
     hipGraph_t cycle_graph = nullptr;
     const void* cycle_base = nullptr;
-    int cycle_key[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int cycle_key[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     bool graph_checked = false;  // launch limits of the captured kernels verified
 };

@@ -82,6 +82,22 @@ __global__ __launch_bounds__(kBlock) void kg_postsmooth(const Hier* hd, int l, i
         gs_colour(L, kind == 1 ? 0 : 1, e);  // 1: red, 2: black
 }
 
+// The levels from lb (8 < side <= 16) down as one dense matrix for the tail
+// kernel (column k at out + k * ld), in FP32 (the FP32 cycle) or FP64: each
+// wave of the launch builds columns with the FP64 recursive cycle.  Scratch:
+// the FP64 operators of levels lb.. (<= 1085 doubles) and x, b of those
+// levels per wave (<= 682 doubles each).
+constexpr int kBigScratchDoubles = 1088 + kWaves * 682;
+template <class OT>
+__global__ __launch_bounds__(kBlock) void kg_big_matrix(const Hier* hd, CycleOpts o, int lb, OT* out, int ld) {
+    __shared__ Hier h;
+    __shared__ __align__(16) double scratch[kBigScratchDoubles];
+    if (threadIdx.x == 0) h = *hd;
+    __syncthreads();
+    build_tail_matrix(h, lb, o, scratch, out, ld, static_cast<int>(blockIdx.x) * kWaves,
+                      static_cast<int>(gridDim.x) * kWaves);
+}
+
 // Largest side of the levels the tail kernel keeps in LDS.  It runs alone on
 // its CU, so it holds far more than a batch workgroup: sides <= 64 in FP32
 // (111 KB with the tail matrix beside it), <= 32 in FP64 (57 KB).  The tail
@@ -137,9 +153,12 @@ DARCY_INLINE void stage_out(P dst, const T* lds, int count, Exec e) {
 // phase computes every cell once, so the count does not change any result.
 constexpr int kTailThreads = 512;
 
+// With `big` (levels from big_l down as one dense matrix in global memory,
+// columns big_ld apart) the levels below big_l are neither staged nor run.
 template <class T>
 __global__ __launch_bounds__(kTailThreads) void kg_tail(const Hier* hd, int l0, int fresh, CycleOpts o,
-                                                        const float* tail_in, const double* tail64_in) {
+                                                        const float* tail_in, const double* tail64_in, const T* big,
+                                                        int big_l, int big_ld) {
     constexpr int NL = TailLdsSide<T>::value;
     __shared__ Shared s;
     __shared__ __align__(16) T pool[LdsPool<T, NL>::kElems];
@@ -157,7 +176,8 @@ __global__ __launch_bounds__(kTailThreads) void kg_tail(const Hier* hd, int l0, 
     if (threadIdx.x == 0) plan_lds<T, NL>(s.h, s.t, l0);
     __syncthreads();
     const Hier& h = s.h;
-    for (int l = s.t.first; l < h.L; ++l) {
+    const int last = big ? big_l + 1 : h.L;  // levels staged into LDS
+    for (int l = s.t.first; l < last; ++l) {
         const GlobalLv<T> g = global_level<T>(h, l);
         const int* f = s.t.off[l];
         const int nf = g.n * (g.n + 1), nc = g.n * g.n;
@@ -169,7 +189,9 @@ __global__ __launch_bounds__(kTailThreads) void kg_tail(const Hier* hd, int l0, 
             if (!fresh) stage_in(pool + f[3], g.x, nc, e);
         }
     }
-    if constexpr (sizeof(T) == 4) {
+    if (big) {
+        // the dense big matrix replaces the small tail matrix
+    } else if constexpr (sizeof(T) == 4) {
         if (s.t.wave < h.L) {
             const int dim = h.n[s.t.wave] * h.n[s.t.wave];
             stage_in(tail, tail_in, dim * dim, e);
@@ -181,7 +203,13 @@ __global__ __launch_bounds__(kTailThreads) void kg_tail(const Hier* hd, int l0, 
         }
     }
     __syncthreads();
-    const Cycle<T> cyc{h, s.t, pool, tail, o, kStepCellsAlone, sizeof(T) == 8 && tail64_in ? tail64 : nullptr};
+    T* scratch = nullptr;  // the small tail matrix's LDS, free when big is used
+    if constexpr (sizeof(T) == 4)
+        scratch = tail;
+    else
+        scratch = tail64;
+    const Cycle<T> cyc{h,      s.t,   pool,   tail, o, kStepCellsAlone, sizeof(T) == 8 && tail64_in ? tail64 : nullptr,
+                       big,    big ? big_l : -1, big_ld, scratch};
     cyc.run(l0, fresh != 0, e);
     if (s.t.first <= l0) {  // the cycle ends with a barrier
         const GlobalLv<T> g = global_level<T>(h, l0);
@@ -409,8 +437,10 @@ template __global__ void kg_restrict<double>(const Hier*, int);
 template __global__ void kg_restrict<float>(const Hier*, int);
 template __global__ void kg_postsmooth<double>(const Hier*, int, int);
 template __global__ void kg_postsmooth<float>(const Hier*, int, int);
-template __global__ void kg_tail<double>(const Hier*, int, int, CycleOpts, const float*, const double*);
-template __global__ void kg_tail<float>(const Hier*, int, int, CycleOpts, const float*, const double*);
+template __global__ void kg_tail<double>(const Hier*, int, int, CycleOpts, const float*, const double*, const double*,
+                                         int, int);
+template __global__ void kg_tail<float>(const Hier*, int, int, CycleOpts, const float*, const double*, const float*, int,
+                                        int);
 
 }  // namespace darcy_hip
 
@@ -462,12 +492,28 @@ extern "C" hipError_t darcy_grid_postsmooth(int mixed, const Hier* hd, int l, in
         hipLaunchKernelGGL(kg_postsmooth<double>, dim3(blocks), dim3(kBlock), 0, s, hd, l, kind);
     return hipGetLastError();
 }
+// big (null: none): the levels from big_l down as one dense matrix, FP32
+// for mixed, FP64 otherwise, columns big_ld elements apart.
 extern "C" hipError_t darcy_grid_tail(int mixed, const Hier* hd, int l0, int fresh, CycleOpts o, const float* tail,
-                                      const double* tail64, hipStream_t s) {
+                                      const double* tail64, const void* big, int big_l, int big_ld, hipStream_t s) {
     if (mixed)
-        hipLaunchKernelGGL(kg_tail<float>, dim3(1), dim3(kTailThreads), 0, s, hd, l0, fresh, o, tail, nullptr);
+        hipLaunchKernelGGL(kg_tail<float>, dim3(1), dim3(kTailThreads), 0, s, hd, l0, fresh, o, tail, nullptr,
+                           static_cast<const float*>(big), big_l, big_ld);
     else
-        hipLaunchKernelGGL(kg_tail<double>, dim3(1), dim3(kTailThreads), 0, s, hd, l0, fresh, o, nullptr, tail64);
+        hipLaunchKernelGGL(kg_tail<double>, dim3(1), dim3(kTailThreads), 0, s, hd, l0, fresh, o, nullptr, tail64,
+                           static_cast<const double*>(big), big_l, big_ld);
+    return hipGetLastError();
+}
+
+// The big tail matrix of levels lb.. (side n_lb <= 16), columns ld apart.
+extern "C" hipError_t darcy_grid_big_matrix(int mixed, const Hier* hd, CycleOpts o, int lb, int dim, void* out, int ld,
+                                            hipStream_t s) {
+    const int blocks = (dim + kWaves - 1) / kWaves;
+    if (mixed)
+        hipLaunchKernelGGL(kg_big_matrix<float>, dim3(blocks), dim3(kBlock), 0, s, hd, o, lb, static_cast<float*>(out), ld);
+    else
+        hipLaunchKernelGGL(kg_big_matrix<double>, dim3(blocks), dim3(kBlock), 0, s, hd, o, lb,
+                           static_cast<double*>(out), ld);
     return hipGetLastError();
 }
 extern "C" hipError_t darcy_grid_cg(int which, const GridCg* g, const double* pold, double* pnew, int blocks,

@@ -42,15 +42,18 @@ def true_residual(a, u, f=1.0):
 @pytest.mark.parametrize("n", [65, 100, 256, 300])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_grid_path_equals_batch_path(n, precision):
-    """With the recursive FP64 cycle (grid_fp64_dense_tail = 0) the grid
-    path gives the persistent kernel's bits; the default dense FP64 tail is
-    checked against it in test_gpu_tuning.py."""
+    """With the recursive FP64 cycle (grid_fp64_dense_tail = 0) and the FP32
+    cycle's dense tail over the levels of side <= 8 (grid_dense_side = 8) the
+    grid path gives the persistent kernel's bits; the default dense tails
+    are checked against them in test_gpu_tuning.py."""
     a = two_phase(n, seed=n)
     darcy._core.set_tuning("grid_fp64_dense_tail", 0)
+    darcy._core.set_tuning("grid_dense_side", 8)  # the FP32 cycle's tail as the persistent kernel runs it
     try:
         single = darcy.solve(a, precision=precision)  # n > 64: grid path
     finally:
         darcy._core.set_tuning("grid_fp64_dense_tail", 1)
+        darcy._core.set_tuning("grid_dense_side", 16)
     us, batch = darcy.solve_batch_uneven([a], precision=precision)  # persistent kernel
     assert single.iterations == batch.iterations[0]
     assert np.array_equal(single.u, us[0])
@@ -139,10 +142,12 @@ def test_grid_path_with_more_sweeps(precision, cycle_index, oracle):
     a = two_phase(150, seed=7)
     kw = dict(pre_sweeps=2, post_sweeps=3, cycle_index=cycle_index, precision=precision)
     darcy._core.set_tuning("grid_fp64_dense_tail", 0)  # the recursive FP64 cycle: the persistent kernel's bits
+    darcy._core.set_tuning("grid_dense_side", 8)  # and the FP32 cycle's tail as the persistent kernel runs it
     try:
         single = darcy.solve(a, **kw)
     finally:
         darcy._core.set_tuning("grid_fp64_dense_tail", 1)
+        darcy._core.set_tuning("grid_dense_side", 16)
     us, batch = darcy.solve_batch_uneven([a], **kw)
     assert single.converged and single.iterations == batch.iterations[0]
     assert np.array_equal(single.u, us[0])

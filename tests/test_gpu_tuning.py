@@ -94,6 +94,7 @@ def test_tail_side_equals_persistent_kernel(precision):
     with capi.Context(apply_tuning=False) as ctx:
         ctx.set_tuning("grid_min_n", 64)
         ctx.set_tuning("grid_fp64_dense_tail", 0)  # the recursive cycle: the persistent kernel's bits
+        ctx.set_tuning("grid_dense_side", 8)  # the FP32 cycle's tail as the persistent kernel runs it
         u_ref, it_ref = _solve_persistent(ctx, a, precision)
         for side in (32, 64, 128):
             for graph in (0, 1):
@@ -164,3 +165,35 @@ def test_fp64_dense_tail_matches_the_recursive_cycle(n, oracle):
         assert ctx.get_tuning("grid_fp64_dense_tail") == 1
     if n <= 300:
         assert rel_l2(u, oracle.solve(a).u) <= 1e-9
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("n", [130, 300, 1024])
+def test_dense_side_16_matches_side_8(precision, n, oracle):
+    """The grid path's tail from the first level of side <= 16 down as one
+    dense matrix read from L2 (grid_dense_side = 16, the default; n = 130 and
+    300 reach that level at sides 9 and 10, 1024 at 16) applies the same
+    linear map as the dense tail over the levels of side <= 8, rounded
+    differently: iteration counts within one, solutions within 1e-10 of each
+    other and within 1e-9 (FP64) / 1e-8 (FP32 cycle) of the oracle's; graph
+    replay gives the eager launches' bits."""
+    from conftest import rel_l2
+
+    a = two_phase(n, seed=23)
+    with capi.Context(apply_tuning=False) as ctx:
+        ctx.set_tuning("grid_min_n", 64)
+        ctx.set_tuning("grid_dense_side", 8)
+        u8, it8 = _solve(ctx, a, precision)
+        ctx.set_tuning("grid_dense_side", 16)
+        runs = []
+        for graph in (0, 1):
+            print(f"dense side 16 n={n} {precision} graph {graph}", flush=True)
+            ctx.set_tuning("grid_graph", graph)
+            runs.append(_solve(ctx, a, precision))
+        assert ctx.get_tuning("grid_dense_side") == 16
+    (u_e, it_e), (u_g, it_g) = runs
+    assert it_e == it_g and np.array_equal(u_e, u_g)
+    assert abs(it_e - it8) <= 1
+    assert rel_l2(u_e, u8) <= 1e-10
+    if n <= 300:
+        assert rel_l2(u_e, oracle.solve(a).u) <= (1e-9 if precision == "fp64" else 1e-8)
