@@ -44,7 +44,7 @@ BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 matrix peak
 # committed weights): rocprofv3 FETCH_SIZE (x 2) + WRITE_SIZE (x 1), the
 # factors that the same counters give for the library's copy kernel of 1 GiB.
 # Reported for that configuration only.
-TRAFFIC_DEFAULT_BYTES = 416.8e9
+TRAFFIC_DEFAULT_BYTES = 410.4e9
 SEED0 = 20000
 
 # Algorithmic bytes per fine cell (counting rule: each array a kernel must
@@ -196,6 +196,43 @@ def suite_large(args) -> int:
     return 0
 
 
+def suite_phases(args) -> int:
+    """Where an iteration of the persistent kernel goes: the default batch
+    solved for a fixed number of iterations (tol far below reach), with the
+    multigrid cycle and with no preconditioner (z = r, one extra copy pass),
+    in each precision; (time at 21 iterations - time at 1) / 20 is the cost
+    of one iteration, kernel event time, best of 3."""
+    from darcy.capi import Context
+
+    n, batch = args.n, args.batch
+    a = media(n, batch, SEED0)
+    ctx = Context(0)
+    cells = n * n
+    ab, ub = ctx.alloc(a.nbytes), ctx.alloc(a.nbytes)
+    ctx.upload(ab, a)
+    ap = [ab + 8 * cells * i for i in range(batch)]
+    up = [ub + 8 * cells * i for i in range(batch)]
+    out = {}
+    for precision in ("mixed", "fp64"):
+        for pc in ("multigrid", "none"):
+            ms = {}
+            for it in (1, 21):
+                best = None
+                for _ in range(3):
+                    ctx.solve_batch_device([n] * batch, ap, up, tol=1e-300, max_iter=it, preconditioner=pc,
+                                           precision=precision, cycle_index=args.cycle)
+                    t = ctx.last_kernel_ms()
+                    best = t if best is None else min(best, t)
+                ms[it] = best
+            out[f"{precision}/{pc}"] = {"ms_per_iteration": round((ms[21] - ms[1]) / 20, 3),
+                                        "ms_setup_and_first": round(ms[1], 3)}
+    ctx.free(ab)
+    ctx.free(ub)
+    ctx.close()
+    print(json.dumps({"suite": "phases", "problems": batch, "n": n, **out}), flush=True)
+    return 0
+
+
 def uneven_sizes(count: int) -> np.ndarray:
     """Grid sides of the uneven batch: default_rng(0).choice over the seven sizes."""
     return np.random.default_rng(0).choice(np.array([64, 96, 128, 192, 256, 384, 512]), size=count)
@@ -257,7 +294,7 @@ def suite_uneven(args) -> int:
 
 def main() -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--suite", choices=["metric", "large", "uneven"], default="metric")
+    ap.add_argument("--suite", choices=["metric", "large", "uneven", "phases"], default="metric")
     ap.add_argument("--sizes", type=int, nargs="+", default=[1024, 4096])
     ap.add_argument("--oracle", action="store_true", help="large suite: compare with the CPU oracle")
     ap.add_argument("--media", nargs="+", choices=["two_phase_12_3", "contrast_1e6"], default=None,
@@ -289,6 +326,8 @@ def main() -> int:
         return suite_large(args)
     if args.suite == "uneven":
         return suite_uneven(args)
+    if args.suite == "phases":
+        return suite_phases(args)
     if args.copy_only:
         with Context(0) as c:
             print(json.dumps({"copy_MiB": args.copy_only, "GBs": c.bandwidth_probe(args.copy_only << 20, 5)}))

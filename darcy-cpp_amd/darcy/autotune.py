@@ -121,8 +121,11 @@ def tune(ctx: capi.Context, sizes: Iterable[int], small: Iterable[int], repeats:
         _, tail, graph = best
         ctx.set_tuning("grid_tail_side", tail)
         ctx.set_tuning("grid_graph", graph)
-        # 2. Where single solves switch from the persistent kernel to the grid path.
-        switch = 1
+        # 2. Where single solves switch from the persistent kernel to the grid
+        #    path: the largest tried size at which the persistent kernel is
+        #    still as fast, and never below the smallest tried size minus one
+        #    (sizes below it stay on the persistent kernel untimed).
+        switch = min(small) - 1
         for n in sorted(small):
             ctx.set_tuning("grid_min_n", n)  # n itself: persistent kernel
             t_batch = _time_solve(ctx, n, *bufs[n], precision, repeats)
@@ -161,7 +164,7 @@ def tune(ctx: capi.Context, sizes: Iterable[int], small: Iterable[int], repeats:
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(prog="python -m darcy.autotune")
     ap.add_argument("--sizes", type=int, nargs="+", default=[256, 512, 1024, 2048])
-    ap.add_argument("--small", type=int, nargs="+", default=[64, 96, 128, 192, 256])
+    ap.add_argument("--small", type=int, nargs="+", default=[65, 96, 128, 192, 256])
     ap.add_argument("--repeats", type=int, default=3)
     ap.add_argument("--batch-n", type=int, default=256)
     ap.add_argument("--batch-count", type=int, default=1000)
