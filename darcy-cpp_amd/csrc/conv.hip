@@ -20,7 +20,8 @@
 //
 // The weight gradient splits K (pixels) over blockIdx.z and sums the FP32
 // partials in split order, so gradients do not depend on scheduling.  The
-// one-channel head is too narrow for the matrix cores and runs in FP32.
+// head keeps FP32 weights and runs on the matrix cores as two bf16 parts
+// (k_head_mfma); its gradients are FP32 kernels.
 // Activations are NHWC bf16; weights W[co][tap][ci] bf16 from FP32 masters.
 // No reference counterpart: the reference solve always starts from u = 0.
 #include <hip/hip_runtime.h>
@@ -771,31 +772,8 @@ __global__ __launch_bounds__(kT) void k_reduce_splits(const float* part, int spl
 // sub = 1, C = 1 and yc is the output y itself.  With sub = 2, C = 5 and
 // the output map below turns yc into the n x n field y.
 
-__global__ __launch_bounds__(kT) void k_head_fwd(const bf16* x, Geo in, int C, const float* w, const float* bias,
-                                                 float* yc) {
-    const size_t P = static_cast<size_t>(in.b) * in.h * in.w;
-    for (size_t p = blockIdx.x * static_cast<size_t>(kT) + threadIdx.x; p < P; p += static_cast<size_t>(gridDim.x) * kT) {
-        const int ox = static_cast<int>(p % in.w), t = static_cast<int>(p / in.w), oy = t % in.h, b = t / in.h;
-        for (int r = 0; r < C; ++r) {
-            float s = bias[r];
-            for (int tap = 0; tap < 9; ++tap) {
-                const int iy = oy + tap / 3 - 1, ix = ox + tap % 3 - 1;
-                if (iy < 0 || iy >= in.h || ix < 0 || ix >= in.w) continue;
-                const bf16* xi = x + at(in, b, iy, ix);
-                const float* wt = w + (r * 9 + tap) * in.c;
-                for (int ci = 0; ci < in.c; ci += 8) {
-                    const v8bf v = *reinterpret_cast<const v8bf*>(xi + ci);
-#pragma unroll
-                    for (int t8 = 0; t8 < 8; ++t8) s += wt[ci + t8] * bf2f(v[t8]);
-                }
-            }
-            yc[p * C + r] = s;
-        }
-    }
-}
-
-// Where the head writes the warm start itself (u0 = f scale pos y in FP64,
-// as k_output forms it) instead of y.
+// Where the head (or the half-resolution output map) writes the warm start
+// itself, u0 = f scale pos y in FP64, instead of y.
 struct HeadOut {
     const float* pos;
     const double* f;
@@ -1170,16 +1148,6 @@ __global__ __launch_bounds__(kT) void k_transpose(const bf16* in, int P, int C, 
         if (c0 + y < C && p0 + tx < P) out[static_cast<size_t>(c0 + y) * P + p0 + tx] = tile[tx][y];
 }
 
-// u0 = f * scale * pos * y, FP64 per problem.
-__global__ __launch_bounds__(kT) void k_output(const float* y, const float* pos, int n, int batch, const double* f,
-                                               double scale, double* const* u0) {
-    const size_t per = static_cast<size_t>(n) * n;
-    for (size_t q = blockIdx.x * static_cast<size_t>(kT) + threadIdx.x; q < per * batch; q += static_cast<size_t>(gridDim.x) * kT) {
-        const size_t b = q / per, k = q % per;
-        u0[b][k] = f[b] * scale * static_cast<double>(pos[k] * y[q]);
-    }
-}
-
 // Training loss against t = u / scale with e = pos y - t:
 //   L = (sum_cells e^2 + lam e^T A e) / cells,
 // A the finite-volume operator of the problem's permeability (harmonic
@@ -1300,10 +1268,10 @@ using namespace darcy_nn;
 
 // ------------------------------------------------------------ launchers
 // All pointers are device pointers; shapes were checked by the caller.
-// The head conv: x [b][h][w][cin] -> yc [b][h][w][C], C 1 or 5.  With the
-// hi / lo weights whl (darcy_nn_head_split), 32 channels and a tileable
-// image it runs on the matrix cores (k_head_mfma), else in FP32 one pixel
-// per thread.  With u0 (C = 1 only) the head stores the warm start itself.
+// The head conv: x [b][h][w][cin] -> yc [b][h][w][C], C 1 or 5, on the
+// matrix cores (k_head_mfma) with the hi / lo weights whl
+// (darcy_nn_head_split); 32 channels and a side that is a multiple of 32.
+// With u0 (C = 1 only) the head stores the warm start itself.
 template <int C>
 static bool launch_head_mfma(const bf16* x, Geo in, const bf16* whl, const float* bias, float* yc, HeadOut out,
                       hipStream_t s) {
@@ -1318,20 +1286,16 @@ static bool launch_head_mfma(const bf16* x, Geo in, const bf16* whl, const float
     return true;
 }
 
-static hipError_t head_conv(const void* x, Geo in, int C, const float* wgt, const void* whl, const float* bias, float* yc,
-                     HeadOut out, hipStream_t s) {
+static hipError_t head_conv(const void* x, Geo in, int C, const void* whl, const float* bias, float* yc, HeadOut out,
+                            hipStream_t s) {
     if (C != 1 && C != 5) return hipErrorInvalidValue;
     const bf16* xb = static_cast<const bf16*>(x);
     const bf16* hl = static_cast<const bf16*>(whl);
     const bool done = C == 1 ? launch_head_mfma<1>(xb, in, hl, bias, yc, out, s)
                              : launch_head_mfma<5>(xb, in, hl, bias, yc, out, s);
-    if (done) return hipGetLastError();
-    k_head_fwd<<<ew_grid(static_cast<size_t>(in.b) * in.h * in.w), kT, 0, s>>>(xb, in, C, wgt, bias, yc);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess || !out.u0) return e;
-    k_output<<<ew_grid(static_cast<size_t>(in.h) * in.w * in.b), kT, 0, s>>>(yc, out.pos, in.h, in.b, out.f, out.scale,
-                                                                             out.u0);
-    return hipGetLastError();
+    // The network's finest level always has 32 channels and a side that is a
+    // multiple of 64, so every head it launches fits the tiles.
+    return done ? hipGetLastError() : hipErrorInvalidValue;
 }
 
 extern "C" {
@@ -1404,9 +1368,9 @@ hipError_t darcy_nn_conv_bwd_weight(const void* x, int b, int h, int w, int cin,
 }
 
 // x: [b][h][w][cin] -> yc [b][h][w][C] (head_conv).
-hipError_t darcy_nn_head_fwd(const void* x, int b, int h, int w, int cin, int C, const float* wgt, const void* whl,
-                             const float* bias, float* yc, hipStream_t s) {
-    return head_conv(x, Geo{b, h, w, cin}, C, wgt, whl, bias, yc, HeadOut{nullptr, nullptr, 0.0, nullptr}, s);
+hipError_t darcy_nn_head_fwd(const void* x, int b, int h, int w, int cin, int C, const void* whl, const float* bias,
+                             float* yc, hipStream_t s) {
+    return head_conv(x, Geo{b, h, w, cin}, C, whl, bias, yc, HeadOut{nullptr, nullptr, 0.0, nullptr}, s);
 }
 
 // hi / lo parts of the head's count FP32 weights: whl [2][count] bf16.
@@ -1493,27 +1457,21 @@ hipError_t darcy_nn_transpose(const void* in, int P, int C, void* out, hipStream
     return hipGetLastError();
 }
 
-hipError_t darcy_nn_output(const float* y, const float* pos, int n, int batch, const double* f, double scale,
-                           double* const* u0, hipStream_t s) {
-    k_output<<<ew_grid(static_cast<size_t>(n) * n * batch), kT, 0, s>>>(y, pos, n, batch, f, scale, u0);
-    return hipGetLastError();
-}
-
 // The head and the output in one: u0 = f scale pos y for n x n problems, x
 // the finest network level (side n / sub).  sub 1: the head stores u0
 // itself; sub 2: the 5-channel head writes yc, then the output map forms u0.
-hipError_t darcy_nn_head_output(const void* x, int b, int n, int cin, int sub, const float* wgt, const void* whl,
-                                const float* bias, float* yc, const float* pos, const double* f, double scale,
-                                double* const* u0, hipStream_t s) {
+hipError_t darcy_nn_head_output(const void* x, int b, int n, int cin, int sub, const void* whl, const float* bias,
+                                float* yc, const float* pos, const double* f, double scale, double* const* u0,
+                                hipStream_t s) {
     if (sub != 1 && sub != 2) return hipErrorInvalidValue;
     if (sub == 2) {
-        hipError_t e = darcy_nn_head_fwd(x, b, n / 2, n / 2, cin, 5, wgt, whl, bias, yc, s);
+        hipError_t e = darcy_nn_head_fwd(x, b, n / 2, n / 2, cin, 5, whl, bias, yc, s);
         if (e != hipSuccess) return e;
         k_sub2_out<<<ew_grid(static_cast<size_t>(n / 2) * (n / 2) * b), kT, 0, s>>>(yc, n, b, nullptr,
                                                                                     HeadOut{pos, f, scale, u0});
         return hipGetLastError();
     }
-    return head_conv(x, Geo{b, n, n, cin}, 1, wgt, whl, bias, yc, HeadOut{pos, f, scale, u0}, s);
+    return head_conv(x, Geo{b, n, n, cin}, 1, whl, bias, yc, HeadOut{pos, f, scale, u0}, s);
 }
 
 // partial: `blocks` floats.

@@ -25,8 +25,8 @@ hipError_t darcy_nn_conv_bwd_weight(const void* x, int b, int h, int w, int cin,
                                     int splits, float* part, float* dw, float* db, hipStream_t s);
 hipError_t darcy_nn_conv_fwd_merge(const void* x, int b, int h, int w, int cin, const void* wgt, const float* bias,
                                    int cout, const void* skip, void* z, void* act, hipStream_t s);
-hipError_t darcy_nn_head_fwd(const void* x, int b, int h, int w, int cin, int C, const float* wgt, const void* whl,
-                             const float* bias, float* yc, hipStream_t s);
+hipError_t darcy_nn_head_fwd(const void* x, int b, int h, int w, int cin, int C, const void* whl, const float* bias,
+                             float* yc, hipStream_t s);
 hipError_t darcy_nn_head_split(const float* w, int count, void* whl, hipStream_t s);
 hipError_t darcy_nn_head_bwd_data(const float* dyc, int b, int h, int w, int cin, int C, const float* wgt, void* dx,
                                   hipStream_t s);
@@ -43,11 +43,9 @@ hipError_t darcy_nn_gelu_bwd(void* d, const void* z, size_t count, hipStream_t s
 hipError_t darcy_nn_down_sum(const void* dz, int b, int h, int w, int ch, void* dc, hipStream_t s);
 hipError_t darcy_nn_add(void* dst, const void* src, size_t count, hipStream_t s);
 hipError_t darcy_nn_transpose(const void* in, int P, int C, void* out, hipStream_t s);
-hipError_t darcy_nn_output(const float* y, const float* pos, int n, int batch, const double* f, double scale,
-                           double* const* u0, hipStream_t s);
-hipError_t darcy_nn_head_output(const void* x, int b, int n, int cin, int sub, const float* wgt, const void* whl,
-                                const float* bias, float* yc, const float* pos, const double* f, double scale,
-                                double* const* u0, hipStream_t s);
+hipError_t darcy_nn_head_output(const void* x, int b, int n, int cin, int sub, const void* whl, const float* bias,
+                                float* yc, const float* pos, const double* f, double scale, double* const* u0,
+                                hipStream_t s);
 hipError_t darcy_nn_loss(const float* y, const double* const* u, const double* const* a, const float* pos, int n,
                          int batch, double scale, float lam, float* dy, float* partial, int blocks, hipStream_t s);
 hipError_t darcy_nn_adam(float* w, const float* g, float* m, float* v, void* wb, size_t count, float lr, float b1,
@@ -292,13 +290,13 @@ int darcy_net::forward(int b, int64_t offset, bool to_u0) {
         x = am[i].ptr;
     }
     if (to_u0) {
-        NN_TRY(darcy_nn_head_output(x, b, n, kHeadCin, sub, params.as<float>() + lay->w[kConvs], whl.ptr,
-                                    params.as<float>() + lay->b[kConvs], head_out(), pos.as<float>(),
+        NN_TRY(darcy_nn_head_output(x, b, n, kHeadCin, sub, whl.ptr, params.as<float>() + lay->b[kConvs],
+                                    head_out(), pos.as<float>(),
                                     tab_f.as<double>() + offset, kScale, tab_u.as<double*>() + offset, s),
                "net head");
     } else {
-        NN_TRY(darcy_nn_head_fwd(x, b, side(0), side(0), kHeadCin, hc(), params.as<float>() + lay->w[kConvs],
-                                 whl.ptr, params.as<float>() + lay->b[kConvs], head_out(), s),
+        NN_TRY(darcy_nn_head_fwd(x, b, side(0), side(0), kHeadCin, hc(), whl.ptr, params.as<float>() + lay->b[kConvs],
+                                 head_out(), s),
                "net head");
         if (sub != 1) NN_TRY(darcy_nn_sub2_out(yc.as<float>(), n, b, y.as<float>(), s), "net head");
     }

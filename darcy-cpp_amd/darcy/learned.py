@@ -227,26 +227,43 @@ class DeviceSet:
 
 
 def train(ctx: capi.Context, data: DeviceSet, epochs: int, batch: int = 32, lr: float = 1e-3, seed: int = 0,
-          log=print, sub: int = 1) -> tuple[np.ndarray, list]:
-    """Adam with linear warm-up and cosine decay; returns (params, history)."""
+          log=print, sub: int = 1, spike: float = 4.0) -> tuple[np.ndarray, list]:
+    """Adam with linear warm-up and cosine decay; returns (params, history).
+
+    Rollback: the parameters at the end of each epoch whose mean loss is the
+    lowest so far are kept; an epoch whose mean loss exceeds `spike` times
+    that (or is not finite) is undone -- the kept parameters are restored,
+    Adam's moments restart from zero and the rate is halved for the rest of
+    the run.  Training under the energy-norm loss otherwise collapses now
+    and then after a loss spike to a near-zero output it never leaves."""
     net = capi.Net(ctx, data.n, batch, train=True, sub=sub)
     net.set_params(init_params(seed, sub))
     rng = np.random.default_rng(seed)
     steps_per_epoch = data.count // batch
     total, warm = epochs * steps_per_epoch, min(500, epochs * steps_per_epoch // 10)
     history, step, t0 = [], 0, time.perf_counter()
+    scale, best_loss, best_params = 1.0, math.inf, None
     for ep in range(epochs):
         order = rng.permutation(data.count)
         losses = []
         for s in range(steps_per_epoch):
             idx = order[s * batch:(s + 1) * batch]
-            rate = lr * min(1.0, (step + 1) / max(1, warm)) * 0.5 * (1.0 + math.cos(math.pi * step / total))
+            rate = scale * lr * min(1.0, (step + 1) / max(1, warm)) * 0.5 * (1.0 + math.cos(math.pi * step / total))
             want = s % 50 == 0 or s == steps_per_epoch - 1
             loss = net.train_step([data.a[i] for i in idx], [data.u[i] for i in idx], rate, want_loss=want)
             if want:
                 losses.append(loss)
             step += 1
-        history.append({"epoch": ep + 1, "loss": float(np.mean(losses)), "seconds": time.perf_counter() - t0})
+        mean = float(np.mean(losses))
+        entry = {"epoch": ep + 1, "loss": mean, "seconds": time.perf_counter() - t0}
+        if best_params is not None and not (mean <= spike * best_loss):
+            net.set_params(best_params)  # also zeroes Adam's moments
+            scale *= 0.5
+            entry["rolled_back"] = True
+            entry["rate_scale"] = scale
+        elif mean < best_loss:
+            best_loss, best_params = mean, net.get_params()
+        history.append(entry)
         log(json.dumps(history[-1]))
     params = net.get_params()
     net.close()
