@@ -227,7 +227,7 @@ class DeviceSet:
 
 
 def train(ctx: capi.Context, data: DeviceSet, epochs: int, batch: int = 32, lr: float = 1e-3, seed: int = 0,
-          log=print, sub: int = 1, spike: float = 4.0) -> tuple[np.ndarray, list]:
+          log=print, sub: int = 1, spike: float = 10.0) -> tuple[np.ndarray, list]:
     """Adam with linear warm-up and cosine decay; returns (params, history).
 
     Rollback: the parameters at the end of each epoch whose mean loss is the
@@ -235,7 +235,9 @@ def train(ctx: capi.Context, data: DeviceSet, epochs: int, batch: int = 32, lr: 
     that (or is not finite) is undone -- the kept parameters are restored,
     Adam's moments restart from zero and the rate is halved for the rest of
     the run.  Training under the energy-norm loss otherwise collapses now
-    and then after a loss spike to a near-zero output it never leaves."""
+    and then after a loss spike to a near-zero output it never leaves (the
+    first epoch of such a collapse sits 80-500 times above the best;
+    spikes the run recovers from by itself stay below 5 times)."""
     net = capi.Net(ctx, data.n, batch, train=True, sub=sub)
     net.set_params(init_params(seed, sub))
     rng = np.random.default_rng(seed)

@@ -39,7 +39,7 @@ def test_committed_weights_load():
 
 
 def test_training_rolls_back_a_loss_spike(monkeypatch):
-    """A loss spike above 4x the best epoch's loss restores that epoch's
+    """A loss spike above 10x the best epoch's loss restores that epoch's
     parameters (which also restarts Adam) and halves the rate; the host
     logic of learned.train, driven by a stand-in network."""
     rates, restored = [], []
