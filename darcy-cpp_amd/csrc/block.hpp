@@ -529,6 +529,10 @@ struct Cycle {
     const T* big = nullptr;
     int big_l = -1, big_ld = 0;
     T* big_scratch = nullptr;
+    // big holds the W-cycle's pair of visits of big_l (build_tail_matrix's
+    // pair): the first visit applies it from zero and the second is skipped.
+    // Valid only with big_l > the level run() starts from.
+    bool big_pair = false;
 
     DARCY_INLINE GlobalLv<T> G(int l) const { return global_level<T>(h, l); }
     DARCY_INLINE LvP<T> S(int l) const {
@@ -694,6 +698,7 @@ struct Cycle {
             }
             if (l == big_l) {
                 dense_big(l, fresh, e);
+                if (big_pair) done |= 1u << (l - 1);  // both visits applied
             } else if (l >= t.wave) {
                 if (threadIdx.x < kWave) {
                     const Exec we{static_cast<int>(threadIdx.x), kWave};
@@ -734,9 +739,12 @@ struct Cycle {
 // col_step); `scratch` (the LDS pool, before it receives the operators, or
 // an array of its own) holds the FP64 operators of the tail levels and each
 // wave's x and b.
+// pair: column k is the W-cycle's two visits of level tl instead -- the
+// cycle from zero, then the cycle again from its result with the same
+// right-hand side -- which is the matrix 2M - M A M of the single cycle M.
 template <class OT>
 DARCY_INLINE void build_tail_matrix(const Hier& h, int tl, const CycleOpts& o, void* scratch, OT* tail, int ld = 0,
-                                    int col0 = 0, int col_step = kWaves) {
+                                    int col0 = 0, int col_step = kWaves, bool pair = false) {
     double* dp = static_cast<double*>(scratch);
     const int L = h.L, lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
     auto op_off = [&](int l) {
@@ -772,6 +780,7 @@ DARCY_INLINE void build_tail_matrix(const Hier& h, int tl, const CycleOpts& o, v
         for (int q = lane; q < dim; q += kWave) top.b[q] = q == k ? 1.0 : 0.0;
         sync();
         cycle_levels<double>(tl, L, true, o, Exec{lane, kWave}, sync, view);
+        if (pair) cycle_levels<double>(tl, L, false, o, Exec{lane, kWave}, sync, view);
         for (int q = lane; q < ld; q += kWave) tail[static_cast<size_t>(k) * ld + q] = q < dim ? static_cast<OT>(top.x[q]) : OT(0);
         sync();
     }

@@ -30,7 +30,7 @@ extern "C" hipError_t darcy_grid_tail_matrix64(const Hier* hd, CycleOptsC o, dou
 extern "C" hipError_t darcy_grid_presmooth(int mixed, const Hier* hd, int l, int kind, int blocks, hipStream_t s);
 extern "C" hipError_t darcy_grid_restrict(int mixed, const Hier* hd, int l, int blocks, hipStream_t s);
 extern "C" hipError_t darcy_grid_postsmooth(int mixed, const Hier* hd, int l, int kind, int blocks, hipStream_t s);
-extern "C" hipError_t darcy_grid_tail(int mixed, const Hier* hd, int l0, int fresh, CycleOptsC o, const float* tail,
+extern "C" hipError_t darcy_grid_tail(int mixed, const Hier* hd, int l0, int mode, CycleOptsC o, const float* tail,
                                       const double* tail64, const void* big, int big_l, int big_ld, hipStream_t s);
 extern "C" hipError_t darcy_grid_big_matrix(int mixed, const Hier* hd, CycleOptsC o, int lb, int dim, void* out, int ld,
                                             hipStream_t s);
@@ -440,14 +440,17 @@ int run_grid(darcy_ctx* ctx, const Job& job) {
 
     // The cycle: large levels phase by phase, the tail in one workgroup.
     int rc = DARCY_OK;
-    std::function<void(int, bool)> cycle = [&](int l, bool fresh) {
+    // mode 1: from x = 0; 0: a repeated visit; 2 (the tail only): both
+    // visits of the W-cycle in one launch.
+    std::function<void(int, int)> cycle = [&](int l, int mode) {
         if (rc) return;
         if (l >= lT) {
-            const hipError_t e = darcy_grid_tail(mixed, hd, l, fresh ? 1 : 0, co, tail, dense64 ? tail64 : nullptr,
+            const hipError_t e = darcy_grid_tail(mixed, hd, l, mode, co, tail, dense64 ? tail64 : nullptr,
                                                  big_l > 0 ? big : nullptr, big_l, big_ld, s);
             if (e != hipSuccess) rc = hip_fail(ctx, e, "tail cycle");
             return;
         }
+        const bool fresh = mode == 1;
         const int64_t nl = static_cast<int64_t>(h.n[l]) * h.n[l];
         // 512 cells per workgroup: the small levels are latency-bound, and
         // the pre-smoothing walk takes as many rows per lane as the launch
@@ -468,8 +471,12 @@ int run_grid(darcy_ctx* ctx, const Job& job) {
             rc = hip_fail(ctx, e, "cycle launch");
             return;
         }
-        cycle(l + 1, true);
-        if (o.cycle_index == 2) cycle(l + 1, false);
+        if (o.cycle_index == 2 && l + 1 >= lT) {
+            cycle(l + 1, 2);
+        } else {
+            cycle(l + 1, 1);
+            if (o.cycle_index == 2) cycle(l + 1, 0);
+        }
         if (rc) return;
         e = darcy_grid_postsmooth(mixed, hd, l, 0, bl, s);
         if (e == hipSuccess) e = darcy_grid_postsmooth(mixed, hd, l, 1, bl, s);
@@ -492,7 +499,7 @@ int run_grid(darcy_ctx* ctx, const Job& job) {
             return DARCY_OK;
         }
         if (!ctx->grid_graph || lT == 0) {
-            cycle(0, true);
+            cycle(0, 1);
             if (rc) return rc;
             HIP_TRY(darcy_grid_cg(1, &g, nullptr, nullptr, cg_blocks, s), "rz");
             return DARCY_OK;
@@ -512,7 +519,7 @@ int run_grid(darcy_ctx* ctx, const Job& job) {
             ctx->cycle_exec = nullptr;
             ctx->cycle_graph = nullptr;
             HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "graph capture");
-            cycle(0, true);
+            cycle(0, 1);
             const hipError_t e1 = rc ? hipErrorUnknown : darcy_grid_cg(1, &g, nullptr, nullptr, cg_blocks, s);
             hipGraph_t graph = nullptr;
             const hipError_t e2 = hipStreamEndCapture(s, &graph);

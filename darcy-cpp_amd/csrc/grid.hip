@@ -84,7 +84,9 @@ __global__ __launch_bounds__(kBlock) void kg_postsmooth(const Hier* hd, int l, i
 
 // The levels from lb (8 < side <= 16) down as one dense matrix for the tail
 // kernel (column k at out + k * ld), in FP32 (the FP32 cycle) or FP64: each
-// wave of the launch builds columns with the FP64 recursive cycle.  Scratch:
+// wave of the launch builds columns with the FP64 recursive cycle; with the
+// W-cycle, the matrix of both visits of lb (build_tail_matrix's pair), which
+// the tail then applies once per pair.  Scratch:
 // the FP64 operators of levels lb.. (<= 1085 doubles) and x, b of those
 // levels per wave (<= 682 doubles each).
 constexpr int kBigScratchDoubles = 1088 + kWaves * 682;
@@ -95,7 +97,7 @@ __global__ __launch_bounds__(kBlock) void kg_big_matrix(const Hier* hd, CycleOpt
     if (threadIdx.x == 0) h = *hd;
     __syncthreads();
     build_tail_matrix(h, lb, o, scratch, out, ld, static_cast<int>(blockIdx.x) * kWaves,
-                      static_cast<int>(gridDim.x) * kWaves);
+                      static_cast<int>(gridDim.x) * kWaves, o.gamma == 2);
 }
 
 // Largest side of the levels the tail kernel keeps in LDS.  It runs alone on
@@ -154,9 +156,13 @@ DARCY_INLINE void stage_out(P dst, const T* lds, int count, Exec e) {
 constexpr int kTailThreads = 512;
 
 // With `big` (levels from big_l down as one dense matrix in global memory,
-// columns big_ld apart) the levels below big_l are neither staged nor run.
+// columns big_ld apart; with the W-cycle, the matrix of both visits of big_l,
+// big_l > l0) the levels below big_l are neither staged nor run.
+// mode 1: the cycle from x = 0; 0: a repeated visit from x as stored; 2:
+// both, one after the other (the W-cycle's two visits of l0, in one launch:
+// the operators are staged once, and x of an LDS level stays there).
 template <class T>
-__global__ __launch_bounds__(kTailThreads) void kg_tail(const Hier* hd, int l0, int fresh, CycleOpts o,
+__global__ __launch_bounds__(kTailThreads) void kg_tail(const Hier* hd, int l0, int mode, CycleOpts o,
                                                         const float* tail_in, const double* tail64_in, const T* big,
                                                         int big_l, int big_ld) {
     constexpr int NL = TailLdsSide<T>::value;
@@ -186,7 +192,7 @@ __global__ __launch_bounds__(kTailThreads) void kg_tail(const Hier* hd, int l0, 
         stage_in(pool + f[2], g.id, nc, e);
         if (l == l0) {
             stage_in(pool + f[4], g.b, nc, e);
-            if (!fresh) stage_in(pool + f[3], g.x, nc, e);
+            if (mode == 0) stage_in(pool + f[3], g.x, nc, e);
         }
     }
     if (big) {
@@ -209,8 +215,9 @@ __global__ __launch_bounds__(kTailThreads) void kg_tail(const Hier* hd, int l0, 
     else
         scratch = tail64;
     const Cycle<T> cyc{h,      s.t,   pool,   tail, o, kStepCellsAlone, sizeof(T) == 8 && tail64_in ? tail64 : nullptr,
-                       big,    big ? big_l : -1, big_ld, scratch};
-    cyc.run(l0, fresh != 0, e);
+                       big,    big ? big_l : -1, big_ld, scratch, big != nullptr && o.gamma == 2};
+    cyc.run(l0, mode != 0, e);
+    if (mode == 2) cyc.run(l0, false, e);  // the cycle ends with a barrier
     if (s.t.first <= l0) {  // the cycle ends with a barrier
         const GlobalLv<T> g = global_level<T>(h, l0);
         stage_out(g.x, pool + s.t.off[l0][3], g.n * g.n, e);
@@ -494,13 +501,15 @@ extern "C" hipError_t darcy_grid_postsmooth(int mixed, const Hier* hd, int l, in
 }
 // big (null: none): the levels from big_l down as one dense matrix, FP32
 // for mixed, FP64 otherwise, columns big_ld elements apart.
-extern "C" hipError_t darcy_grid_tail(int mixed, const Hier* hd, int l0, int fresh, CycleOpts o, const float* tail,
+// mode: kg_tail's (1 fresh, 0 repeated, 2 both visits of the W-cycle).
+extern "C" hipError_t darcy_grid_tail(int mixed, const Hier* hd, int l0, int mode, CycleOpts o, const float* tail,
                                       const double* tail64, const void* big, int big_l, int big_ld, hipStream_t s) {
+    if (mode < 0 || mode > 2 || (big && big_l <= l0)) return hipErrorInvalidValue;
     if (mixed)
-        hipLaunchKernelGGL(kg_tail<float>, dim3(1), dim3(kTailThreads), 0, s, hd, l0, fresh, o, tail, nullptr,
+        hipLaunchKernelGGL(kg_tail<float>, dim3(1), dim3(kTailThreads), 0, s, hd, l0, mode, o, tail, nullptr,
                            static_cast<const float*>(big), big_l, big_ld);
     else
-        hipLaunchKernelGGL(kg_tail<double>, dim3(1), dim3(kTailThreads), 0, s, hd, l0, fresh, o, nullptr, tail64,
+        hipLaunchKernelGGL(kg_tail<double>, dim3(1), dim3(kTailThreads), 0, s, hd, l0, mode, o, nullptr, tail64,
                            static_cast<const double*>(big), big_l, big_ld);
     return hipGetLastError();
 }
