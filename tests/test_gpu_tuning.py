@@ -15,12 +15,12 @@ capi = pytest.importorskip("darcy.capi")
 autotune = pytest.importorskip("darcy.autotune")
 
 
-def _solve(ctx, a, precision):
+def _solve(ctx, a, precision, cycle_index=2):
     n = a.shape[0]
     pa, pu = ctx.alloc(a.nbytes), ctx.alloc(a.nbytes)
     try:
         ctx.upload(pa, a)
-        st = ctx.solve_device(n, pa, pu, precision=precision)
+        st = ctx.solve_device(n, pa, pu, precision=precision, cycle_index=cycle_index)
         u = np.empty_like(a)
         ctx.download(u, pu)
     finally:
@@ -172,11 +172,12 @@ def test_fp64_dense_tail_matches_the_recursive_cycle(n, oracle):
 def test_dense_side_16_matches_side_8(precision, n, oracle):
     """The grid path's tail from the first level of side <= 16 down as one
     dense matrix read from L2 (grid_dense_side = 16, the default; n = 130 and
-    300 reach that level at sides 9 and 10, 1024 at 16) applies the same
-    linear map as the dense tail over the levels of side <= 8, rounded
-    differently: iteration counts within one, solutions within 1e-10 of each
-    other and within 1e-9 (FP64) / 1e-8 (FP32 cycle) of the oracle's; graph
-    replay gives the eager launches' bits."""
+    300 reach that level at sides 9 and 10, 1024 at 16; with the W-cycle the
+    matrix of both visits of that level, 2M - M A M, applied once per pair)
+    applies the same linear map as the dense tail over the levels of side
+    <= 8, rounded differently: iteration counts within one, solutions within
+    1e-10 of each other and within 1e-9 (FP64) / 1e-8 (FP32 cycle) of the
+    oracle's; graph replay gives the eager launches' bits."""
     from conftest import rel_l2
 
     a = two_phase(n, seed=23)
@@ -197,3 +198,24 @@ def test_dense_side_16_matches_side_8(precision, n, oracle):
     assert rel_l2(u_e, u8) <= 1e-10
     if n <= 300:
         assert rel_l2(u_e, oracle.solve(a).u) <= (1e-9 if precision == "fp64" else 1e-8)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_dense_side_16_v_cycle(precision, oracle):
+    """With the V-cycle the side-16 matrix is the single visit's (one visit
+    per coarse level, no pair): n = 300, same bar as the W-cycle test, and
+    the oracle's V-cycle iteration count within one."""
+    from conftest import rel_l2
+
+    a = two_phase(300, seed=29)
+    with capi.Context(apply_tuning=False) as ctx:
+        ctx.set_tuning("grid_min_n", 64)
+        ctx.set_tuning("grid_dense_side", 8)
+        u8, it8 = _solve(ctx, a, precision, cycle_index=1)
+        ctx.set_tuning("grid_dense_side", 16)
+        print(f"dense side 16 V-cycle n=300 {precision}", flush=True)
+        u16, it16 = _solve(ctx, a, precision, cycle_index=1)
+    ref = oracle.solve(a, cycle_index=1)
+    assert abs(it16 - it8) <= 1 and abs(it16 - ref.iterations) <= 1
+    assert rel_l2(u16, u8) <= 1e-10
+    assert rel_l2(u16, ref.u) <= (1e-9 if precision == "fp64" else 1e-8)
