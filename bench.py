@@ -154,15 +154,13 @@ def suite_large(args) -> int:
     """Large single solves (1024^2 and 4096^2, two-phase 12/3 and 1e3/1e-3
     media, seed 0): FP64 and FP32-cycle paths, event time, iterations and,
     with --oracle, the relative L2 distance to the CPU oracle's solution."""
-    from darcy.capi import Context
-
     oracle = None
     if args.oracle:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyoracle
 
         oracle = pyoracle.Oracle()
-    ctx = Context(0)
+    ctx = _context(args)
     if args.no_graph:
         ctx.set_tuning("grid_graph", 0)
     for n in args.sizes:
@@ -202,11 +200,9 @@ def suite_phases(args) -> int:
     multigrid cycle and with no preconditioner (z = r, one extra copy pass),
     in each precision; (time at 21 iterations - time at 1) / 20 is the cost
     of one iteration, kernel event time, best of 3."""
-    from darcy.capi import Context
-
     n, batch = args.n, args.batch
     a = media(n, batch, SEED0)
-    ctx = Context(0)
+    ctx = _context(args)
     cells = n * n
     ab, ub = ctx.alloc(a.nbytes), ctx.alloc(a.nbytes)
     ctx.upload(ab, a)
@@ -241,12 +237,10 @@ def uneven_sizes(count: int) -> np.ndarray:
 def suite_uneven(args) -> int:
     """2000 problems, side drawn from {64,...,512} by seed 0, problem i on GRF
     seed i, two-phase 12/3, solved in one persistent launch per step."""
-    from darcy.capi import Context
-
     count = args.batch if args.batch != 1000 else 2000
     sizes = uneven_sizes(count)
     media = [two_phase_permeability(gaussian_random_field(int(n), seed=i)) for i, n in enumerate(sizes)]
-    ctx = Context(0)
+    ctx = _context(args)
     total = int(sum(int(n) * int(n) for n in sizes))
     ab, ub = ctx.alloc(total * 8), ctx.alloc(total * 8)
     a_ptrs, u_ptrs, off = [], [], 0
@@ -292,6 +286,17 @@ def suite_uneven(args) -> int:
     return 0
 
 
+def _context(args):
+    """A context with the tuning table applied, then each --tune KEY=VALUE."""
+    from darcy.capi import Context
+
+    ctx = Context(0)
+    for item in args.tune:
+        key, _, value = item.partition("=")
+        ctx.set_tuning(key, int(value))
+    return ctx
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--suite", choices=["metric", "large", "uneven", "phases"], default="metric")
@@ -312,6 +317,8 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=320)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--weights", default=None, help="warm-start weight file (default: the committed weights)")
+    ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
+                    help="set a tuning key after the tuning table (repeatable)")
     ap.add_argument("--copy-only", type=int, default=0,
                     help="only run the streaming copy kernel on this many MiB (profiler calibration)")
     args = ap.parse_args()
@@ -335,7 +342,7 @@ def main() -> int:
 
     n, batch = args.n, args.batch
     a = media(n, batch, SEED0)
-    ctx = Context(0)
+    ctx = _context(args)
     copy_gbs = ctx.bandwidth_probe(1 << 30, 10)
     cells = n * n
     a_base = ctx.alloc(a.nbytes)

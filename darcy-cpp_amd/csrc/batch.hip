@@ -16,10 +16,15 @@
 
 namespace darcy_hip {
 
-template <class T>
+// B threads per workgroup: kBlock, or kWideBlock for the large problems of a
+// batch (one workgroup per CU, all its wave slots on one problem).  Every
+// phase takes the thread count from Exec and the reductions keep their
+// canonical order, so both widths give the same bits.
+template <class T, int B>
 DARCY_INLINE void run_problem(Shared& s, T* pool, float* tail, const Problem& pb, ProblemStats& st,
                               const SolveParams& P, char* slot) {
-    const Exec e{static_cast<int>(threadIdx.x), kBlock};
+    constexpr int kWavesB = B / kWave;  // this workgroup's waves
+    const Exec e{static_cast<int>(threadIdx.x), B};
     const int wave = threadIdx.x / kWave;
     const int n = pb.n, N = n * n, C = chunks_of(N);
     constexpr bool mixed = sizeof(T) == 4;
@@ -111,7 +116,7 @@ DARCY_INLINE void run_problem(Shared& s, T* pool, float* tail, const Problem& pb
     // would branch around its loads).
     auto init = [&](auto warm) {
         chunk_pass<2>(
-            N, wave, kWaves, h.part,
+            N, wave, kWavesB, h.part,
             [&](int k) {
                 Init v;
                 v.bk = (pb.rhs ? pb.rhs[k] : pb.f) * h2;
@@ -156,7 +161,7 @@ DARCY_INLINE void run_problem(Shared& s, T* pool, float* tail, const Problem& pb
 
     precondition(cyc, h, P, N, e);
     chunk_pass<1>(
-        N, wave, kWaves, h.part, [&](int k) { return h.r[k] * zval(k); },
+        N, wave, kWavesB, h.part, [&](int k) { return h.r[k] * zval(k); },
         [&](int, double v, double* acc) { acc[0] += v; });
     reduce_parts(s, 1, C);
     double rz = s.red[0];
@@ -185,11 +190,11 @@ DARCY_INLINE void run_problem(Shared& s, T* pool, float* tail, const Problem& pb
     for (int it = 0; it < P.max_iter; ++it) {
         // p = z + beta p, ap = A p, <p, ap>; the pending u update.
         if constexpr (mixed)
-            cg_direction<1, defer>(n, A0, x0f, pold, beta, first, pnew, h.ap, wave, kWaves, h.part,
+            cg_direction<1, defer>(n, A0, x0f, pold, beta, first, pnew, h.ap, wave, kWavesB, h.part,
                                    pending ? u : nullptr, alpha);
         else
             cg_direction<1, defer>(n, A0, static_cast<const double*>(h.z), pold, beta, first, pnew, h.ap, wave,
-                                   kWaves, h.part);
+                                   kWavesB, h.part);
         pending = false;
         reduce_parts(s, 1, C);
         const double pap = s.red[0];
@@ -203,7 +208,7 @@ DARCY_INLINE void run_problem(Shared& s, T* pool, float* tail, const Problem& pb
         pending = defer;
         // u += alpha p (unless deferred); r -= alpha ap; <r, r>  (blas.hpp:62-77)
         chunk_pass<1>(
-            N, wave, kWaves, h.part,
+            N, wave, kWavesB, h.part,
             [&](int k) { return defer ? Upd{0.0, 0.0, h.r[k], h.ap[k]} : Upd{u[k], pnew[k], h.r[k], h.ap[k]}; },
             [&](int k, const Upd& v, double* acc) {
                 if (!defer) u[k] = v.u + alpha * v.p;
@@ -224,7 +229,7 @@ DARCY_INLINE void run_problem(Shared& s, T* pool, float* tail, const Problem& pb
         precondition(cyc, h, P, N, e);
         const T* const z0 = mixed ? reinterpret_cast<const T*>(h.x[0]) : reinterpret_cast<const T*>(h.z);
         chunk_pass<2>(
-            N, wave, kWaves, h.part, [&](int k) { return Rz{h.r[k], z0[k], mixed ? h.ap[k] : 0.0}; },
+            N, wave, kWavesB, h.part, [&](int k) { return Rz{h.r[k], z0[k], mixed ? h.ap[k] : 0.0}; },
             [&](int, const Rz& v, double* acc) {
                 const double z = static_cast<double>(v.z);
                 acc[0] += v.r * z;
@@ -244,8 +249,8 @@ DARCY_INLINE void run_problem(Shared& s, T* pool, float* tail, const Problem& pb
     if (threadIdx.x == 0) st = ProblemStats{iters, status, levels, hist ? iters + 1 : 0, res, -1};
 }
 
-template <class T>
-__global__ __launch_bounds__(kBlock, 4) void k_batch(BatchArgs A) {
+template <class T, int B>
+__global__ __launch_bounds__(B, B == kBlock ? 4 : 1) void k_batch(BatchArgs A) {
     __shared__ Shared s;
     __shared__ __align__(16) T pool[pool_elems<T>()];
     __shared__ __align__(16) float tail[sizeof(T) == 4 ? kTailLds : 1];
@@ -256,13 +261,15 @@ __global__ __launch_bounds__(kBlock, 4) void k_batch(BatchArgs A) {
         const int t = s.prob;
         if (t >= A.count) return;
         const int p = A.order ? A.order[t] : t;
-        run_problem<T>(s, pool, tail, A.probs[p], A.stats[p], A.prm, slot);
+        run_problem<T, B>(s, pool, tail, A.probs[p], A.stats[p], A.prm, slot);
         __syncthreads();
     }
 }
 
-template __global__ void k_batch<double>(BatchArgs);
-template __global__ void k_batch<float>(BatchArgs);
+template __global__ void k_batch<double, kBlock>(BatchArgs);
+template __global__ void k_batch<float, kBlock>(BatchArgs);
+template __global__ void k_batch<double, kWideBlock>(BatchArgs);
+template __global__ void k_batch<float, kWideBlock>(BatchArgs);
 
 // Scratch of the allocation-free entry point (darcy_solve_batch_noalloc):
 // its problem tables, work-queue counter and solve slots.  Each device the
@@ -290,21 +297,32 @@ __global__ __launch_bounds__(256) void k_stream_copy(const double2* __restrict__
 }  // namespace darcy_hip
 
 // Launch entry points for the host code of this library.
-extern "C" hipError_t darcy_hip_launch_batch(const darcy_hip::BatchArgs* args, int grid, int mixed,
+// wide: the kWideBlock-thread kernel.
+extern "C" hipError_t darcy_hip_launch_batch(const darcy_hip::BatchArgs* args, int grid, int mixed, int wide,
                                              hipStream_t stream) {
-    if (mixed)
-        hipLaunchKernelGGL(darcy_hip::k_batch<float>, dim3(grid), dim3(darcy_hip::kBlock), 0, stream, *args);
-    else
-        hipLaunchKernelGGL(darcy_hip::k_batch<double>, dim3(grid), dim3(darcy_hip::kBlock), 0, stream, *args);
+    using namespace darcy_hip;
+    if (wide) {
+        if (mixed)
+            hipLaunchKernelGGL((k_batch<float, kWideBlock>), dim3(grid), dim3(kWideBlock), 0, stream, *args);
+        else
+            hipLaunchKernelGGL((k_batch<double, kWideBlock>), dim3(grid), dim3(kWideBlock), 0, stream, *args);
+    } else if (mixed) {
+        hipLaunchKernelGGL((k_batch<float, kBlock>), dim3(grid), dim3(kBlock), 0, stream, *args);
+    } else {
+        hipLaunchKernelGGL((k_batch<double, kBlock>), dim3(grid), dim3(kBlock), 0, stream, *args);
+    }
     return hipGetLastError();
 }
 
-extern "C" hipError_t darcy_hip_batch_occupancy(int mixed, int* blocks_per_cu) {
-    if (mixed)
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, darcy_hip::k_batch<float>,
-                                                            darcy_hip::kBlock, 0);
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, darcy_hip::k_batch<double>,
-                                                        darcy_hip::kBlock, 0);
+extern "C" hipError_t darcy_hip_batch_occupancy(int mixed, int wide, int* blocks_per_cu) {
+    using namespace darcy_hip;
+    if (wide)
+        return mixed ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_batch<float, kWideBlock>,
+                                                                    kWideBlock, 0)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_batch<double, kWideBlock>,
+                                                                    kWideBlock, 0);
+    return mixed ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_batch<float, kBlock>, kBlock, 0)
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_batch<double, kBlock>, kBlock, 0);
 }
 
 // Device address of `scratch` on the current device.  The array's name is a

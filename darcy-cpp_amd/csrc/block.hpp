@@ -12,6 +12,9 @@ namespace darcy_hip {
 
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / kWave;
+// The persistent kernel's workgroup for large problems: a whole CU's wave
+// slots (16 waves at the 128-VGPR budget of four kBlock workgroups).
+constexpr int kWideBlock = 1024;
 
 // Largest side of the levels kept in LDS: 16 (14 KB in FP64, 7.5 KB in FP32;
 // the FP32 kernel also keeps the 16 KB tail matrix), so four workgroups fit
@@ -738,7 +741,8 @@ struct Cycle {
 // launch take columns in turn (this workgroup's first is col0, then every
 // col_step); `scratch` (the LDS pool, before it receives the operators, or
 // an array of its own) holds the FP64 operators of the tail levels and each
-// wave's x and b.
+// wave's x and b; waves past the first kWaves (a wider workgroup) take no
+// columns, so the scratch holds kWaves waves' vectors whatever the width.
 // pair: column k is the W-cycle's two visits of level tl instead -- the
 // cycle from zero, then the cycle again from its result with the same
 // right-hand side -- which is the matrix 2M - M A M of the single cycle M.
@@ -755,11 +759,11 @@ DARCY_INLINE void build_tail_matrix(const Hier& h, int tl, const CycleOpts& o, v
     const int ops_end = op_off(L);
     for (int l = tl; l < L; ++l) {
         const int n = h.n[l], nf = n * (n + 1), base = op_off(l);
-        for (int q = threadIdx.x; q < nf; q += kBlock) {
+        for (int q = threadIdx.x; q < nf; q += blockDim.x) {
             dp[base + q] = h.ex[l][q];
             dp[base + nf + q] = h.ny[l][q];
         }
-        for (int q = threadIdx.x; q < n * n; q += kBlock) dp[base + 2 * nf + q] = h.id[l][q];
+        for (int q = threadIdx.x; q < n * n; q += blockDim.x) dp[base + 2 * nf + q] = h.id[l][q];
     }
     __syncthreads();
     int vsz = 0;  // x and b of every level from tl, per wave
@@ -775,7 +779,7 @@ DARCY_INLINE void build_tail_matrix(const Hier& h, int tl, const CycleOpts& o, v
     const int dim = h.n[tl] * h.n[tl];
     if (ld == 0) ld = dim;
     const WaveSync sync;
-    for (int k = col0 + wave; k < dim; k += col_step) {
+    for (int k = col0 + wave; wave < kWaves && k < dim; k += col_step) {
         const LvP<double> top = view(tl);
         for (int q = lane; q < dim; q += kWave) top.b[q] = q == k ? 1.0 : 0.0;
         sync();

@@ -20,7 +20,7 @@
 using namespace darcy_hip;
 
 using CycleOptsC = darcy_hip::CycleOpts;
-extern "C" hipError_t darcy_hip_launch_batch(const BatchArgs* args, int grid, int mixed, hipStream_t stream);
+extern "C" hipError_t darcy_hip_launch_batch(const BatchArgs* args, int grid, int mixed, int wide, hipStream_t stream);
 extern "C" hipError_t darcy_grid_fine(const double* a, const Hier* hd, unsigned long long* bad, int blocks,
                                       hipStream_t s);
 extern "C" hipError_t darcy_grid_galerkin(const Hier* hd, int l, int blocks, hipStream_t s);
@@ -36,7 +36,7 @@ extern "C" hipError_t darcy_grid_big_matrix(int mixed, const Hier* hd, CycleOpts
                                             hipStream_t s);
 extern "C" hipError_t darcy_grid_cg(int which, const GridCg* g, const double* pold, double* pnew, int blocks,
                                     hipStream_t s);
-extern "C" hipError_t darcy_hip_batch_occupancy(int mixed, int* blocks_per_cu);
+extern "C" hipError_t darcy_hip_batch_occupancy(int mixed, int wide, int* blocks_per_cu);
 extern "C" hipError_t darcy_grid_check_kernels(int device, const char** what);
 extern "C" hipError_t darcy_hip_launch_copy(const void* src, void* dst, size_t bytes, int grid, hipStream_t stream);
 extern "C" hipError_t darcy_hip_scratch(void** ptr, size_t* bytes);
@@ -83,6 +83,10 @@ int check_opts(darcy_ctx* ctx, const darcy_opts& o) {
 // Largest grid side one call may use: flat indices and face counts are
 // 32-bit inside the kernels.
 constexpr int64_t kMaxSide = 32768;
+// Pipeline step of a wide (1024-thread) batch workgroup: the tuned step of a
+// 256-thread one times 4, the same cells per thread and per L2 share (a
+// quarter as many workgroups share each L2).
+constexpr int kWideStepFactor = 4;
 
 // Checks every entry point makes before any device work: the side, and the
 // arrays every solve reads and writes.
@@ -256,8 +260,15 @@ int run(darcy_ctx* ctx, const Job& job) {
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return job.n[x] > job.n[y]; });
 
+    // A batch whose largest side is >= batch_wide_side runs on kWideBlock-
+    // thread workgroups, one per CU, any other on kBlock-thread ones, four to
+    // a CU.  One workgroup per problem either way, largest first: a large
+    // problem on a narrow workgroup runs only as fast as its few waves'
+    // memory requests in flight allow and ends the launch, while a batch of
+    // equal mid-size problems is fastest four to a CU.
+    const int wide = ctx->batch_wide_side > 0 && nmax >= ctx->batch_wide_side ? 1 : 0;
     int per_cu = 1;
-    HIP_TRY(darcy_hip_batch_occupancy(mixed, &per_cu), "occupancy query");
+    HIP_TRY(darcy_hip_batch_occupancy(mixed, wide, &per_cu), "occupancy query");
     per_cu = std::max(per_cu, 1);
     Hier scratch_h;
     const size_t slot_bytes = plan_slot(nullptr, nmax, mixed, &scratch_h, o.min_coarse_n, o.max_levels);
@@ -288,9 +299,10 @@ int run(darcy_ctx* ctx, const Job& job) {
     args.slot_bytes = slot_bytes;
     args.count = static_cast<int32_t>(count);
     args.prm = solve_params(ctx, o, job.mode, mixed, static_cast<int32_t>(hist_n));
+    if (wide) args.prm.step_cells *= kWideStepFactor;
 
     HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream), "event record");
-    HIP_TRY(darcy_hip_launch_batch(&args, static_cast<int>(grid), mixed, ctx->stream), "solve kernel launch");
+    HIP_TRY(darcy_hip_launch_batch(&args, static_cast<int>(grid), mixed, wide, ctx->stream), "solve kernel launch");
     HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream), "event record");
 
     std::vector<ProblemStats> st(static_cast<size_t>(count));
@@ -827,6 +839,8 @@ extern "C" int darcy_ctx_set_tuning(darcy_ctx* ctx, const char* key, int32_t val
         ctx->grid_fp64_dense_tail = value;
     } else if (k == "grid_dense_side" && (value == 8 || value == 16)) {
         ctx->grid_dense_side = value;
+    } else if (k == "batch_wide_side" && (value == 0 || (value >= 64 && value <= 32768))) {
+        ctx->batch_wide_side = value;
     } else {
         return fail(ctx, DARCY_INVALID_INPUT, "unknown tuning key or value out of range: " + k);
     }
@@ -852,6 +866,8 @@ extern "C" int darcy_ctx_get_tuning(const darcy_ctx* ctx, const char* key, int32
         *value = ctx->grid_fp64_dense_tail;
     else if (k == "grid_dense_side")
         *value = ctx->grid_dense_side;
+    else if (k == "batch_wide_side")
+        *value = ctx->batch_wide_side;
     else
         return DARCY_INVALID_INPUT;
     return DARCY_OK;
@@ -976,8 +992,10 @@ extern "C" int darcy_solve_batch_noalloc(darcy_ctx* ctx, int64_t count, const in
     std::vector<int32_t> order(c);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return n[x] > n[y]; });
+    // The workgroup width as in darcy_solve_batch (batch_wide_side).
+    const int wide = ctx->batch_wide_side > 0 && nmax >= ctx->batch_wide_side ? 1 : 0;
     int per_cu = 1;
-    HIP_TRY(darcy_hip_batch_occupancy(mixed, &per_cu), "occupancy query");
+    HIP_TRY(darcy_hip_batch_occupancy(mixed, wide, &per_cu), "occupancy query");
     const int64_t fit = static_cast<int64_t>((ctx->scratch_bytes - o_slots) / slot_bytes);
     const int64_t grid = std::max<int64_t>(
         1, std::min<int64_t>({count, static_cast<int64_t>(ctx->cus) * std::max(per_cu, 1), fit}));
@@ -997,8 +1015,9 @@ extern "C" int darcy_solve_batch_noalloc(darcy_ctx* ctx, int64_t count, const in
     args.slot_bytes = slot_bytes;
     args.count = static_cast<int32_t>(count);
     args.prm = solve_params(ctx, o, kModeSolve, mixed, 0);
+    if (wide) args.prm.step_cells *= kWideStepFactor;
     HIP_TRY(hipEventRecord(ctx->ev0, s), "event record");
-    HIP_TRY(darcy_hip_launch_batch(&args, static_cast<int>(grid), mixed, s), "solve kernel launch");
+    HIP_TRY(darcy_hip_launch_batch(&args, static_cast<int>(grid), mixed, wide, s), "solve kernel launch");
     HIP_TRY(hipEventRecord(ctx->ev1, s), "event record");
     std::vector<ProblemStats> st(c);
     HIP_TRY(hipMemcpyAsync(st.data(), base + o_stats, sizeof(ProblemStats) * c, hipMemcpyDeviceToHost, s),

@@ -69,6 +69,10 @@ struct darcy_ctx {
     // side <= 16, read from L2).  The FP64 cycle uses either only with
     // grid_fp64_dense_tail.
     int grid_dense_side = 16;
+    // Persistent batch kernel: a batch whose largest side is >= this runs on
+    // 1024-thread workgroups, one per CU (0: always 256-thread workgroups,
+    // four per CU).  Same bits either way.
+    int batch_wide_side = 384;
     // Device address and size of the library's static scratch array (the
     // allocation-free entry point), looked up when the context is created.
     void* scratch = nullptr;

@@ -108,6 +108,10 @@ void darcy_ctx_destroy(darcy_ctx* ctx);
  *                     persistent batch kernel keeps in global memory, in
  *                     the FP64 cycle (128..16384, default 1024)
  *   "batch_step_cells_fp32" the same in the FP32 cycle (default 4096)
+ *   "batch_wide_side" a batch whose largest side is >= this runs on
+ *                     1024-thread workgroups, one per CU, with 4x the
+ *                     pipeline step (0: always 256-thread workgroups, four
+ *                     per CU; 64..32768, default 384; not autotuned)
  * None of these changes any result: only the launch structure.  Two more
  * keys change the rounding of the grid path's cycle, not the method:
  *   "grid_fp64_dense_tail" 1 (default): the FP64 cycle applies its tail

@@ -142,6 +142,50 @@ def test_batch_step_cells_do_not_change_results(precision):
                 ctx.free(pu)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_wide_workgroups_do_not_change_results(precision, oracle):
+    """A batch whose largest side is >= batch_wide_side runs on 1024-thread
+    workgroups (one per CU): an uneven batch with sides 37 ... 512 gives the
+    same bits and iteration counts on narrow workgroups (0, 1024) and wide
+    ones (384, 64); the 512^2 problem's u is within 1e-9 (FP64) / 1e-8 (FP32
+    cycle, against the oracle's FP32 restatement) of the oracle's, its
+    iterations equal."""
+    from conftest import rel_l2
+
+    sizes = [512, 37, 384, 64, 100, 256, 129]
+    media = [two_phase(n, seed=80 + i) for i, n in enumerate(sizes)]
+    with capi.Context(apply_tuning=False) as ctx:
+        bufs = [(ctx.alloc(a.nbytes), ctx.alloc(a.nbytes)) for a in media]
+        try:
+            for (pa, _), a in zip(bufs, media):
+                ctx.upload(pa, a)
+            ref = None
+            for side in (0, 384, 1024, 64):
+                print(f"batch wide side {side} {precision}", flush=True)
+                ctx.set_tuning("batch_wide_side", side)
+                assert ctx.get_tuning("batch_wide_side") == side
+                st = ctx.solve_batch_device(sizes, [b[0] for b in bufs], [b[1] for b in bufs], precision=precision)
+                us = []
+                for (_, pu), a in zip(bufs, media):
+                    u = np.empty_like(a)
+                    ctx.download(u, pu)
+                    us.append(u)
+                if ref is None:
+                    ref = (st["iterations"].copy(), us)
+                    assert st["converged"].all()
+                else:
+                    assert np.array_equal(st["iterations"], ref[0])
+                    for u, u_ref in zip(us, ref[1]):
+                        assert np.array_equal(u, u_ref)
+        finally:
+            for pa, pu in bufs:
+                ctx.free(pa)
+                ctx.free(pu)
+    want = oracle.solve(media[0], fp32=precision == "mixed")
+    assert int(ref[0][0]) == want.iterations
+    assert rel_l2(ref[1][0], want.u) <= (1e-9 if precision == "fp64" else 1e-8)
+
+
 @pytest.mark.parametrize("n", [130, 300, 1024])
 def test_fp64_dense_tail_matches_the_recursive_cycle(n, oracle):
     """The grid path's FP64 dense tail applies the same linear map as the
