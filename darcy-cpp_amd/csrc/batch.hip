@@ -43,7 +43,7 @@ DARCY_INLINE void run_problem(Shared& s, T* pool, float* tail, const Problem& pb
     const bool mg = P.mode == kModeApplyPreconditioner || (P.mode == kModeSolve && P.precond == kPcMultigrid);
     if (threadIdx.x == 0) {
         plan_slot(slot, n, mixed, &s.h, P.min_coarse, P.max_levels);
-        plan_lds<T, LdsSide<T>::value>(s.h, s.t);
+        plan_lds<T, LdsSide<T, B>::value>(s.h, s.t);
     }
     __syncthreads();
     const Hier& h = s.h;
@@ -252,7 +252,7 @@ DARCY_INLINE void run_problem(Shared& s, T* pool, float* tail, const Problem& pb
 template <class T, int B>
 __global__ __launch_bounds__(B, B == kBlock ? 4 : 1) void k_batch(BatchArgs A) {
     __shared__ Shared s;
-    __shared__ __align__(16) T pool[pool_elems<T>()];
+    __shared__ __align__(16) T pool[pool_elems<T, B>()];
     __shared__ __align__(16) float tail[sizeof(T) == 4 ? kTailLds : 1];
     char* slot = A.slots + static_cast<size_t>(blockIdx.x) * A.slot_bytes;
     for (;;) {

@@ -16,21 +16,23 @@ constexpr int kWaves = kBlock / kWave;
 // slots (16 waves at the 128-VGPR budget of four kBlock workgroups).
 constexpr int kWideBlock = 1024;
 
-// Largest side of the levels kept in LDS: 16 (14 KB in FP64, 7.5 KB in FP32;
-// the FP32 kernel also keeps the 16 KB tail matrix), so four workgroups fit
-// on a CU.
-template <class T>
+// Largest side of the levels kept in LDS by a B-thread batch workgroup: 16
+// for kBlock (14 KB in FP64, 7.5 KB in FP32; the FP32 kernel also keeps the
+// 16 KB tail matrix), so four workgroups fit on a CU; for kWideBlock, alone
+// on its CU, 64 in FP32 (about 111 KB with the tail matrix) and 32 in FP64
+// (57 KB: a 64^2 FP64 level alone would need 163 KB).
+template <class T, int B = kBlock>
 struct LdsSide {
-    static constexpr int value = 16;
+    static constexpr int value = B == kBlock ? 16 : (sizeof(T) == 4 ? 64 : 32);
 };
 
 // LDS elements of the coarse-level pool.  The FP32 pool doubles as FP64
 // scratch while the tail matrix is built: FP64 operators of the tail levels
 // (<= 285 doubles) and x, b for one unit vector per wave (4 x 170 doubles).
 constexpr int kTailScratchDoubles = 285 + 4 * 170;
-template <class T>
+template <class T, int B = kBlock>
 constexpr int pool_elems() {
-    constexpr int need = LdsPool<T, LdsSide<T>::value>::kElems;
+    constexpr int need = LdsPool<T, LdsSide<T, B>::value>::kElems;
     constexpr int scratch = sizeof(T) == 4 ? 2 * kTailScratchDoubles : 0;
     return need > scratch ? need : scratch;
 }
