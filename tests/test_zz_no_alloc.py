@@ -1,7 +1,8 @@
 """darcy_solve_batch_noalloc (the library's static scratch array, no device
 allocation) against darcy_solve_batch on the same device-resident inputs:
-the same bits and statistics, on two calls in a row (the second with a
-smaller batch), and its size and pointer checks."""
+the same bits and statistics, on calls in a row (a batch with a side of 384,
+which runs on 1024-thread workgroups, then a smaller batch), and its size
+and pointer checks."""
 
 from __future__ import annotations
 
@@ -59,8 +60,10 @@ def test_same_bits_as_the_allocating_entry_point_twice(precision):
         assert ctx.scratch_bytes() == 64 << 20
         first = [two_phase(n, seed=50 + i) for i, n in enumerate([64, 33, 128, 17, 96, 5, 256])]
         u0s = [1e-3 * rng.standard_normal(a.shape) for a in first]
+        # sides >= batch_wide_side (384): both entry points on 1024-thread workgroups
+        wide = [two_phase(n, seed=60 + i) for i, n in enumerate([384, 40])]
         second = [two_phase(n, seed=70 + i) for i, n in enumerate([16, 8, 24])]
-        for media, warm in ((first, u0s), (second, None)):
+        for media, warm in ((first, u0s), (wide, None), (second, None)):
             bufs = _device_batch(ctx, media, warm)
             try:
                 ra, ua = _solve(ctx, media, bufs, False, precision=precision)
