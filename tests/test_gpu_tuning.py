@@ -145,14 +145,14 @@ def test_batch_step_cells_do_not_change_results(precision):
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_wide_workgroups_do_not_change_results(precision, oracle):
     """A batch whose largest side is >= batch_wide_side runs on 1024-thread
-    workgroups (one per CU): an uneven batch with sides 37 ... 512 gives the
+    workgroups (one per CU): an uneven batch with sides 1 ... 512 gives the
     same bits and iteration counts on narrow workgroups (0, 1024) and wide
     ones (384, 64); the 512^2 problem's u is within 1e-9 (FP64) / 1e-8 (FP32
     cycle, against the oracle's FP32 restatement) of the oracle's, its
     iterations equal."""
     from conftest import rel_l2
 
-    sizes = [512, 37, 384, 64, 100, 256, 129]
+    sizes = [512, 37, 384, 64, 1, 100, 2, 256, 3, 129]
     media = [two_phase(n, seed=80 + i) for i, n in enumerate(sizes)]
     with capi.Context(apply_tuning=False) as ctx:
         bufs = [(ctx.alloc(a.nbytes), ctx.alloc(a.nbytes)) for a in media]
