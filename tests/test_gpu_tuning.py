@@ -186,6 +186,30 @@ def test_wide_workgroups_do_not_change_results(precision, oracle):
     assert rel_l2(ref[1][0], want.u) <= (1e-9 if precision == "fp64" else 1e-8)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_wide_workgroups_name_the_first_bad_problem(precision):
+    """A batch on 1024-thread workgroups (largest side 400) with invalid
+    media: the kernel's validation names the lowest bad problem and its
+    first offending flat index, as on narrow workgroups (solver.hpp:68-77)."""
+    media = [two_phase(n, seed=90 + i) for i, n in enumerate([16, 400, 40])]
+    media[1].flat[1234] = -1.0
+    media[1].flat[5000] = 0.0
+    media[2][1, 1] = np.nan
+    with capi.Context(apply_tuning=False) as ctx:
+        assert ctx.get_tuning("batch_wide_side") == 384
+        bufs = [(ctx.alloc(a.nbytes), ctx.alloc(a.nbytes)) for a in media]
+        try:
+            for (pa, _), a in zip(bufs, media):
+                ctx.upload(pa, a)
+            print(f"wide batch with invalid media {precision}", flush=True)
+            with pytest.raises(ValueError, match=r"flat index 1234 of problem 1"):
+                ctx.solve_batch_device([16, 400, 40], [b[0] for b in bufs], [b[1] for b in bufs], precision=precision)
+        finally:
+            for pa, pu in bufs:
+                ctx.free(pa)
+                ctx.free(pu)
+
+
 @pytest.mark.parametrize("n", [130, 300, 1024])
 def test_fp64_dense_tail_matches_the_recursive_cycle(n, oracle):
     """The grid path's FP64 dense tail applies the same linear map as the
