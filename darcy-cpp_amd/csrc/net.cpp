@@ -251,6 +251,9 @@ int darcy_net::refresh_weights() {
 }
 
 int darcy_net::upload_tables(const void* const* a, const void* const* u, const double* f, int64_t count) {
+    // A null entry would reach the kernels as an address: rejected here.
+    for (int64_t i = 0; i < count; ++i)
+        if (!a[i] || (u && !u[i])) return fail(ctx, DARCY_INVALID_INPUT, "null pointer in the a or u table");
     const size_t bytes = static_cast<size_t>(count) * sizeof(void*);
     NN_TRY(tab_a.reserve(bytes), "net tables");
     NN_TRY(hipMemcpyAsync(tab_a.ptr, a, bytes, hipMemcpyHostToDevice, ctx->stream), "net tables");

@@ -303,6 +303,8 @@ class Context:
         n = np.ascontiguousarray(sizes, dtype=np.int64)
         a, u = _ptr_table(a_ptrs), _ptr_table(u_ptrs)
         u0 = _ptr_table(u0_ptrs) if u0_ptrs is not None else None
+        if n.shape != (count,) or a.shape != (count,) or u.shape != (count,) or (u0 is not None and u0.shape != (count,)):
+            raise ValueError("one size, one a, one u (and one u0) pointer per problem")
         fs = np.full(count, float(f))
         st = np.zeros(count, dtype=STATS_DTYPE)
         entry = self.lib.darcy_solve_batch_noalloc if no_alloc else self.lib.darcy_solve_batch
@@ -422,6 +424,8 @@ class Net:
 
     def train_step(self, a_ptrs: Sequence[int], u_ptrs: Sequence[int], lr: float, want_loss: bool = True) -> float:
         count = len(a_ptrs)
+        if len(u_ptrs) != count:
+            raise ValueError("one a and one u pointer per sample")
         a = (C.c_void_p * count)(*a_ptrs)
         u = (C.c_void_p * count)(*u_ptrs)
         loss = C.c_double(float("nan"))
@@ -431,6 +435,8 @@ class Net:
 
     def gradient(self, a_ptrs: Sequence[int], u_ptrs: Sequence[int]) -> tuple[float, np.ndarray]:
         count = len(a_ptrs)
+        if len(u_ptrs) != count:
+            raise ValueError("one a and one u pointer per sample")
         a = (C.c_void_p * count)(*a_ptrs)
         u = (C.c_void_p * count)(*u_ptrs)
         loss = C.c_double()

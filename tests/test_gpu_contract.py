@@ -300,3 +300,25 @@ def test_hierarchy_options_solve_matches_oracle(ctx, oracle, n, min_coarse_n, ma
     assert mixed["converged"] and mixed["levels"] == want.levels
     assert rel_l2(mixed["u"], want.u) <= 1e-8
 
+
+
+def test_pointer_tables_must_match_the_problem_count(ctx):
+    """A short pointer table is refused on the host, and a null entry in a
+    network table before any launch."""
+    a = np.full((8, 8), 2.0)
+    pa, pu = ctx.alloc(a.nbytes), ctx.alloc(a.nbytes)
+    try:
+        ctx.upload(pa, a)
+        with pytest.raises(ValueError, match="per problem"):
+            ctx.solve_batch_device([8, 8], [pa], [pu, pu])
+        with pytest.raises(ValueError, match="per problem"):
+            ctx.solve_batch_device([8], [pa], [pu], u0_ptrs=[pu, pu])
+        net = capi.Net(ctx, 64, 2)
+        try:
+            with pytest.raises(ValueError, match="null pointer"):
+                net.predict_device([pa, 0], [1.0, 1.0], [pu, pu])
+        finally:
+            net.close()
+    finally:
+        ctx.free(pa)
+        ctx.free(pu)
