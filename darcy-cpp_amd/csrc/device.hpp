@@ -468,12 +468,14 @@ DARCY_HD __forceinline__ Pair2<T> walk_pair(P p, int n, int c0, int c1, int a) {
 
 // b, the ex faces and the south faces of the pair's cells on row a.  b and
 // ny pairs start at even offsets; the ex row (n + 1 long) holds an aligned
-// pair at f on even rows and at f + 1 on odd ones.
+// pair at f on even rows and at f + 1 on odd ones.  `ss` is the pair's
+// south faces (ny of row a), which a walk has already loaded as the north
+// faces of the row below: one load per row fewer.
 template <class T, bool kPairs, template <class> class Sp = FlatPtr>
-DARCY_HD __forceinline__ WalkRow<T> walk_row(const LvP<T, Sp>& L, int c0, int c1, int a) {
+DARCY_HD __forceinline__ WalkRow<T> walk_row(const LvP<T, Sp>& L, int c0, int c1, int a, Pair2<T> ss) {
     const int n = L.n, f = a * (n + 1) + c0;
     const auto ex = L.ex;
-    const Pair2<T> bb = walk_pair<T, kPairs>(L.b, n, c0, c1, a), ss = walk_pair<T, kPairs>(L.ny, n, c0, c1, a);
+    const Pair2<T> bb = walk_pair<T, kPairs>(L.b, n, c0, c1, a);
     if constexpr (kPairs) {
         if (a & 1) {
             const Pair2<T> q = load_pair<T>(ex + f + 1);
@@ -484,6 +486,10 @@ DARCY_HD __forceinline__ WalkRow<T> walk_row(const LvP<T, Sp>& L, int c0, int c1
     } else {
         return WalkRow<T>{bb.lo, bb.hi, ex[f], ex[f + 1], ex[f + 1 + (c1 - c0)], ss.lo, ss.hi};
     }
+}
+template <class T, bool kPairs, template <class> class Sp = FlatPtr>
+DARCY_HD __forceinline__ WalkRow<T> walk_row(const LvP<T, Sp>& L, int c0, int c1, int a) {
+    return walk_row<T, kPairs>(L, c0, c1, a, walk_pair<T, kPairs>(L.ny, L.n, c0, c1, a));
 }
 
 // Where the walk takes the adjacent pair's red value from.  NoShare: from
@@ -542,7 +548,7 @@ DARCY_HD __forceinline__ void presmooth_walk_item(const LvP<T, Sp>& L, int k, in
     WalkRow<T> nx = cur;
     T nn0 = T(0), nn1 = T(0);
     if (i0 + 1 < n) {
-        nx = row(i0 + 1);
+        nx = walk_row<T, kPairs>(L, c0, c1, i0 + 1, Pair2<T>{cn0, cn1});
         south(i0 + 2, nn0, nn1);
     }
     WalkCell<T> cx{};
@@ -554,7 +560,7 @@ DARCY_HD __forceinline__ void presmooth_walk_item(const LvP<T, Sp>& L, int k, in
         WalkRow<T> nx2 = nx;
         T mm0 = T(0), mm1 = T(0);
         if (more && a + 2 < n) {
-            nx2 = row(a + 2);
+            nx2 = walk_row<T, kPairs>(L, c0, c1, a + 2, Pair2<T>{nn0, nn1});  // nn: ny of row a + 2
             south(a + 3, mm0, mm1);
         }
         WalkCell<T> cx2 = cx;
@@ -621,7 +627,7 @@ DARCY_HD __forceinline__ void restrict_walk_item(const LvP<T, Sp>& L, BP bc, int
         WalkRow<T> nx = cur;
         T xw2 = T(0), xe2 = T(0);
         if (more) {
-            nx = walk_row<T, kPairs>(L, c0, c1, a + 1);
+            nx = walk_row<T, kPairs>(L, c0, c1, a + 1, north);  // north: ny of row a + 1
             north2 = walk_pair<T, kPairs>(ny, n, c0, c1, a + 2);
             if (a + 2 < n) xn2 = xpair(a + 2);
             if (!sh.west && west_on) xw2 = x[(a + 1) * n + c0 - 1];
