@@ -1038,7 +1038,17 @@ DARCY_INLINE void down_stream(const LvP<T, Sp>& L, BP bc, bool fresh, int step_c
         // residual row i needs final rows i-1..i+1; coarse row I needs fine rows 2I, 2I+1
         const int ready = black_hi == n ? n : black_hi - 1;
         const int coarse_hi = ready == n ? m : ready / 2;
-        restrict_rows(L, bc, coarse_done, coarse_hi, e);
+        // FP32: the restriction walks (a column pair's rows held in
+        // registers: a few loads per cell where the per-cell residuals
+        // re-read every neighbour); FP64 keeps the per-cell pass, which
+        // needs fewer registers.  Same values either way.
+        if constexpr (sizeof(T) == 4) {
+            if (coarse_hi > coarse_done)
+                restrict_walk_rows(L, bc, coarse_done, coarse_hi,
+                                   walk_strip(coarse_hi - coarse_done, m, e.nthr, kWalkRows / 2), e);
+        } else {
+            restrict_rows(L, bc, coarse_done, coarse_hi, e);
+        }
         coarse_done = coarse_hi;
         if (coarse_done == m) break;
     }
