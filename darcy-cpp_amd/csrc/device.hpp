@@ -654,6 +654,65 @@ DARCY_HD __forceinline__ void restrict_walk_item(const LvP<T, Sp>& L, BP bc, int
     }
 }
 
+// The red half of a GS sweep over rows [i0, i1) of column pair k as a
+// vertical walk, with every black cell final (the second colour of the
+// post-smoothing, level.hpp:145-161): each row's red cell from b, its faces
+// and its four black neighbours -- the pair's other cell, the rows above and
+// below (held in registers as the walk goes down), and the cell beside the
+// pair (from the neighbouring lane through `sh`, else loaded).  A red cell
+// reads no red cell, so items need no order among themselves.  gs_at's
+// operations in its order: the same bits.
+template <class T, bool kPairs, class Share = NoShare, template <class> class Sp = FlatPtr>
+DARCY_HD __forceinline__ void red_walk_item(const LvP<T, Sp>& L, int k, int i0, int i1, Share sh = Share{}) {
+    const int n = L.n, c0 = 2 * k;
+    const bool two = c0 + 1 < n, west_on = c0 > 0, east_on = c0 + 2 < n;
+    const int c1 = two ? c0 + 1 : c0;
+    const auto x = L.x;
+    const auto ny = L.ny;
+    auto xpair = [&](int a) { return walk_pair<T, kPairs>(x, n, c0, c1, a); };
+    // the black cell beside the red one where no lane shares it: west of c0
+    // on even rows, east of c1 on odd rows
+    auto side = [&](int a) -> T {
+        if (a & 1) return (!sh.east && east_on) ? x[a * n + c1 + 1] : T(0);
+        return (!sh.west && west_on) ? x[a * n + c0 - 1] : T(0);
+    };
+    Pair2<T> xp{T(0), T(0)}, xc = xpair(i0), xn{T(0), T(0)};
+    if (i0 > 0) xp = xpair(i0 - 1);
+    if (i0 + 1 < n) xn = xpair(i0 + 1);
+    WalkRow<T> cur = walk_row<T, kPairs>(L, c0, c1, i0);
+    Pair2<T> north = walk_pair<T, kPairs>(ny, n, c0, c1, i0 + 1);
+    T xs = side(i0);
+    for (int a = i0; a < i1; ++a) {
+        const bool hN = a + 1 < n, hS = a > 0, odd = (a & 1) != 0, more = a + 1 < i1;
+        Pair2<T> xn2 = xn, north2 = north;
+        WalkRow<T> nx = cur;
+        T xs2 = T(0);
+        if (more) {
+            nx = walk_row<T, kPairs>(L, c0, c1, a + 1, north);  // north: ny of row a + 1
+            north2 = walk_pair<T, kPairs>(ny, n, c0, c1, a + 2);
+            if (a + 2 < n) xn2 = xpair(a + 2);
+            xs2 = side(a + 1);
+        }
+        // Every lane takes part in both shuffles.
+        const T vw = sh.from_west(xc.hi), ve = sh.from_east(xc.lo);
+        if (!odd) {
+            const T XW = sh.west ? vw : xs;
+            x[a * n + c0] =
+                gs_combine(cur.m, cur.w0, north.lo, cur.s0, cur.b0, xc.hi, XW, xn.lo, xp.lo, two, west_on, hN, hS);
+        } else if (two) {
+            const T XE = sh.east ? ve : xs;
+            x[a * n + c1] =
+                gs_combine(cur.e1, cur.m, north.hi, cur.s1, cur.b1, XE, xc.lo, xn.hi, xp.hi, east_on, true, hN, hS);
+        }
+        xp = xc;
+        xc = xn;
+        xn = xn2;
+        cur = nx;
+        north = north2;
+        xs = xs2;
+    }
+}
+
 struct LaneShare {
     bool east, west;
     template <class T>
@@ -724,6 +783,23 @@ template <class T, template <class> class Sp, class BP>
 DARCY_INLINE void restrict_walk(const LvP<T, Sp>& L, BP bc, Exec e) {
     const int m = (L.n + 1) >> 1;
     restrict_walk_rows(L, bc, 0, m, walk_strip(m, m, e.nthr, kWalkRows / 2), e);
+}
+
+// The red half of a GS sweep over rows [ra, rb) as walks of `strip` rows;
+// every black cell of rows ra - 1 .. rb must be final.
+template <class T, template <class> class Sp>
+DARCY_INLINE void red_walk_rows(const LvP<T, Sp>& L, int ra, int rb, int strip, Exec e) {
+    const int n = L.n, h = (n + 1) >> 1;
+    const int strips = (rb - ra + strip - 1) / strip, total = strips * h;
+    const int lane = threadIdx.x & (kWave - 1);
+    for (int w = e.tid; w < total; w += e.nthr) {
+        const int st = w / h, k = w - st * h, i0 = ra + st * strip, i1 = i0 + strip < rb ? i0 + strip : rb;
+        const LaneShare sh{lane + 1 < kWave && k + 1 < h, lane > 0 && k > 0};
+        if (n & 1)
+            red_walk_item<T, false>(L, k, i0, i1, sh);
+        else
+            red_walk_item<T, true>(L, k, i0, i1, sh);
+    }
 }
 
 // x and the coarse correction x_c of its aggregate, read together; XPlusC
@@ -1076,7 +1152,15 @@ DARCY_INLINE void up_stream(const LvP<T, Sp>& L, PC xc, int step_cells, Exec e) 
         // meet the black rows of step t+1, so no barrier follows.
         const int red_lo = r0 > 0 ? r0 - 1 : 0;
         const int red_hi = black_hi == n ? n : black_hi - 1;
-        update_rows(x, n, 0, red_lo, red_hi, e, [&](int i, int j) { return gs_at(L, i, j, b[i * n + j], xv); });
+        // FP32: the red half as walks (the black rows around a red cell held
+        // in registers); FP64 keeps the per-cell pass, which needs fewer
+        // registers.  Same values either way.
+        if constexpr (sizeof(T) == 4) {
+            if (red_hi > red_lo)
+                red_walk_rows(L, red_lo, red_hi, walk_strip(red_hi - red_lo, m, e.nthr, kWalkRows), e);
+        } else {
+            update_rows(x, n, 0, red_lo, red_hi, e, [&](int i, int j) { return gs_at(L, i, j, b[i * n + j], xv); });
+        }
         if (red_hi == n) break;
     }
     __syncthreads();

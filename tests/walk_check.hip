@@ -211,6 +211,53 @@ int check_down(int n, int step_cells, int nthr, unsigned seed, bool pairs) {
     return bad;
 }
 
+// The red half of the post-smoothing as a walk (red_walk_item): black cells
+// hold final values, red cells are updated from them; items run one after
+// the other (a red cell reads no red cell), strips of several heights.
+template <class T>
+int check_red(int n, int strip, unsigned seed, bool pairs) {
+    std::mt19937 g(seed);
+    std::uniform_real_distribution<double> U(0.1, 12.0), V(-1.0, 1.0);
+    const int h = (n + 1) / 2;
+    T* ex = new T[(size_t)n * (n + 1)];
+    T* ny = new T[(size_t)(n + 1) * n];
+    T* b = new T[(size_t)n * n];
+    T* x = new T[(size_t)n * n];
+    std::vector<T> xr((size_t)n * n);
+    for (size_t i = 0; i < (size_t)n * (n + 1); ++i) ex[i] = (T)U(g), ny[i] = (T)U(g);
+    for (size_t i = 0; i < (size_t)n * n; ++i) b[i] = (T)V(g), x[i] = (T)V(g), xr[i] = x[i];
+    for (int i = 0; i < n; ++i)
+        for (int j = (i & 1); j < n; j += 2) {
+            const int f = i * (n + 1) + j, kk = i * n + j;
+            const T e = ex[f + 1], w = ex[f], nn = ny[(i + 1) * n + j], so = ny[i * n + j];
+            const T inv = inv_from_faces(e, w, nn, so);
+            T v = b[kk];
+            if (j + 1 < n) v += e * x[kk + 1];
+            if (j > 0) v += w * x[kk - 1];
+            if (i + 1 < n) v += nn * x[kk + n];
+            if (i > 0) v += so * x[kk - n];
+            xr[kk] = v * inv;
+        }
+    LvP<T> L{n, ex, ny, nullptr, x, b};
+    for (int i0 = 0; i0 < n; i0 += strip)
+        for (int k = 0; k < h; ++k) {
+            const int i1 = i0 + strip < n ? i0 + strip : n;
+            if ((n & 1) || !pairs)
+                red_walk_item<T, false>(L, k, i0, i1);
+            else
+                red_walk_item<T, true>(L, k, i0, i1);
+        }
+    int bad = 0;
+    for (size_t i = 0; i < (size_t)n * n; ++i) bad += std::memcmp(&x[i], &xr[i], sizeof(T)) != 0;
+    printf("red %s n=%d strip=%d pairs=%d: %s\n", sizeof(T) == 8 ? "f64" : "f32", n, strip, (int)pairs,
+           bad ? "MISMATCH" : "ok");
+    delete[] ex;
+    delete[] ny;
+    delete[] b;
+    delete[] x;
+    return bad;
+}
+
 int main() {
     int bad = 0;
     const int sizes[] = {1, 2, 3, 5, 8, 65, 66, 67, 100, 125, 128, 129, 257, 512};
@@ -234,6 +281,15 @@ int main() {
             for (int p = 0; p < 2; ++p) {
                 bad += check_down<double>(n, st, 256, seed++, p);
                 bad += check_down<float>(n, st, 256, seed++, p);
+            }
+    // the red half of the post-smoothing as walks
+    const int rsizes[] = {1, 2, 3, 4, 5, 8, 17, 18, 33, 64, 65, 100, 128, 129, 257};
+    const int rstrips[] = {1, 2, 3, 8, 16, 1 << 20};
+    for (int n : rsizes)
+        for (int st : rstrips)
+            for (int p = 0; p < 2; ++p) {
+                bad += check_red<double>(n, st, seed++, p);
+                bad += check_red<float>(n, st, seed++, p);
             }
     printf(bad ? "FAILED\n" : "ALL OK\n");
     return bad != 0;
