@@ -713,6 +713,84 @@ DARCY_HD __forceinline__ void red_walk_item(const LvP<T, Sp>& L, int k, int i0, 
     }
 }
 
+// The black half of the first post-smoothing sweep fused with the
+// prolongation (black_with_prolong's values) over rows [i0, i1) of column
+// pair k as a vertical walk: each row's black cell from b, its faces and its
+// four red neighbours read as x + x_c of their aggregates -- the pair's other
+// cell, the rows above and below, and the cell beside the pair (from the
+// neighbouring lane, which forms the same value for its own red cell, else
+// loaded).  Black cells are written, red cells and x_c only read: items need
+// no order among themselves.  gs_at's operations with XPlusC, in its order.
+template <class T, bool kPairs, class Share = NoShare, template <class> class Sp = FlatPtr, class PC = const T*>
+DARCY_HD __forceinline__ void black_prolong_walk_item(const LvP<T, Sp>& L, PC xcc, int k, int i0, int i1,
+                                                      Share sh = Share{}) {
+    const int n = L.n, m = (n + 1) >> 1, c0 = 2 * k;
+    const bool two = c0 + 1 < n, west_on = c0 > 0, east_on = c0 + 2 < n;
+    const int c1 = two ? c0 + 1 : c0;
+    const auto x = L.x;
+    const auto ny = L.ny;
+    auto xpair = [&](int a) { return walk_pair<T, kPairs>(x, n, c0, c1, a); };
+    auto coarse = [&](int a) { return xcc[(a >> 1) * m + k]; };
+    // the red cell beside the black one, prolongated, where no lane shares
+    // it: east of c1 on even rows, west of c0 on odd rows
+    auto side = [&](int a) -> T {
+        if (a & 1) return (!sh.west && west_on) ? T(x[a * n + c0 - 1] + xcc[(a >> 1) * m + k - 1]) : T(0);
+        return (!sh.east && east_on) ? T(x[a * n + c1 + 1] + xcc[(a >> 1) * m + k + 1]) : T(0);
+    };
+    Pair2<T> xp{T(0), T(0)}, xc = xpair(i0), xn{T(0), T(0)};
+    T cp = T(0), ca = coarse(i0), cn = T(0);  // x_c of the aggregates of rows a - 1, a, a + 1
+    if (i0 > 0) {
+        xp = xpair(i0 - 1);
+        cp = coarse(i0 - 1);
+    }
+    if (i0 + 1 < n) {
+        xn = xpair(i0 + 1);
+        cn = coarse(i0 + 1);
+    }
+    WalkRow<T> cur = walk_row<T, kPairs>(L, c0, c1, i0);
+    Pair2<T> north = walk_pair<T, kPairs>(ny, n, c0, c1, i0 + 1);
+    T xs = side(i0);
+    for (int a = i0; a < i1; ++a) {
+        const bool hN = a + 1 < n, hS = a > 0, odd = (a & 1) != 0, more = a + 1 < i1;
+        Pair2<T> xn2 = xn, north2 = north;
+        WalkRow<T> nx = cur;
+        T cn2 = cn, xs2 = T(0);
+        if (more) {
+            nx = walk_row<T, kPairs>(L, c0, c1, a + 1, north);  // north: ny of row a + 1
+            north2 = walk_pair<T, kPairs>(ny, n, c0, c1, a + 2);
+            if (a + 2 < n) {
+                xn2 = xpair(a + 2);
+                cn2 = coarse(a + 2);
+            }
+            xs2 = side(a + 1);
+        }
+        // this pair's red cell on row a, prolongated (c0 on even rows, c1 on
+        // odd ones); every lane forms it and takes part in both shuffles
+        const T pv = (odd ? xc.hi : xc.lo) + ca;
+        const T vw = sh.from_west(pv), ve = sh.from_east(pv);
+        if (!odd) {
+            if (two) {
+                const T XE = sh.east ? ve : xs;
+                x[a * n + c1] = gs_combine(cur.e1, cur.m, north.hi, cur.s1, cur.b1, XE, pv, T(xn.hi + cn),
+                                           T(xp.hi + cp), east_on, true, hN, hS);
+            }
+        } else {
+            const T XW = sh.west ? vw : xs;
+            x[a * n + c0] = gs_combine(cur.m, cur.w0, north.lo, cur.s0, cur.b0, pv, XW, T(xn.lo + cn), T(xp.lo + cp),
+                                       two, west_on, hN, hS);
+        }
+        xp = xc;
+        xc = xn;
+        xn = xn2;
+        cp = ca;
+        ca = cn;
+        cn = cn2;
+        cur = nx;
+        north = north2;
+        xs = xs2;
+    }
+}
+
 struct LaneShare {
     bool east, west;
     template <class T>
@@ -799,6 +877,24 @@ DARCY_INLINE void red_walk_rows(const LvP<T, Sp>& L, int ra, int rb, int strip, 
             red_walk_item<T, false>(L, k, i0, i1, sh);
         else
             red_walk_item<T, true>(L, k, i0, i1, sh);
+    }
+}
+
+// The black half of the first post-smoothing sweep with the prolongation
+// over rows [ra, rb) as walks of `strip` rows (x_c: the coarse correction,
+// m = (n + 1) / 2 per row).
+template <class T, template <class> class Sp, class PC>
+DARCY_INLINE void black_prolong_walk_rows(const LvP<T, Sp>& L, PC xcc, int ra, int rb, int strip, Exec e) {
+    const int n = L.n, h = (n + 1) >> 1;
+    const int strips = (rb - ra + strip - 1) / strip, total = strips * h;
+    const int lane = threadIdx.x & (kWave - 1);
+    for (int w = e.tid; w < total; w += e.nthr) {
+        const int st = w / h, k = w - st * h, i0 = ra + st * strip, i1 = i0 + strip < rb ? i0 + strip : rb;
+        const LaneShare sh{lane + 1 < kWave && k + 1 < h, lane > 0 && k > 0};
+        if (n & 1)
+            black_prolong_walk_item<T, false>(L, xcc, k, i0, i1, sh);
+        else
+            black_prolong_walk_item<T, true>(L, xcc, k, i0, i1, sh);
     }
 }
 
@@ -1144,9 +1240,14 @@ DARCY_INLINE void up_stream(const LvP<T, Sp>& L, PC xc, int step_cells, Exec e) 
     const Prolonged<T, decltype(x), PC> pr{x, xc, n, m};
     for (int r0 = 0;; r0 += R) {
         const int black_hi = r0 + R < n ? r0 + R : n;
-        if (r0 < n)
-            update_rows(x, n, 1, r0, black_hi, e,
-                        [&](int i, int j) { return gs_at(L, i, j, b[i * n + j], pr, XPlusC{}); });
+        if (r0 < n) {
+            // FP32: walks, as for the red half below
+            if constexpr (sizeof(T) == 4)
+                black_prolong_walk_rows(L, xc, r0, black_hi, walk_strip(black_hi - r0, m, e.nthr, kWalkRows), e);
+            else
+                update_rows(x, n, 1, r0, black_hi, e,
+                            [&](int i, int j) { return gs_at(L, i, j, b[i * n + j], pr, XPlusC{}); });
+        }
         __syncthreads();
         // red row i needs black rows i-1 and i+1; red rows of step t never
         // meet the black rows of step t+1, so no barrier follows.

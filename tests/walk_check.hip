@@ -258,6 +258,57 @@ int check_red(int n, int strip, unsigned seed, bool pairs) {
     return bad;
 }
 
+// The black half of the first post-smoothing sweep with the prolongation as
+// a walk (black_prolong_walk_item): red neighbours read as x + x_c of their
+// aggregates, black cells written; items one after the other.
+template <class T>
+int check_black_prolong(int n, int strip, unsigned seed, bool pairs) {
+    std::mt19937 g(seed);
+    std::uniform_real_distribution<double> U(0.1, 12.0), V(-1.0, 1.0);
+    const int h = (n + 1) / 2, m = h;
+    T* ex = new T[(size_t)n * (n + 1)];
+    T* ny = new T[(size_t)(n + 1) * n];
+    T* b = new T[(size_t)n * n];
+    T* x = new T[(size_t)n * n];
+    T* xc = new T[(size_t)m * m];
+    std::vector<T> xr((size_t)n * n);
+    for (size_t i = 0; i < (size_t)n * (n + 1); ++i) ex[i] = (T)U(g), ny[i] = (T)U(g);
+    for (size_t i = 0; i < (size_t)n * n; ++i) b[i] = (T)V(g), x[i] = (T)V(g), xr[i] = x[i];
+    for (size_t i = 0; i < (size_t)m * m; ++i) xc[i] = (T)V(g);
+    auto pv = [&](int i, int j) { return T(x[(size_t)i * n + j] + xc[(size_t)(i >> 1) * m + (j >> 1)]); };
+    for (int i = 0; i < n; ++i)
+        for (int j = 1 - (i & 1); j < n; j += 2) {
+            const int f = i * (n + 1) + j, kk = i * n + j;
+            const T e = ex[f + 1], w = ex[f], nn = ny[(i + 1) * n + j], so = ny[i * n + j];
+            const T inv = inv_from_faces(e, w, nn, so);
+            T v = b[kk];
+            if (j + 1 < n) v += e * pv(i, j + 1);
+            if (j > 0) v += w * pv(i, j - 1);
+            if (i + 1 < n) v += nn * pv(i + 1, j);
+            if (i > 0) v += so * pv(i - 1, j);
+            xr[kk] = v * inv;
+        }
+    LvP<T> L{n, ex, ny, nullptr, x, b};
+    for (int i0 = 0; i0 < n; i0 += strip)
+        for (int k = h - 1; k >= 0; --k) {
+            const int i1 = i0 + strip < n ? i0 + strip : n;
+            if ((n & 1) || !pairs)
+                black_prolong_walk_item<T, false>(L, (const T*)xc, k, i0, i1);
+            else
+                black_prolong_walk_item<T, true>(L, (const T*)xc, k, i0, i1);
+        }
+    int bad = 0;
+    for (size_t i = 0; i < (size_t)n * n; ++i) bad += std::memcmp(&x[i], &xr[i], sizeof(T)) != 0;
+    printf("black+prolong %s n=%d strip=%d pairs=%d: %s\n", sizeof(T) == 8 ? "f64" : "f32", n, strip, (int)pairs,
+           bad ? "MISMATCH" : "ok");
+    delete[] ex;
+    delete[] ny;
+    delete[] b;
+    delete[] x;
+    delete[] xc;
+    return bad;
+}
+
 int main() {
     int bad = 0;
     const int sizes[] = {1, 2, 3, 5, 8, 65, 66, 67, 100, 125, 128, 129, 257, 512};
@@ -290,6 +341,12 @@ int main() {
             for (int p = 0; p < 2; ++p) {
                 bad += check_red<double>(n, st, seed++, p);
                 bad += check_red<float>(n, st, seed++, p);
+            }
+    for (int n : rsizes)
+        for (int st : rstrips)
+            for (int p = 0; p < 2; ++p) {
+                bad += check_black_prolong<double>(n, st, seed++, p);
+                bad += check_black_prolong<float>(n, st, seed++, p);
             }
     printf(bad ? "FAILED\n" : "ALL OK\n");
     return bad != 0;
