@@ -721,7 +721,11 @@ DARCY_HD __forceinline__ void red_walk_item(const LvP<T, Sp>& L, int k, int i0, 
 // neighbouring lane, which forms the same value for its own red cell, else
 // loaded).  Black cells are written, red cells and x_c only read: items need
 // no order among themselves.  gs_at's operations with XPlusC, in its order.
-template <class T, bool kPairs, class Share = NoShare, template <class> class Sp = FlatPtr, class PC = const T*>
+// kProlong false: the black half of a plain GS sweep (red cells final, read
+// as stored; x_c unused), the second colour of a repeated visit's
+// pre-smoothing.
+template <class T, bool kPairs, bool kProlong = true, class Share = NoShare, template <class> class Sp = FlatPtr,
+          class PC = const T*>
 DARCY_HD __forceinline__ void black_prolong_walk_item(const LvP<T, Sp>& L, PC xcc, int k, int i0, int i1,
                                                       Share sh = Share{}) {
     const int n = L.n, m = (n + 1) >> 1, c0 = 2 * k;
@@ -730,12 +734,34 @@ DARCY_HD __forceinline__ void black_prolong_walk_item(const LvP<T, Sp>& L, PC xc
     const auto x = L.x;
     const auto ny = L.ny;
     auto xpair = [&](int a) { return walk_pair<T, kPairs>(x, n, c0, c1, a); };
-    auto coarse = [&](int a) { return xcc[(a >> 1) * m + k]; };
+    auto coarse = [&](int a) -> T {
+        if constexpr (kProlong)
+            return xcc[(a >> 1) * m + k];
+        else
+            return T(0);
+    };
+    // x + x_c with the prolongation, x without it
+    auto plus = [](T v, T c) -> T {
+        if constexpr (kProlong)
+            return T(v + c);
+        else
+            return v;
+    };
     // the red cell beside the black one, prolongated, where no lane shares
     // it: east of c1 on even rows, west of c0 on odd rows
     auto side = [&](int a) -> T {
-        if (a & 1) return (!sh.west && west_on) ? T(x[a * n + c0 - 1] + xcc[(a >> 1) * m + k - 1]) : T(0);
-        return (!sh.east && east_on) ? T(x[a * n + c1 + 1] + xcc[(a >> 1) * m + k + 1]) : T(0);
+        if (a & 1) {
+            if (sh.west || !west_on) return T(0);
+            if constexpr (kProlong)
+                return T(x[a * n + c0 - 1] + xcc[(a >> 1) * m + k - 1]);
+            else
+                return x[a * n + c0 - 1];
+        }
+        if (sh.east || !east_on) return T(0);
+        if constexpr (kProlong)
+            return T(x[a * n + c1 + 1] + xcc[(a >> 1) * m + k + 1]);
+        else
+            return x[a * n + c1 + 1];
     };
     Pair2<T> xp{T(0), T(0)}, xc = xpair(i0), xn{T(0), T(0)};
     T cp = T(0), ca = coarse(i0), cn = T(0);  // x_c of the aggregates of rows a - 1, a, a + 1
@@ -766,18 +792,18 @@ DARCY_HD __forceinline__ void black_prolong_walk_item(const LvP<T, Sp>& L, PC xc
         }
         // this pair's red cell on row a, prolongated (c0 on even rows, c1 on
         // odd ones); every lane forms it and takes part in both shuffles
-        const T pv = (odd ? xc.hi : xc.lo) + ca;
+        const T pv = plus(odd ? xc.hi : xc.lo, ca);
         const T vw = sh.from_west(pv), ve = sh.from_east(pv);
         if (!odd) {
             if (two) {
                 const T XE = sh.east ? ve : xs;
-                x[a * n + c1] = gs_combine(cur.e1, cur.m, north.hi, cur.s1, cur.b1, XE, pv, T(xn.hi + cn),
-                                           T(xp.hi + cp), east_on, true, hN, hS);
+                x[a * n + c1] = gs_combine(cur.e1, cur.m, north.hi, cur.s1, cur.b1, XE, pv, plus(xn.hi, cn),
+                                           plus(xp.hi, cp), east_on, true, hN, hS);
             }
         } else {
             const T XW = sh.west ? vw : xs;
-            x[a * n + c0] = gs_combine(cur.m, cur.w0, north.lo, cur.s0, cur.b0, pv, XW, T(xn.lo + cn), T(xp.lo + cp),
-                                       two, west_on, hN, hS);
+            x[a * n + c0] = gs_combine(cur.m, cur.w0, north.lo, cur.s0, cur.b0, pv, XW, plus(xn.lo, cn),
+                                       plus(xp.lo, cp), two, west_on, hN, hS);
         }
         xp = xc;
         xc = xn;
@@ -882,8 +908,8 @@ DARCY_INLINE void red_walk_rows(const LvP<T, Sp>& L, int ra, int rb, int strip, 
 
 // The black half of the first post-smoothing sweep with the prolongation
 // over rows [ra, rb) as walks of `strip` rows (x_c: the coarse correction,
-// m = (n + 1) / 2 per row).
-template <class T, template <class> class Sp, class PC>
+// m = (n + 1) / 2 per row); kProlong false: of a plain GS sweep (x_c unused).
+template <class T, bool kProlong = true, template <class> class Sp, class PC>
 DARCY_INLINE void black_prolong_walk_rows(const LvP<T, Sp>& L, PC xcc, int ra, int rb, int strip, Exec e) {
     const int n = L.n, h = (n + 1) >> 1;
     const int strips = (rb - ra + strip - 1) / strip, total = strips * h;
@@ -892,9 +918,9 @@ DARCY_INLINE void black_prolong_walk_rows(const LvP<T, Sp>& L, PC xcc, int ra, i
         const int st = w / h, k = w - st * h, i0 = ra + st * strip, i1 = i0 + strip < rb ? i0 + strip : rb;
         const LaneShare sh{lane + 1 < kWave && k + 1 < h, lane > 0 && k > 0};
         if (n & 1)
-            black_prolong_walk_item<T, false>(L, xcc, k, i0, i1, sh);
+            black_prolong_walk_item<T, false, kProlong>(L, xcc, k, i0, i1, sh);
         else
-            black_prolong_walk_item<T, true>(L, xcc, k, i0, i1, sh);
+            black_prolong_walk_item<T, true, kProlong>(L, xcc, k, i0, i1, sh);
     }
 }
 
@@ -1198,6 +1224,8 @@ DARCY_INLINE void down_stream(const LvP<T, Sp>& L, BP bc, bool fresh, int step_c
             if (fresh)  // the red sweep from zero: b * inv_d (each neighbour term is e * 0)
                 update_rows(x, n, 0, r0, red_hi, e,
                             [&](int i, int j) { return b[i * n + j] * inv_diag_at(L, i, j); });
+            else if constexpr (sizeof(T) == 4)  // FP32: walks, as in up_stream
+                red_walk_rows(L, r0, red_hi, walk_strip(red_hi - r0, m, e.nthr, kWalkRows), e);
             else
                 update_rows(x, n, 0, r0, red_hi, e, [&](int i, int j) { return gs_at(L, i, j, b[i * n + j], xv); });
         }
@@ -1205,7 +1233,13 @@ DARCY_INLINE void down_stream(const LvP<T, Sp>& L, BP bc, bool fresh, int step_c
         // black row i needs red rows i-1 and i+1
         const int black_lo = r0 > 0 ? r0 - 1 : 0;
         const int black_hi = red_hi == n ? n : red_hi - 1;
-        update_rows(x, n, 1, black_lo, black_hi, e, [&](int i, int j) { return gs_at(L, i, j, b[i * n + j], xv); });
+        if constexpr (sizeof(T) == 4) {
+            if (black_hi > black_lo)
+                black_prolong_walk_rows<T, false>(L, static_cast<const T*>(nullptr), black_lo, black_hi,
+                                                  walk_strip(black_hi - black_lo, m, e.nthr, kWalkRows), e);
+        } else {
+            update_rows(x, n, 1, black_lo, black_hi, e, [&](int i, int j) { return gs_at(L, i, j, b[i * n + j], xv); });
+        }
         __syncthreads();
         // residual row i needs final rows i-1..i+1; coarse row I needs fine rows 2I, 2I+1
         const int ready = black_hi == n ? n : black_hi - 1;

@@ -261,7 +261,7 @@ int check_red(int n, int strip, unsigned seed, bool pairs) {
 // The black half of the first post-smoothing sweep with the prolongation as
 // a walk (black_prolong_walk_item): red neighbours read as x + x_c of their
 // aggregates, black cells written; items one after the other.
-template <class T>
+template <class T, bool kProlong = true>
 int check_black_prolong(int n, int strip, unsigned seed, bool pairs) {
     std::mt19937 g(seed);
     std::uniform_real_distribution<double> U(0.1, 12.0), V(-1.0, 1.0);
@@ -275,7 +275,12 @@ int check_black_prolong(int n, int strip, unsigned seed, bool pairs) {
     for (size_t i = 0; i < (size_t)n * (n + 1); ++i) ex[i] = (T)U(g), ny[i] = (T)U(g);
     for (size_t i = 0; i < (size_t)n * n; ++i) b[i] = (T)V(g), x[i] = (T)V(g), xr[i] = x[i];
     for (size_t i = 0; i < (size_t)m * m; ++i) xc[i] = (T)V(g);
-    auto pv = [&](int i, int j) { return T(x[(size_t)i * n + j] + xc[(size_t)(i >> 1) * m + (j >> 1)]); };
+    auto pv = [&](int i, int j) {
+        if constexpr (kProlong)
+            return T(x[(size_t)i * n + j] + xc[(size_t)(i >> 1) * m + (j >> 1)]);
+        else
+            return x[(size_t)i * n + j];
+    };
     for (int i = 0; i < n; ++i)
         for (int j = 1 - (i & 1); j < n; j += 2) {
             const int f = i * (n + 1) + j, kk = i * n + j;
@@ -293,13 +298,14 @@ int check_black_prolong(int n, int strip, unsigned seed, bool pairs) {
         for (int k = h - 1; k >= 0; --k) {
             const int i1 = i0 + strip < n ? i0 + strip : n;
             if ((n & 1) || !pairs)
-                black_prolong_walk_item<T, false>(L, (const T*)xc, k, i0, i1);
+                black_prolong_walk_item<T, false, kProlong>(L, (const T*)xc, k, i0, i1);
             else
-                black_prolong_walk_item<T, true>(L, (const T*)xc, k, i0, i1);
+                black_prolong_walk_item<T, true, kProlong>(L, (const T*)xc, k, i0, i1);
         }
     int bad = 0;
     for (size_t i = 0; i < (size_t)n * n; ++i) bad += std::memcmp(&x[i], &xr[i], sizeof(T)) != 0;
-    printf("black+prolong %s n=%d strip=%d pairs=%d: %s\n", sizeof(T) == 8 ? "f64" : "f32", n, strip, (int)pairs,
+    printf("black%s %s n=%d strip=%d pairs=%d: %s\n", kProlong ? "+prolong" : "", sizeof(T) == 8 ? "f64" : "f32", n,
+           strip, (int)pairs,
            bad ? "MISMATCH" : "ok");
     delete[] ex;
     delete[] ny;
@@ -347,6 +353,8 @@ int main() {
             for (int p = 0; p < 2; ++p) {
                 bad += check_black_prolong<double>(n, st, seed++, p);
                 bad += check_black_prolong<float>(n, st, seed++, p);
+                bad += check_black_prolong<double, false>(n, st, seed++, p);
+                bad += check_black_prolong<float, false>(n, st, seed++, p);
             }
     printf(bad ? "FAILED\n" : "ALL OK\n");
     return bad != 0;
