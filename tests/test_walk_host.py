@@ -1,8 +1,9 @@
 """Host build of the vertical walks (pre-smoothing from zero and residual +
 restriction, darcy-cpp_amd/csrc/device.hpp) against plain loops, bit for
 bit, under AddressSanitizer (tests/walk_check.hip): whole levels as the grid
-path runs them, and the persistent kernel's row-pipelined fresh down visit
-step by step."""
+path runs them, the persistent kernel's row-pipelined fresh down visit step
+by step, and the FP32 cycle's single-colour walks (red from final black
+cells; black from final red cells, with and without the prolongation)."""
 
 import os
 import shutil
@@ -25,4 +26,4 @@ def test_walks_match_plain_loops(tmp_path):
     assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
     lines = out.stdout.strip().splitlines()
     assert lines[-1] == "ALL OK"
-    assert sum(1 for l in lines if l.endswith(": ok")) == 800
+    assert sum(1 for l in lines if l.endswith(": ok")) == 1880

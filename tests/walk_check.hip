@@ -1,5 +1,6 @@
-// Host build of the vertical walks (presmooth_walk_item, restrict_walk_item
-// in device.hpp), checked bit for bit against plain loops:
+// Host build of the vertical walks (presmooth_walk_item, restrict_walk_item,
+// red_walk_item, black_prolong_walk_item in device.hpp), checked bit for bit
+// against plain loops:
 // the red sweep from zero then the black sweep with the stored inv_diag
 // (level.hpp:145-161), and the residual summed over each 2x2 aggregate
 // (level.hpp:122-137, :214-227).  Every size from 1 cell up, odd and even
