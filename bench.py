@@ -44,7 +44,7 @@ BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 matrix peak
 # committed weights): rocprofv3 FETCH_SIZE (x 2) + WRITE_SIZE (x 1), the
 # factors that the same counters give for the library's copy kernel of 1 GiB.
 # Reported for that configuration only.
-TRAFFIC_DEFAULT_BYTES = 406.5e9
+TRAFFIC_DEFAULT_BYTES = 406.4e9
 SEED0 = 20000
 
 # Algorithmic bytes per fine cell (counting rule: each array a kernel must
