@@ -655,8 +655,9 @@ DARCY_HD __forceinline__ void restrict_walk_item(const LvP<T, Sp>& L, BP bc, int
 }
 
 // The red half of a GS sweep over rows [i0, i1) of column pair k as a
-// vertical walk, with every black cell final (the second colour of the
-// post-smoothing, level.hpp:145-161): each row's red cell from b, its faces
+// vertical walk (level.hpp:145-161), the black cells holding the values the
+// sweep reads (final for the post-smoothing's second colour, as stored for a
+// repeated visit's first): each row's red cell from b, its faces
 // and its four black neighbours -- the pair's other cell, the rows above and
 // below (held in registers as the walk goes down), and the cell beside the
 // pair (from the neighbouring lane through `sh`, else loaded).  A red cell
@@ -890,7 +891,7 @@ DARCY_INLINE void restrict_walk(const LvP<T, Sp>& L, BP bc, Exec e) {
 }
 
 // The red half of a GS sweep over rows [ra, rb) as walks of `strip` rows;
-// every black cell of rows ra - 1 .. rb must be final.
+// the black cells of rows ra - 1 .. rb must hold the values the sweep reads.
 template <class T, template <class> class Sp>
 DARCY_INLINE void red_walk_rows(const LvP<T, Sp>& L, int ra, int rb, int strip, Exec e) {
     const int n = L.n, h = (n + 1) >> 1;
